@@ -1,0 +1,252 @@
+"""ctypes bindings for the CPU oracle (oracle/liborc.so) and, when built, the
+reference's own CPU code (oracle/_ref/libref.so).
+
+Test infrastructure: only tests, the smoke check and the bench's CPU-baseline
+leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIBORC = os.path.join(ORACLE_DIR, "liborc.so")
+LIBREF = os.path.join(ORACLE_DIR, "_ref", "libref.so")
+
+CURVE_BN254 = 1
+CURVE_BLS12_381 = 2
+FIELD_BLS_FP, FIELD_BLS_FR, FIELD_BN_FP, FIELD_BN_FR = 0, 1, 2, 3
+
+POINT_BYTES = {CURVE_BN254: 64, CURVE_BLS12_381: 96}
+COMP_BYTES = {CURVE_BN254: 32, CURVE_BLS12_381: 48}
+
+_u8p = ctypes.c_char_p
+_sz = ctypes.c_size_t
+
+_orc = None
+_ref = None
+
+
+def _build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liborc.so"], check=True)
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(LIBORC):
+            _build_oracle()
+        lib = ctypes.CDLL(LIBORC)
+        lib.orc_splitmix64.restype = ctypes.c_uint64
+        lib.orc_splitmix64.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        lib.orc_gen_scalars.argtypes = [ctypes.c_int, ctypes.c_uint64, _sz, _u8p]
+        lib.orc_msm.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
+        lib.orc_msm_naive.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
+        lib.orc_msm_mt.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p, ctypes.c_int]
+        lib.orc_ntt.argtypes = [ctypes.c_int, _u8p, _sz, ctypes.c_int]
+        lib.orc_dft_naive.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, ctypes.c_int]
+        lib.orc_root_of_unity.argtypes = [ctypes.c_int, _sz, _u8p]
+        lib.orc_kzg_commit.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
+        lib.orc_kzg_commit_closed_form.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
+        lib.orc_powers.argtypes = [ctypes.c_int, _u8p, _sz, _u8p]
+        lib.orc_gen_points.argtypes = [ctypes.c_int, _u8p, _sz, _u8p]
+        lib.orc_field_mont_consts.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.POINTER(ctypes.c_uint64)]
+        _orc = lib
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(LIBREF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(LIBREF)
+        _ref.gpukit_multi_pippenger_cpu.argtypes = [ctypes.c_uint32, _u8p, _u8p, _sz, _u8p]
+    return _ref
+
+
+def _buf(n):
+    return ctypes.create_string_buffer(n)
+
+
+# ---------------------------------------------------------------- oracle API
+def field_op(field: int, op: int, a: bytes, b: bytes | None = None) -> bytes | None:
+    nb = orc().orc_field_nbytes(field)
+    out = _buf(nb)
+    rc = orc().orc_field_op(field, op, a, b, out)
+    return None if rc else out.raw
+
+
+def field_modulus(field: int) -> int:
+    nb = orc().orc_field_nbytes(field)
+    out = _buf(nb)
+    orc().orc_field_modulus(field, out)
+    return int.from_bytes(out.raw, "little")
+
+
+def mont_consts(field: int):
+    nb = orc().orc_field_nbytes(field)
+    r1, r2, pinv = _buf(nb), _buf(nb), ctypes.c_uint64()
+    orc().orc_field_mont_consts(field, r1, r2, ctypes.byref(pinv))
+    return int.from_bytes(r1.raw, "little"), int.from_bytes(r2.raw, "little"), pinv.value
+
+
+def gen_scalars(curve: int, seed: int, n: int) -> bytes:
+    out = _buf(32 * n)
+    orc().orc_gen_scalars(curve, seed, n, out)
+    return out.raw
+
+
+def generator(curve: int) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_g1_generator(curve, out)
+    return out.raw
+
+
+def g1_mul_gen(curve: int, k: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_g1_mul_gen(curve, k, out)
+    return out.raw
+
+
+def g1_mul(curve: int, p: bytes, k: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_g1_mul(curve, p, k, out)
+    return out.raw
+
+
+def g1_add(curve: int, a: bytes, b: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_g1_add(curve, a, b, out)
+    return out.raw
+
+
+def g1_dbl(curve: int, a: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_g1_dbl(curve, a, out)
+    return out.raw
+
+
+def on_curve(curve: int, p: bytes) -> bool:
+    return bool(orc().orc_g1_on_curve(curve, p))
+
+
+def compress(curve: int, p: bytes) -> bytes:
+    out = _buf(COMP_BYTES[curve])
+    orc().orc_g1_compress(curve, p, out)
+    return out.raw
+
+
+def decompress(curve: int, c: bytes) -> bytes | None:
+    out = _buf(POINT_BYTES[curve])
+    rc = orc().orc_g1_decompress(curve, c, out)
+    return None if rc else out.raw
+
+
+def gen_points(curve: int, ks: bytes) -> bytes:
+    n = len(ks) // 32
+    out = _buf(POINT_BYTES[curve] * n)
+    orc().orc_gen_points(curve, ks, n, out)
+    return out.raw
+
+
+def msm(curve: int, scalars: bytes, points: bytes, threads: int = 1) -> bytes:
+    n = len(scalars) // 32
+    out = _buf(POINT_BYTES[curve])
+    if threads > 1:
+        orc().orc_msm_mt(curve, scalars, points, n, out, threads)
+    else:
+        orc().orc_msm(curve, scalars, points, n, out)
+    return out.raw
+
+
+def msm_naive(curve: int, scalars: bytes, points: bytes) -> bytes:
+    n = len(scalars) // 32
+    out = _buf(POINT_BYTES[curve])
+    orc().orc_msm_naive(curve, scalars, points, n, out)
+    return out.raw
+
+
+def ntt(fr: int, elems: bytes, inverse: bool = False) -> bytes | None:
+    n = len(elems) // 32
+    buf = ctypes.create_string_buffer(elems, len(elems))
+    rc = orc().orc_ntt(fr, buf, n, 1 if inverse else 0)
+    return None if rc else buf.raw[: len(elems)]
+
+
+def dft_naive(fr: int, elems: bytes, inverse: bool = False) -> bytes | None:
+    n = len(elems) // 32
+    out = _buf(len(elems))
+    rc = orc().orc_dft_naive(fr, elems, out, n, 1 if inverse else 0)
+    return None if rc else out.raw
+
+
+def root_of_unity(fr: int, n: int) -> int:
+    out = _buf(32)
+    if orc().orc_root_of_unity(fr, n, out):
+        raise ValueError(n)
+    return int.from_bytes(out.raw, "little")
+
+
+def powers(curve: int, tau: bytes, n: int) -> bytes:
+    out = _buf(32 * n)
+    orc().orc_powers(curve, tau, n, out)
+    return out.raw
+
+
+def kzg_commit(curve: int, srs: bytes, evals: bytes) -> bytes | None:
+    n = len(evals) // 32
+    out = _buf(COMP_BYTES[curve])
+    rc = orc().orc_kzg_commit(curve, srs, evals, n, out)
+    return None if rc else out.raw
+
+
+def kzg_commit_closed_form(curve: int, tau: bytes, evals: bytes) -> bytes | None:
+    n = len(evals) // 32
+    out = _buf(COMP_BYTES[curve])
+    rc = orc().orc_kzg_commit_closed_form(curve, tau, evals, n, out)
+    return None if rc else out.raw
+
+
+# ------------------------------------------------------------- reference API
+def ref_msm(curve: int, scalars: bytes, points: bytes) -> bytes:
+    n = len(scalars) // 32
+    out = _buf(POINT_BYTES[curve])
+    rc = ref().gpukit_multi_pippenger_cpu(curve, scalars, points, n, out)
+    assert rc == 0, rc
+    return out.raw
+
+
+def ref_field_mul(field: int, a: bytes, b: bytes) -> bytes:
+    nb = 48 if field == FIELD_BLS_FP else 32
+    out = _buf(nb)
+    ref().ref_field_mul(field, a, b, out)
+    return out.raw
+
+
+def ref_g1_add(curve: int, a: bytes, b: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    ref().ref_g1_add(curve, a, b, out)
+    return out.raw
+
+
+def ref_g1_dbl(curve: int, a: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    ref().ref_g1_dbl(curve, a, out)
+    return out.raw
+
+
+def ref_g1_mul(curve: int, p: bytes, k: bytes) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    ref().ref_g1_mul(curve, p, k, out)
+    return out.raw
+
+
+def ref_generator(curve: int) -> bytes:
+    out = _buf(POINT_BYTES[curve])
+    ref().ref_g1_generator(curve, out)
+    return out.raw

@@ -1,0 +1,322 @@
+"""Pins the CPU oracle: against the reference's fixtures (tests/golden/), against
+textbook arithmetic on Python integers, and against closed forms.
+
+Each check is chosen so that a plausible slip in the oracle (a wrong constant,
+a dropped carry, a transposed operand, a wrong root or bit order) fails it.
+"""
+import hashlib
+import json
+import os
+import random
+
+import pytest
+
+import oracle_lib as O
+import pymath as M
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------- fields
+# Montgomery constants as printed in the reference:
+#   BLS12-381 Fp  bls/cpp/bls12_381_fp.hpp:88-98
+#   BLS12-381 Fr  kzg/cpp/kzg_oracle.hpp:74-90
+#   BN254 Fp/Fr   bn254/cpp/bn254_fp.hpp:83-97
+def _limbs(*ls):
+    return sum(v << (64 * i) for i, v in enumerate(ls))
+
+
+REF_CONSTS = {
+    O.FIELD_BLS_FP: (
+        _limbs(0x760900000002FFFD, 0xEBF4000BC40C0002, 0x5F48985753C758BA, 0x77CE585370525745,
+               0x5C071A97A256EC6D, 0x15F65EC3FA80E493),
+        _limbs(0xF4DF1F341C341746, 0x0A76E6A609D104F1, 0x8DE5476C4C95B6D5, 0x67EB88A9939D83C0,
+               0x9A793E85B519952D, 0x11988FE592CAE3AA),
+        0x89F3FFFCFFFCFFFD),
+    O.FIELD_BLS_FR: (
+        _limbs(0x00000001FFFFFFFE, 0x5884B7FA00034802, 0x998C4FEFECBC4FF5, 0x1824B159ACC5056F),
+        _limbs(0xC999E990F3F29C6D, 0x2B6CEDCB87925C23, 0x05D314967254398F, 0x0748D9D99F59FF11),
+        0xFFFFFFFEFFFFFFFF),
+    O.FIELD_BN_FP: (
+        _limbs(0xD35D438DC58F0D9D, 0x0A78EB28F5C70B3D, 0x666EA36F7879462C, 0x0E0A77C19A07DF2F),
+        _limbs(0xF32CFC5B538AFA89, 0xB5E71911D44501FB, 0x47AB1EFF0A417FF6, 0x06D89F71CAB8351F),
+        0x87D20782E4866389),
+    O.FIELD_BN_FR: (
+        _limbs(0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E, 0x0E0A77C19A07DF2F),
+        _limbs(0x1BB8E645AE216DA7, 0x53FE3AB1E35C59E3, 0x8C49833D53BB8085, 0x0216D0B17F4E44A5),
+        0xC2E1F593EFFFFFFF),
+}
+
+MODULI = {O.FIELD_BLS_FP: M.BLS_P, O.FIELD_BLS_FR: M.BLS_R, O.FIELD_BN_FP: M.BN_P, O.FIELD_BN_FR: M.BN_R}
+
+
+@pytest.mark.parametrize("field", sorted(REF_CONSTS))
+def test_montgomery_constants_match_reference(field):
+    assert O.field_modulus(field) == MODULI[field]
+    assert O.mont_consts(field) == REF_CONSTS[field]
+
+
+@pytest.mark.parametrize("field", sorted(MODULI))
+def test_field_ops_against_python_ints(field):
+    p = MODULI[field]
+    nb = 48 if field == O.FIELD_BLS_FP else 32
+    rnd = random.Random(field)
+    vals = [0, 1, p - 1, p - 2, (p - 1) // 2] + [rnd.randrange(p) for _ in range(40)]
+    for a in vals:
+        b = rnd.randrange(p)
+        A, B = a.to_bytes(nb, "little"), b.to_bytes(nb, "little")
+        g = lambda op: int.from_bytes(O.field_op(field, op, A, B), "little")  # noqa: E731
+        assert g(0) == (a + b) % p
+        assert g(1) == (a - b) % p
+        assert g(2) == a * b % p
+        if a:
+            assert g(3) == pow(a, p - 2, p)
+
+
+def test_fr_kat_gnark_vectors():
+    for v in load("fr_bls_kat.json"):
+        op = 0 if v["op"] == "add" else 2
+        got = O.field_op(O.FIELD_BLS_FR, op, bytes.fromhex(v["a"]), bytes.fromhex(v["b"]))
+        assert got.hex() == v["r"], v["name"]
+
+
+def test_field_golden_reference_products():
+    for v in load("field_golden.json"):
+        got = O.field_op(v["field"], 2, bytes.fromhex(v["a"]), bytes.fromhex(v["b"]))
+        assert got.hex() == v["mul"]
+
+
+# ---------------------------------------------------------------- G1
+@pytest.mark.parametrize("curve", [1, 2])
+def test_generator_and_order(curve):
+    g = O.generator(curve)
+    assert M.from_wire(curve, g) == M.CURVES[curve]["g"]
+    assert O.on_curve(curve, g)
+    r = M.CURVES[curve]["r"]
+    assert O.g1_mul_gen(curve, r.to_bytes(32, "little")) == bytes(len(g))
+    assert O.g1_mul_gen(curve, (r - 1).to_bytes(32, "little")) == M.to_wire(curve, M.neg(curve, M.CURVES[curve]["g"]))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_g1_against_affine_python(curve):
+    rnd = random.Random(curve * 7)
+    G = M.CURVES[curve]["g"]
+    for _ in range(4):
+        k = rnd.getrandbits(256)
+        P = M.mul(curve, G, k)
+        assert O.g1_mul_gen(curve, k.to_bytes(32, "little")) == M.to_wire(curve, P)
+        Q = M.mul(curve, G, rnd.getrandbits(100))
+        assert O.g1_add(curve, M.to_wire(curve, P), M.to_wire(curve, Q)) == M.to_wire(curve, M.add(curve, P, Q))
+        assert O.g1_dbl(curve, M.to_wire(curve, P)) == M.to_wire(curve, M.add(curve, P, P))
+        assert O.g1_add(curve, M.to_wire(curve, P), M.to_wire(curve, M.neg(curve, P))) == bytes(2 * M.CURVES[curve]["fb"])
+
+
+def test_bn254_eip196_vectors():
+    v = load("bn254_eip196.json")
+    g = bytes.fromhex(v["G_x"] + v["G_y"])
+    neg = bytes.fromhex(v["G_x"] + v["negG_y"])
+    two = bytes.fromhex(v["twoG_x"] + v["twoG_y"])
+    assert O.g1_add(1, g, bytes(64)) == g
+    assert O.g1_add(1, g, neg) == bytes(64)
+    assert O.g1_add(1, g, g) == two
+    assert O.g1_mul_gen(1, (2).to_bytes(32, "little")) == two
+    assert O.g1_mul_gen(1, (1).to_bytes(32, "little")) == g
+
+
+def test_g1_golden_reference_ops():
+    for v in load("g1_golden.json"):
+        c = v["curve"]
+        assert O.g1_mul_gen(c, bytes.fromhex(v["k"])).hex() == v["kG"]
+        assert O.g1_add(c, bytes.fromhex(v["kG"]), bytes.fromhex(v["k2G"])).hex() == v["sum"]
+        assert O.g1_dbl(c, bytes.fromhex(v["kG"])).hex() == v["dbl"]
+
+
+def test_bls_compressed_fixtures():
+    v = load("bls_compressed.json")
+    assert O.compress(2, bytes(96)).hex() == v["infinity"]
+    assert O.compress(2, O.generator(2)).hex() == v["generator"]
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_compression_roundtrip_and_sign_rule(curve):
+    rnd = random.Random(99 + curve)
+    G = M.CURVES[curve]["g"]
+    for _ in range(6):
+        P = M.mul(curve, G, rnd.getrandbits(128))
+        for Q in (P, M.neg(curve, P)):
+            w = M.to_wire(curve, Q)
+            c = O.compress(curve, w)
+            assert c == M.compress(curve, Q)
+            assert O.decompress(curve, c) == w
+    inf = O.compress(curve, bytes(2 * M.CURVES[curve]["fb"]))
+    assert O.decompress(curve, inf) == bytes(2 * M.CURVES[curve]["fb"])
+
+
+# ---------------------------------------------------------------- MSM
+def _golden_inputs(case):
+    curve, n = case["curve"], case["n"]
+    if "scalars" in case:
+        return bytes.fromhex(case["scalars"]), bytes.fromhex(case["points"])
+    scalars = O.gen_scalars(curve, case["seed"], n)
+    ks = O.gen_scalars(curve, case["seed"] ^ 0x5EED, n)
+    assert hashlib.sha256(scalars).hexdigest() == case["sha256_scalars"]
+    points = O.gen_points(curve, ks)
+    assert hashlib.sha256(points).hexdigest() == case["sha256_points"]
+    return scalars, points
+
+
+MSM_CASES = load("msm_golden.json")
+
+
+@pytest.mark.parametrize("case", [c for c in MSM_CASES if c["n"] <= 1024],
+                         ids=lambda c: f"c{c['curve']}-n{c['n']}-{c.get('name', 'seeded')}")
+def test_msm_golden(case):
+    scalars, points = _golden_inputs(case)
+    assert O.msm(case["curve"], scalars, points).hex() == case["expected"]
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_msm_closed_form(curve):
+    """P_i = k_i G  =>  sum s_i P_i = (sum s_i k_i mod r) G."""
+    r = M.CURVES[curve]["r"]
+    n = 300
+    scalars = O.gen_scalars(curve, 5 + curve, n)
+    ks = O.gen_scalars(curve, 6 + curve, n)
+    points = O.gen_points(curve, ks)
+    t = sum(int.from_bytes(scalars[32 * i:32 * i + 32], "little") * int.from_bytes(ks[32 * i:32 * i + 32], "little")
+            for i in range(n)) % r
+    assert O.msm(curve, scalars, points) == O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_msm_top_carry_beyond_reference(curve):
+    """Scalars >= 2^255 with c = 8 (n in 692..2217) need the carry out of the last
+    window, which the reference drops; the oracle keeps it and must equal the
+    term-by-term sum."""
+    n = 700
+    assert O.orc().orc_best_c(n) == 8
+    rnd = random.Random(3)
+    scalars = b"".join(((0xFF << 248) | rnd.getrandbits(248)).to_bytes(32, "little") for _ in range(n))
+    ks = O.gen_scalars(curve, 11, n)
+    points = O.gen_points(curve, ks)
+    r = M.CURVES[curve]["r"]
+    t = sum(int.from_bytes(scalars[32 * i:32 * i + 32], "little") * int.from_bytes(ks[32 * i:32 * i + 32], "little")
+            for i in range(n)) % r
+    assert O.msm(curve, scalars, points) == O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+
+
+def test_msm_empty_is_identity():
+    for curve in (1, 2):
+        assert O.msm(curve, b"", b"") == bytes(O.POINT_BYTES[curve])
+
+
+def test_best_c_table():
+    # multi_pippenger.cpp:67-88 cost model; values quoted in the survey's row a5
+    bc = O.orc().orc_best_c
+    assert [bc(1 << k) for k in (12, 16, 20, 22)] == [10, 12, 12, 12]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference library not built")
+@pytest.mark.parametrize("curve", [1, 2])
+def test_msm_against_reference_library(curve):
+    for n in (3, 33, 700):
+        scalars = O.gen_scalars(curve, 40 + n, n)
+        points = O.gen_points(curve, O.gen_scalars(curve, 41 + n, n))
+        assert O.msm(curve, scalars, points) == O.ref_msm(curve, scalars, points)
+
+
+# ---------------------------------------------------------------- NTT
+FRS = {0: M.BLS_R, 1: M.BN_R}
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+def test_roots_of_unity(fr):
+    r = FRS[fr]
+    adic = 32 if fr == 0 else 28
+    gen = 7 if fr == 0 else 5
+    assert (r - 1) % (1 << adic) == 0 and ((r - 1) >> adic) & 1
+    for lg in (1, 2, 10, adic):
+        n = 1 << lg
+        w = O.root_of_unity(fr, n)
+        assert w == pow(gen, (r - 1) // n, r)
+        assert pow(w, n, r) == 1 and pow(w, n // 2, r) == r - 1
+
+
+def _rand_fr(fr, n, seed):
+    rnd = random.Random(seed)
+    return [rnd.randrange(FRS[fr]) for _ in range(n)]
+
+
+def _enc(vals):
+    return b"".join(v.to_bytes(32, "little") for v in vals)
+
+
+def _dec(b):
+    return [int.from_bytes(b[i:i + 32], "little") for i in range(0, len(b), 32)]
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 64])
+def test_ntt_against_python_dft(fr, n):
+    r = FRS[fr]
+    vals = _rand_fr(fr, n, n + fr)
+    w = O.root_of_unity(fr, n) if n > 1 else 1
+    want = M.dft(vals, w, r)
+    assert _dec(O.ntt(fr, _enc(vals))) == want
+    winv = pow(w, r - 2, r)
+    ninv = pow(n, r - 2, r)
+    assert _dec(O.ntt(fr, _enc(want), inverse=True)) == vals
+    assert _dec(O.ntt(fr, _enc(vals), inverse=True)) == [v * ninv % r for v in M.dft(vals, winv, r)]
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+def test_ntt_structure(fr):
+    r = FRS[fr]
+    n = 256
+    vals = _rand_fr(fr, n, 5)
+    assert O.ntt(fr, _enc(vals)) == O.dft_naive(fr, _enc(vals))
+    delta = [1] + [0] * (n - 1)
+    assert _dec(O.ntt(fr, _enc(delta))) == [1] * n
+    assert _dec(O.ntt(fr, _enc([1] * n))) == [n] + [0] * (n - 1)
+    # convolution theorem against a schoolbook product mod X^n - 1
+    a, b = _rand_fr(fr, n, 6), _rand_fr(fr, n, 7)
+    conv = [0] * n
+    for i in range(n):
+        for j in range(n):
+            conv[(i + j) % n] = (conv[(i + j) % n] + a[i] * b[j]) % r
+    fa, fb = _dec(O.ntt(fr, _enc(a))), _dec(O.ntt(fr, _enc(b)))
+    assert _dec(O.ntt(fr, _enc([x * y % r for x, y in zip(fa, fb)]), inverse=True)) == conv
+    big = _rand_fr(fr, 1 << 12, 8)
+    assert _dec(O.ntt(fr, O.ntt(fr, _enc(big)), inverse=True)) == big
+
+
+def test_ntt_rejects_bad_input():
+    assert O.ntt(0, bytes(32 * 3)) is None                     # not a power of two
+    assert O.ntt(0, M.BLS_R.to_bytes(32, "little") * 4) is None  # non-canonical
+
+
+# ---------------------------------------------------------------- KZG
+@pytest.mark.parametrize("curve", [1, 2])
+def test_kzg_commit_closed_form(curve):
+    n = 64
+    tau = O.gen_scalars(curve, 123, 1)
+    srs = O.gen_points(curve, O.powers(curve, tau, n))
+    evals = O.gen_scalars(curve, 321, n)
+    c1 = O.kzg_commit(curve, srs, evals)
+    c2 = O.kzg_commit_closed_form(curve, tau, evals)
+    assert c1 == c2
+    # f(tau) computed independently: Lagrange form sum e_j L_j(tau)
+    r = M.CURVES[curve]["r"]
+    t = int.from_bytes(tau, "little")
+    fr = 0 if curve == 2 else 1
+    w = O.root_of_unity(fr, n)
+    e = _dec(evals)
+    # L_j(t) = w^j (t^n - 1) / (n (t - w^j))
+    ft = sum(e[j] * pow(w, j, r) * (pow(t, n, r) - 1) * pow(n * (t - pow(w, j, r)), r - 2, r) for j in range(n)) % r
+    P = M.mul(curve, M.CURVES[curve]["g"], ft)
+    assert c1 == M.compress(curve, P)
