@@ -1,0 +1,156 @@
+// C ABI of the MSM path.
+//
+//   gpukit_multi_pippenger      drop-in for gpukit/c-abi/c_multi_pippenger.cpp:9-42
+//   gpukit_multi_pippenger_hip  new HIP backend next to the reference's
+//                               _cpu/_cuda/_metal/_wgsl entries
+//                               (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92)
+//   kinet_hip_*                 device-pointer variants for callers whose data
+//                               already lives in HBM (bench, batching layers)
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "engine.h"
+#include "kinet/gpukit/multi_pippenger.h"
+#include "kinet/crypto/hip_runtime.h"
+
+namespace kz {
+
+static std::mutex g_err_mu;
+static std::string g_last_error;
+
+void set_last_error(const std::string& msg) {
+    std::lock_guard<std::mutex> g(g_err_mu);
+    g_last_error = msg;
+}
+
+Engine& engine() {
+    static Engine* e = [] {
+        int count = 0;
+        hipError_t rc = hipGetDeviceCount(&count);
+        if (rc != hipSuccess || count == 0) throw HipError(hipErrorNoDevice, "no HIP device available");
+        Engine* eng = new Engine();
+        KZ_HIP(hipGetDevice(&eng->device));
+        KZ_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+        return eng;
+    }();
+    return *e;
+}
+
+static bool valid_curve(uint32_t curve) { return curve == 1 || curve == 2; }
+
+}  // namespace kz
+
+using namespace kz;
+
+extern "C" {
+
+const char* kinet_hip_last_error(void) {
+    std::lock_guard<std::mutex> g(g_err_mu);
+    return g_last_error.c_str();
+}
+
+int kinet_hip_available(void) {
+    try {
+        (void)engine();
+        return 1;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return 0;
+    }
+}
+
+int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
+                               uint8_t* result) {
+    if (!result) return GPUKIT_ERR_NULL_ARG;
+    if (n > 0 && (!scalars || !points)) return GPUKIT_ERR_NULL_ARG;
+    if (curve == GPUKIT_CURVE_SECP256K1 || curve == GPUKIT_CURVE_BANDERWAGON) return GPUKIT_ERR_NOTIMPL;
+    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    const size_t pb = point_wire_bytes((int)curve);
+    if (n == 0) {
+        std::memset(result, 0, pb);
+        return GPUKIT_OK;
+    }
+    if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        hipStream_t s = e.stream;
+        uint8_t* d_sc = static_cast<uint8_t*>(e.h2d_a.ensure(32 * n));
+        uint8_t* d_wire = static_cast<uint8_t*>(e.h2d_b.ensure(pb * n));
+        uint32_t* d_pts = static_cast<uint32_t*>(e.pts.ensure(2 * fp_words((int)curve) * 4 * n));
+        uint8_t* d_inf = static_cast<uint8_t*>(e.inf.ensure(n));
+        uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes((int)curve)));
+        KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));
+        KZ_HIP(hipMemcpyAsync(d_wire, points, pb * n, hipMemcpyHostToDevice, s));
+        points_from_wire((int)curve, d_wire, d_pts, d_inf, n, s);
+        MsmParams prm;
+        msm_device((int)curve, d_pts, d_inf, reinterpret_cast<const uint32_t*>(d_sc), n, prm, e.ws, s, d_out);
+        KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
+        KZ_HIP(hipStreamSynchronize(s));
+        return GPUKIT_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
+// Dispatcher: this library carries only the HIP backend.  GPUKIT_BACKEND unset,
+// empty or "hip" selects it; any other value names a backend this library
+// does not have and is reported as GPUKIT_ERR_BACKEND (no silent CPU route).
+int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
+                           uint8_t* result) {
+    if (!result) return GPUKIT_ERR_NULL_ARG;
+    if (n > 0 && (!scalars || !points)) return GPUKIT_ERR_NULL_ARG;
+    const char* env = std::getenv("GPUKIT_BACKEND");
+    if (env && *env && std::strcmp(env, "hip") != 0) {
+        set_last_error(std::string("GPUKIT_BACKEND=") + env + " is not provided by the HIP library");
+        return GPUKIT_ERR_BACKEND;
+    }
+    return gpukit_multi_pippenger_hip(curve, scalars, points, n, result);
+}
+
+int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
+                             void* stream) {
+    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (n > 0 && (!d_wire || !d_pts || !d_inf)) return GPUKIT_ERR_NULL_ARG;
+    try {
+        (void)engine();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        points_from_wire((int)curve, static_cast<const uint8_t*>(d_wire), static_cast<uint32_t*>(d_pts),
+                         static_cast<uint8_t*>(d_inf), n, s);
+        return GPUKIT_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
+int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                           size_t n, int window_bits, void* d_out, void* stream) {
+    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
+    if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
+    if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (n == 0) {
+            KZ_HIP(hipMemsetAsync(d_out, 0, msm_out_bytes((int)curve), s));
+            uint8_t* d_zero = static_cast<uint8_t*>(d_out);
+            compress_on_device((int)curve, d_zero, d_zero + point_wire_bytes((int)curve), s);
+            return GPUKIT_OK;
+        }
+        MsmParams prm;
+        prm.c = window_bits;
+        msm_device((int)curve, static_cast<const uint32_t*>(d_pts), static_cast<const uint8_t*>(d_inf),
+                   static_cast<const uint32_t*>(d_scalars), n, prm, e.ws, s, static_cast<uint8_t*>(d_out));
+        return GPUKIT_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+// Device G1 arithmetic for short-Weierstrass curves with a = 0 (BLS12-381 G1,
+// BN254 G1), in XYZZ coordinates: x = X/ZZ, y = Y/ZZZ with ZZ^3 = ZZZ^2.
+//
+// XYZZ ("extended Jacobian") needs no inversion, its mixed addition is 8M+2S
+// against 7M+4S for Jacobian, and infinity is simply ZZ = ZZZ = 0.  The group
+// law (and hence every MSM result) is the same as the reference's Jacobian
+// add-2007-bl / dbl-2009-l (bls/cpp/bls12_381_g1.hpp:74-151); only the
+// projective representation differs, and outputs are normalised to affine.
+//
+// Formulas: Bernstein-Lange EFD g1p/short-w "xyzz": madd-2008-s, add-2008-s,
+// dbl-2008-s-1, mdbl-2008-s-1.
+#pragma once
+
+#include "field.h"
+
+namespace kz {
+
+struct CurveBN254 {
+    using Fp = BnFp;
+    using Fr = BnFr;
+    static constexpr int FB = 32;        // bytes per coordinate on the wire
+    static constexpr uint32_t B = 3;
+    static constexpr int ID = 1;         // gpukit_curve
+};
+
+struct CurveBLS12381 {
+    using Fp = BlsFp;
+    using Fr = BlsFr;
+    static constexpr int FB = 48;
+    static constexpr uint32_t B = 4;
+    static constexpr int ID = 2;
+};
+
+template <class F>
+struct Affine {
+    Fe<F> x, y;
+};
+
+template <class F>
+struct Xyzz {
+    Fe<F> X, Y, ZZ, ZZZ;
+};
+
+template <class F>
+KZ_DEV Xyzz<F> xyzz_inf() {
+    Xyzz<F> r;
+    r.X = fe_zero<F>();
+    r.Y = fe_zero<F>();
+    r.ZZ = fe_zero<F>();
+    r.ZZZ = fe_zero<F>();
+    return r;
+}
+
+template <class F>
+KZ_DEV bool xyzz_is_inf(const Xyzz<F>& p) {
+    return fe_is_zero<F>(p.ZZ);
+}
+
+template <class F>
+KZ_DEV Xyzz<F> xyzz_from_affine(const Affine<F>& a) {
+    Xyzz<F> r;
+    r.X = a.x;
+    r.Y = a.y;
+    r.ZZ = fe_one<F>();
+    r.ZZZ = fe_one<F>();
+    return r;
+}
+
+template <class F>
+KZ_DEV Xyzz<F> xyzz_neg(const Xyzz<F>& p) {
+    Xyzz<F> r = p;
+    r.Y = fe_neg<F>(p.Y);
+    return r;
+}
+
+// 2*(x, y) for an affine point (mdbl-2008-s-1).
+template <class F>
+KZ_DEV Xyzz<F> xyzz_dbl_affine(const Affine<F>& a) {
+    Fe<F> U = fe_dbl<F>(a.y);
+    Fe<F> V = fe_sqr<F>(U);
+    Fe<F> W = fe_mul<F>(U, V);
+    Fe<F> S = fe_mul<F>(a.x, V);
+    Fe<F> xx = fe_sqr<F>(a.x);
+    Fe<F> M = fe_add<F>(fe_dbl<F>(xx), xx);
+    Xyzz<F> r;
+    r.X = fe_sub<F>(fe_sqr<F>(M), fe_dbl<F>(S));
+    r.Y = fe_sub<F>(fe_mul<F>(M, fe_sub<F>(S, r.X)), fe_mul<F>(W, a.y));
+    r.ZZ = V;
+    r.ZZZ = W;
+    return r;
+}
+
+// 2*P (dbl-2008-s-1, a = 0).  Y = 0 gives V = 0, i.e. infinity.
+template <class F>
+KZ_DEV Xyzz<F> xyzz_dbl(const Xyzz<F>& p) {
+    Fe<F> U = fe_dbl<F>(p.Y);
+    Fe<F> V = fe_sqr<F>(U);
+    Fe<F> W = fe_mul<F>(U, V);
+    Fe<F> S = fe_mul<F>(p.X, V);
+    Fe<F> xx = fe_sqr<F>(p.X);
+    Fe<F> M = fe_add<F>(fe_dbl<F>(xx), xx);
+    Xyzz<F> r;
+    r.X = fe_sub<F>(fe_sqr<F>(M), fe_dbl<F>(S));
+    r.Y = fe_sub<F>(fe_mul<F>(M, fe_sub<F>(S, r.X)), fe_mul<F>(W, p.Y));
+    r.ZZ = fe_mul<F>(V, p.ZZ);
+    r.ZZZ = fe_mul<F>(W, p.ZZZ);
+    return r;
+}
+
+// acc + (x, y), (x, y) an affine point that is not infinity (madd-2008-s).
+// The rare equal-points branch stays inline: an out-of-line call there makes
+// the caller save registers around it and spill in the hot loop.
+template <class F>
+KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
+    if (xyzz_is_inf<F>(acc)) {
+        acc = xyzz_from_affine<F>(q);
+        return;
+    }
+    Fe<F> U2 = fe_mul<F>(q.x, acc.ZZ);
+    Fe<F> S2 = fe_mul<F>(q.y, acc.ZZZ);
+    Fe<F> P = fe_sub<F>(U2, acc.X);
+    Fe<F> R = fe_sub<F>(S2, acc.Y);
+    if (fe_is_zero<F>(P)) {
+        if (fe_is_zero<F>(R)) acc = xyzz_dbl_affine<F>(q);
+        else acc = xyzz_inf<F>();
+        return;
+    }
+    Fe<F> PP = fe_sqr<F>(P);
+    Fe<F> PPP = fe_mul<F>(P, PP);
+    Fe<F> Q = fe_mul<F>(acc.X, PP);
+    Fe<F> X3 = fe_sub<F>(fe_sub<F>(fe_sqr<F>(R), PPP), fe_dbl<F>(Q));
+    Fe<F> Y3 = fe_sub<F>(fe_mul<F>(R, fe_sub<F>(Q, X3)), fe_mul<F>(acc.Y, PPP));
+    acc.X = X3;
+    acc.Y = Y3;
+    acc.ZZ = fe_mul<F>(acc.ZZ, PP);
+    acc.ZZZ = fe_mul<F>(acc.ZZZ, PPP);
+}
+
+// a + b, general (add-2008-s), with the doubling and inverse special cases.
+template <class F>
+KZ_DEV Xyzz<F> xyzz_add(const Xyzz<F>& a, const Xyzz<F>& b) {
+    if (xyzz_is_inf<F>(a)) return b;
+    if (xyzz_is_inf<F>(b)) return a;
+    Fe<F> U1 = fe_mul<F>(a.X, b.ZZ);
+    Fe<F> U2 = fe_mul<F>(b.X, a.ZZ);
+    Fe<F> S1 = fe_mul<F>(a.Y, b.ZZZ);
+    Fe<F> S2 = fe_mul<F>(b.Y, a.ZZZ);
+    Fe<F> P = fe_sub<F>(U2, U1);
+    Fe<F> R = fe_sub<F>(S2, S1);
+    if (fe_is_zero<F>(P)) {
+        if (fe_is_zero<F>(R)) return xyzz_dbl<F>(a);
+        return xyzz_inf<F>();
+    }
+    Fe<F> PP = fe_sqr<F>(P);
+    Fe<F> PPP = fe_mul<F>(P, PP);
+    Fe<F> Q = fe_mul<F>(U1, PP);
+    Xyzz<F> r;
+    r.X = fe_sub<F>(fe_sub<F>(fe_sqr<F>(R), PPP), fe_dbl<F>(Q));
+    r.Y = fe_sub<F>(fe_mul<F>(R, fe_sub<F>(Q, r.X)), fe_mul<F>(S1, PPP));
+    r.ZZ = fe_mul<F>(fe_mul<F>(a.ZZ, b.ZZ), PP);
+    r.ZZZ = fe_mul<F>(fe_mul<F>(a.ZZZ, b.ZZZ), PPP);
+    return r;
+}
+
+// Out-of-line point operations for latency-bound stages (bucket reduction,
+// window combine, fix-ups).  Each is compiled once per field, which keeps
+// those kernels small; the throughput-bound accumulation inlines instead.
+template <class F>
+__device__ __noinline__ Xyzz<F> xyzz_add_ol(Xyzz<F> a, Xyzz<F> b) {
+    return xyzz_add<F>(a, b);
+}
+
+template <class F>
+__device__ __noinline__ Xyzz<F> xyzz_dbl_ol(Xyzz<F> a) {
+    return xyzz_dbl<F>(a);
+}
+
+template <class F>
+__device__ __noinline__ Fe<F> fe_mul_ol(Fe<F> a, Fe<F> b) {
+    return fe_mul<F>(a, b);
+}
+
+// Affine normalisation (x = X/ZZ, y = Y/ZZZ) with one inversion of ZZ*ZZZ.
+template <class F>
+KZ_DEV void xyzz_to_affine(const Xyzz<F>& p, Affine<F>& out, bool& inf) {
+    inf = xyzz_is_inf<F>(p);
+    if (inf) {
+        out.x = fe_zero<F>();
+        out.y = fe_zero<F>();
+        return;
+    }
+    Fe<F> I = fe_inv<F>(fe_mul_ol<F>(p.ZZ, p.ZZZ));   // Z^-5
+    Fe<F> zz_inv = fe_mul_ol<F>(I, p.ZZZ);             // Z^-2
+    Fe<F> zzz_inv = fe_mul_ol<F>(I, p.ZZ);             // Z^-3
+    out.x = fe_mul_ol<F>(p.X, zz_inv);
+    out.y = fe_mul_ol<F>(p.Y, zzz_inv);
+}
+
+// ------------------------------------------------------------ memory I/O
+template <class F>
+KZ_DEV Fe<F> fe_load(const uint32_t* __restrict__ p) {
+    Fe<F> r;
+    if constexpr (F::N % 4 == 0) {
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int i = 0; i < F::N / 4; ++i) {
+            uint4 t = q[i];
+            r.v[4 * i] = t.x;
+            r.v[4 * i + 1] = t.y;
+            r.v[4 * i + 2] = t.z;
+            r.v[4 * i + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < F::N; ++i) r.v[i] = p[i];
+    }
+    return r;
+}
+
+template <class F>
+KZ_DEV void fe_store(uint32_t* __restrict__ p, const Fe<F>& a) {
+    if constexpr (F::N % 4 == 0) {
+        uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+        for (int i = 0; i < F::N / 4; ++i)
+            q[i] = make_uint4(a.v[4 * i], a.v[4 * i + 1], a.v[4 * i + 2], a.v[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < F::N; ++i) p[i] = a.v[i];
+    }
+}
+
+// Affine point array: x[N] y[N] per point (2N words).
+template <class F>
+KZ_DEV Affine<F> affine_load(const uint32_t* __restrict__ base, uint32_t idx) {
+    const uint32_t* p = base + (size_t)idx * (2 * F::N);
+    Affine<F> a;
+    a.x = fe_load<F>(p);
+    a.y = fe_load<F>(p + F::N);
+    return a;
+}
+
+template <class F>
+KZ_DEV void affine_store(uint32_t* __restrict__ base, size_t idx, const Affine<F>& a) {
+    uint32_t* p = base + idx * (2 * F::N);
+    fe_store<F>(p, a.x);
+    fe_store<F>(p + F::N, a.y);
+}
+
+// XYZZ point array: X Y ZZ ZZZ per point (4N words).
+template <class F>
+KZ_DEV Xyzz<F> xyzz_load(const uint32_t* __restrict__ base, size_t idx) {
+    const uint32_t* p = base + idx * (4 * F::N);
+    Xyzz<F> r;
+    r.X = fe_load<F>(p);
+    r.Y = fe_load<F>(p + F::N);
+    r.ZZ = fe_load<F>(p + 2 * F::N);
+    r.ZZZ = fe_load<F>(p + 3 * F::N);
+    return r;
+}
+
+template <class F>
+KZ_DEV void xyzz_store(uint32_t* __restrict__ base, size_t idx, const Xyzz<F>& a) {
+    uint32_t* p = base + idx * (4 * F::N);
+    fe_store<F>(p, a.X);
+    fe_store<F>(p + F::N, a.Y);
+    fe_store<F>(p + 2 * F::N, a.ZZ);
+    fe_store<F>(p + 3 * F::N, a.ZZZ);
+}
+
+}  // namespace kz

@@ -1,0 +1,104 @@
+"""ctypes mirror of the HIP library's C ABI.
+
+Names, argument meaning and return codes follow the reference's C entry points
+(gpukit/include/kinet/gpukit/multi_pippenger.h:57-92, c-abi/crypto.h:60-66);
+Python callers get bytes in, bytes out, and an exception for a non-OK status.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+try:
+    # torch ships its own libamdhip64.so.7 (same SONAME as /opt/rocm's).  Loading
+    # torch first makes the HIP library bind to that one runtime, so device
+    # pointers and streams from torch and from this library share a context.
+    import torch  # noqa: F401
+except ImportError:  # the C ABI itself needs no torch
+    torch = None
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libkinet_crypto_hip.so")
+
+CURVE_BN254_G1 = 1
+CURVE_BLS12_381_G1 = 2
+FIELD_BLS12_381_FR = 0
+FIELD_BN254_FR = 1
+
+GPUKIT_OK = 0
+GPUKIT_STATUS = {0: "GPUKIT_OK", -1: "GPUKIT_ERR_NULL_ARG", -2: "GPUKIT_ERR_BAD_SIZE", -3: "GPUKIT_ERR_BACKEND",
+                 -4: "GPUKIT_ERR_DISPATCH", -5: "GPUKIT_ERR_NOTIMPL", -6: "GPUKIT_ERR_DETERMINISM"}
+CRYPTO_STATUS = {0: "CRYPTO_OK", -1: "CRYPTO_ERR_INPUT", -2: "CRYPTO_ERR_LENGTH", -3: "CRYPTO_ERR_VERIFY",
+                 -4: "CRYPTO_ERR_BACKEND", -5: "CRYPTO_ERR_NOTIMPL", -6: "CRYPTO_ERR_INTERNAL"}
+
+
+class HipUnavailable(RuntimeError):
+    """The HIP library is missing or no HIP device is usable."""
+
+
+class KinetError(RuntimeError):
+    def __init__(self, fn, code, table):
+        self.code = code
+        super().__init__(f"{fn} returned {code} ({table.get(code, '?')}): {last_error()}")
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def library():
+    """Loads the HIP library (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise HipUnavailable(f"HIP library not built: {_LIB_PATH} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(_LIB_PATH)
+        vp, sz, u32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int
+        lib.kinet_hip_last_error.restype = ctypes.c_char_p
+        lib.kinet_hip_available.restype = i32
+        for name in ("gpukit_multi_pippenger", "gpukit_multi_pippenger_hip"):
+            getattr(lib, name).argtypes = [u32, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p]
+            getattr(lib, name).restype = i32
+        lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
+        lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, vp, vp]
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    if _lib is None:
+        return ""
+    msg = _lib.kinet_hip_last_error()
+    return msg.decode() if msg else ""
+
+
+def require_device():
+    lib = library()
+    if not lib.kinet_hip_available():
+        raise HipUnavailable(f"no usable HIP device: {last_error()}")
+    return lib
+
+
+def point_bytes(curve: int) -> int:
+    return 96 if curve == CURVE_BLS12_381_G1 else 64
+
+
+def compressed_bytes(curve: int) -> int:
+    return 48 if curve == CURVE_BLS12_381_G1 else 32
+
+
+def multi_pippenger(curve: int, scalars: bytes, points: bytes, backend: str = "dispatch") -> bytes:
+    """sum s_i P_i through gpukit_multi_pippenger (or _hip with backend="hip")."""
+    lib = require_device()
+    n = len(scalars) // 32
+    if len(scalars) != 32 * n or len(points) != point_bytes(curve) * n:
+        raise ValueError("scalars must be n*32 bytes and points n*point_bytes(curve) bytes")
+    out = ctypes.create_string_buffer(point_bytes(curve))
+    fn = lib.gpukit_multi_pippenger_hip if backend == "hip" else lib.gpukit_multi_pippenger
+    rc = fn(curve, scalars, points, n, out)
+    if rc != GPUKIT_OK:
+        raise KinetError(fn.__name__, rc, GPUKIT_STATUS)
+    return out.raw

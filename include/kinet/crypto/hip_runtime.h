@@ -1,0 +1,46 @@
+/*
+ * Runtime controls and device-pointer entry points of the HIP library.
+ *
+ * No counterpart in the reference: the reference's GPU drivers take host
+ * buffers only (e.g. kzg/gpu/cuda/kzg_driver_cuda.cpp:44-58).  These entries
+ * serve callers whose inputs already live in device memory (benchmarks,
+ * batching and serving layers).  Device pointers are HIP device addresses;
+ * `stream` is a hipStream_t (NULL = the null stream).  Calls are asynchronous
+ * on that stream; concurrent calls that share the library's MSM scratch are
+ * serialised on the host.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1 if a HIP device is usable, else 0 (reason via kinet_hip_last_error). */
+int kinet_hip_available(void);
+
+/* Message of the last failure inside the library (thread-shared, best effort). */
+const char* kinet_hip_last_error(void);
+
+/* Wire points (BE affine, zero = identity) -> device form: Montgomery affine
+ * (2 x 12 words BLS12-381, 2 x 8 words BN254 per point) and a byte per point
+ * flagging the identity. */
+int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
+                             void* stream);
+
+/* MSM over prepared points.  d_scalars: n x 32 bytes LE.  window_bits: 0 picks
+ * from n, else 5..18.  Writes the affine wire result (64/96 bytes) followed by
+ * its compressed form (32/48 bytes) at d_out. */
+int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                           size_t n, int window_bits, void* d_out, void* stream);
+
+/* Measured VALU throughput of the current device, in instructions per second
+ * summed over all lanes: op 0 v_mad_u64_u32 (the roofline denominator of the
+ * MSM and NTT kernels), 1 v_mul_lo_u32 + add, 2 v_mul_hi_u32 + xor,
+ * 3 v_mad_u32_u24, 4 v_fma_f64, 5 add3 + shift.  Returns 0 on success. */
+int kinet_hip_valu_peak(int op, double* ops_per_second);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

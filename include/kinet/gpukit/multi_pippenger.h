@@ -1,0 +1,55 @@
+/*
+ * Multi-curve Pippenger MSM, HIP backend for AMD MI355X (gfx950).
+ *
+ *   R = sum_{i=0..n-1} s_i * P_i
+ *
+ * Wire ABI, identical to the reference (gpukit/include/kinet/gpukit/multi_pippenger.h:27-39,
+ * as the reference code implements it in gpukit/cpp/cpu_reference/multi_pippenger.cpp:267-290
+ * and gpukit/curve_traits/bls12_381_g1_first_party_traits.h:58-85):
+ *   scalars : n x 32 bytes, little-endian; any 256-bit value (not reduced mod r)
+ *   points  : BN254 G1 n x 64 bytes, BLS12-381 G1 n x 96 bytes; affine x || y,
+ *             each coordinate big-endian; all-zero bytes = identity;
+ *             coordinates are read mod p
+ *   result  : same layout as one point; identity = all-zero bytes
+ *
+ * Results equal sum(s_i * P_i) exactly for points on the curve, so they match
+ * the reference byte for byte for every scalar below 2^255 (every s < r).  For
+ * scalars >= 2^255 this backend keeps the signed-digit carry that the
+ * reference drops out of its last window (multi_pippenger.cpp:118-126).
+ *
+ * Return codes are gpukit_status (gpukit.h).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kinet/gpukit/gpukit.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gpukit_curve {
+    GPUKIT_CURVE_SECP256K1 = 0,   /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
+    GPUKIT_CURVE_BN254_G1 = 1,
+    GPUKIT_CURVE_BLS12_381_G1 = 2,
+    GPUKIT_CURVE_BANDERWAGON = 3, /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
+} gpukit_curve;
+
+/* Replaces the dispatcher of gpukit/c-abi/c_multi_pippenger.cpp:9-42.
+ * GPUKIT_BACKEND unset, empty or "hip" runs the HIP backend; any other value
+ * returns GPUKIT_ERR_BACKEND, since this library holds no other backend. */
+int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
+                           uint8_t* result);
+
+/* The HIP backend itself, a sibling of gpukit_multi_pippenger_{cpu,cuda,metal,wgsl}
+ * (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92).  n == 0 gives the
+ * identity; n >= 2^26 gives GPUKIT_ERR_BAD_SIZE; secp256k1 and Banderwagon give
+ * GPUKIT_ERR_NOTIMPL; device failures give GPUKIT_ERR_BACKEND (message via
+ * kinet_hip_last_error()). */
+int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
+                               uint8_t* result);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

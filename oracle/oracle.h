@@ -103,6 +103,8 @@ void orc_gen_scalars(int curve, uint64_t seed, size_t n, uint8_t* out);
 void orc_powers(int curve, const uint8_t* tau_le, size_t n, uint8_t* out);
 /* points k_i * G (slow: one scalar multiplication each). */
 void orc_gen_points(int curve, const uint8_t* ks, size_t n, uint8_t* out_wire);
+/* points (k0 + i*d) * G for i < n (fast: one addition each, batch-normalised). */
+void orc_gen_points_progression(int curve, const uint8_t* k0_le, const uint8_t* d_le, size_t n, uint8_t* out_wire);
 
 #ifdef __cplusplus
 }

@@ -236,3 +236,52 @@ void orc_gen_points(int curve, const uint8_t* ks, size_t n, uint8_t* out) {
     const int pb = orc_point_bytes(curve);
     for (size_t i = 0; i < n; ++i) orc_g1_mul_gen(curve, ks + 32 * i, out + pb * i);
 }
+
+/* (k0 + i*d) * G for i < n: one Jacobian addition per point and a Montgomery
+ * batch inversion (prefix products, one inverse, backward sweep) for the
+ * affine normalisation, so large test sets are cheap.  A point that hits
+ * infinity is written as all-zero bytes. */
+void orc_gen_points_progression(int curve, const uint8_t* k0_le, const uint8_t* d_le, size_t n, uint8_t* out) {
+    const orc_curve* C = orc_get_curve(curve);
+    const orc_field* F = C->F;
+    const int fb = C->fp_bytes;
+    const int nl = F->n;
+    orc_jac g, cur, step;
+    u64 k0[4], d[4];
+    le_to_limbs(k0, k0_le, 32);
+    le_to_limbs(d, d_le, 32);
+    g_generator(C, &g, curve);
+    g_mul(C, &cur, &g, k0);
+    g_mul(C, &step, &g, d);
+    orc_jac* pts = (orc_jac*)malloc(sizeof(orc_jac) * n);
+    u64 (*pref)[6] = (u64 (*)[6])malloc(sizeof(u64[6]) * (n + 1));
+    memcpy(pref[0], F->r1, 8 * nl);
+    for (size_t i = 0; i < n; ++i) {
+        pts[i] = cur;
+        const u64* z = (cur.inf || mp_is_zero(cur.Z, nl)) ? F->r1 : cur.Z;
+        f_mul(F, pref[i + 1], pref[i], z);
+        g_add(C, &cur, &cur, &step);
+    }
+    u64 inv[6];
+    f_inv(F, inv, pref[n]);
+    for (size_t i = n; i-- > 0;) {
+        u64 zi[6], t[6];
+        const int skip = pts[i].inf || mp_is_zero(pts[i].Z, nl);
+        const u64* z = skip ? F->r1 : pts[i].Z;
+        f_mul(F, zi, inv, pref[i]);   /* 1/z_i */
+        f_mul(F, inv, inv, z);        /* drop z_i from the running inverse */
+        uint8_t* w = out + (size_t)2 * fb * i;
+        if (skip) { memset(w, 0, 2 * fb); continue; }
+        u64 zi2[6], x[6], y[6], xc[6], yc[6];
+        f_mul(F, zi2, zi, zi);
+        f_mul(F, x, pts[i].X, zi2);
+        f_mul(F, t, zi2, zi);
+        f_mul(F, y, pts[i].Y, t);
+        f_from_mont(F, xc, x);
+        f_from_mont(F, yc, y);
+        limbs_to_be(w, xc, fb);
+        limbs_to_be(w + fb, yc, fb);
+    }
+    free(pts);
+    free(pref);
+}

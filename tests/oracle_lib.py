@@ -52,6 +52,7 @@ def orc():
         lib.orc_kzg_commit_closed_form.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
         lib.orc_powers.argtypes = [ctypes.c_int, _u8p, _sz, _u8p]
         lib.orc_gen_points.argtypes = [ctypes.c_int, _u8p, _sz, _u8p]
+        lib.orc_gen_points_progression.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
         lib.orc_field_mont_consts.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.POINTER(ctypes.c_uint64)]
         _orc = lib
     return _orc
@@ -151,6 +152,13 @@ def gen_points(curve: int, ks: bytes) -> bytes:
     n = len(ks) // 32
     out = _buf(POINT_BYTES[curve] * n)
     orc().orc_gen_points(curve, ks, n, out)
+    return out.raw
+
+
+def gen_points_progression(curve: int, k0: int, d: int, n: int) -> bytes:
+    """(k0 + i*d) * G for i < n."""
+    out = _buf(POINT_BYTES[curve] * n)
+    orc().orc_gen_points_progression(curve, k0.to_bytes(32, "little"), d.to_bytes(32, "little"), n, out)
     return out.raw
 
 

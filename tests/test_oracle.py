@@ -210,6 +210,19 @@ def test_msm_top_carry_beyond_reference(curve):
     assert O.msm(curve, scalars, points) == O.g1_mul_gen(curve, t.to_bytes(32, "little"))
 
 
+@pytest.mark.parametrize("curve", [1, 2])
+def test_progression_points(curve):
+    r = M.CURVES[curve]["r"]
+    k0, d, n = 12345678901234567890123, 987654321987654321, 40
+    pts = O.gen_points_progression(curve, k0, d, n)
+    ks = b"".join(((k0 + i * d) % r).to_bytes(32, "little") for i in range(n))
+    assert pts == O.gen_points(curve, ks)
+    # a progression that passes through infinity: k0 = r - 2d, so i = 2 hits 0
+    pts = O.gen_points_progression(curve, r - 2 * d, d, 4)
+    pb = O.POINT_BYTES[curve]
+    assert pts[2 * pb:3 * pb] == bytes(pb) and O.on_curve(curve, pts[3 * pb:])
+
+
 def test_msm_empty_is_identity():
     for curve in (1, 2):
         assert O.msm(curve, b"", b"") == bytes(O.POINT_BYTES[curve])
