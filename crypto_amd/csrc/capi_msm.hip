@@ -78,7 +78,7 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
         hipStream_t s = e.stream;
         uint8_t* d_sc = static_cast<uint8_t*>(e.h2d_a.ensure(32 * n));
         uint8_t* d_wire = static_cast<uint8_t*>(e.h2d_b.ensure(pb * n));
-        uint32_t* d_pts = static_cast<uint32_t*>(e.pts.ensure(2 * fp_words((int)curve) * 4 * n));
+        uint32_t* d_pts = static_cast<uint32_t*>(e.pts.ensure((size_t)affine_words((int)curve) * 4 * n));
         uint8_t* d_inf = static_cast<uint8_t*>(e.inf.ensure(n));
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes((int)curve)));
         KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));

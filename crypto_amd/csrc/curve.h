@@ -162,21 +162,10 @@ KZ_DEV Xyzz<F> xyzz_add(const Xyzz<F>& a, const Xyzz<F>& b) {
     return r;
 }
 
-// Out-of-line point operations for latency-bound stages (bucket reduction,
-// window combine, fix-ups).  Each is compiled once per field, which keeps
-// those kernels small; the throughput-bound accumulation inlines instead.
+// Out-of-line multiply for the single-thread affine normalisation, which
+// keeps that rarely-run code small.
 template <class F>
-__device__ __noinline__ Xyzz<F> xyzz_add_ol(Xyzz<F> a, Xyzz<F> b) {
-    return xyzz_add<F>(a, b);
-}
-
-template <class F>
-__device__ __noinline__ Xyzz<F> xyzz_dbl_ol(Xyzz<F> a) {
-    return xyzz_dbl<F>(a);
-}
-
-template <class F>
-__device__ __noinline__ Fe<F> fe_mul_ol(Fe<F> a, Fe<F> b) {
+__host__ __device__ __noinline__ Fe<F> fe_mul_ol(Fe<F> a, Fe<F> b) {
     return fe_mul<F>(a, b);
 }
 
@@ -197,75 +186,96 @@ KZ_DEV void xyzz_to_affine(const Xyzz<F>& p, Affine<F>& out, bool& inf) {
 }
 
 // ------------------------------------------------------------ memory I/O
+// Point records in HBM are padded to a multiple of 4 words so every record is
+// 16-byte aligned and moves as whole dwordx4 transfers:
+//   affine  x[N] y[N]            -> kAffWords  (BLS12-381 28, BN254 20)
+//   XYZZ    X[N] Y[N] ZZ[N] ZZZ[N] -> kXyzzWords (BLS12-381 56, BN254 36)
 template <class F>
-KZ_DEV Fe<F> fe_load(const uint32_t* __restrict__ p) {
-    Fe<F> r;
-    if constexpr (F::N % 4 == 0) {
-        const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-        for (int i = 0; i < F::N / 4; ++i) {
-            uint4 t = q[i];
-            r.v[4 * i] = t.x;
-            r.v[4 * i + 1] = t.y;
-            r.v[4 * i + 2] = t.z;
-            r.v[4 * i + 3] = t.w;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < F::N; ++i) r.v[i] = p[i];
-    }
-    return r;
-}
+struct Rec {
+    static constexpr int kAffWords = (2 * F::N + 3) / 4 * 4;
+    static constexpr int kXyzzWords = (4 * F::N + 3) / 4 * 4;
+};
 
-template <class F>
-KZ_DEV void fe_store(uint32_t* __restrict__ p, const Fe<F>& a) {
-    if constexpr (F::N % 4 == 0) {
-        uint4* q = reinterpret_cast<uint4*>(p);
+template <int K>
+KZ_DEV void load_words(const uint32_t* __restrict__ p, uint32_t (&w)[K]) {
+    static_assert(K % 4 == 0, "records are whole dwordx4");
+    const uint4* q = reinterpret_cast<const uint4*>(p);
 #pragma unroll
-        for (int i = 0; i < F::N / 4; ++i)
-            q[i] = make_uint4(a.v[4 * i], a.v[4 * i + 1], a.v[4 * i + 2], a.v[4 * i + 3]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < F::N; ++i) p[i] = a.v[i];
+    for (int i = 0; i < K / 4; ++i) {
+        const uint4 t = q[i];
+        w[4 * i] = t.x;
+        w[4 * i + 1] = t.y;
+        w[4 * i + 2] = t.z;
+        w[4 * i + 3] = t.w;
     }
 }
 
-// Affine point array: x[N] y[N] per point (2N words).
+template <int K>
+KZ_DEV void store_words(uint32_t* __restrict__ p, const uint32_t (&w)[K]) {
+    static_assert(K % 4 == 0, "records are whole dwordx4");
+    uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+    for (int i = 0; i < K / 4; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
 template <class F>
 KZ_DEV Affine<F> affine_load(const uint32_t* __restrict__ base, uint32_t idx) {
-    const uint32_t* p = base + (size_t)idx * (2 * F::N);
+    constexpr int K = Rec<F>::kAffWords;
+    uint32_t w[K];
+    load_words<K>(base + (size_t)idx * K, w);
     Affine<F> a;
-    a.x = fe_load<F>(p);
-    a.y = fe_load<F>(p + F::N);
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        a.x.v[i] = w[i];
+        a.y.v[i] = w[F::N + i];
+    }
     return a;
 }
 
 template <class F>
 KZ_DEV void affine_store(uint32_t* __restrict__ base, size_t idx, const Affine<F>& a) {
-    uint32_t* p = base + idx * (2 * F::N);
-    fe_store<F>(p, a.x);
-    fe_store<F>(p + F::N, a.y);
+    constexpr int K = Rec<F>::kAffWords;
+    uint32_t w[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) w[i] = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        w[i] = a.x.v[i];
+        w[F::N + i] = a.y.v[i];
+    }
+    store_words<K>(base + idx * K, w);
 }
 
-// XYZZ point array: X Y ZZ ZZZ per point (4N words).
 template <class F>
 KZ_DEV Xyzz<F> xyzz_load(const uint32_t* __restrict__ base, size_t idx) {
-    const uint32_t* p = base + idx * (4 * F::N);
+    constexpr int K = Rec<F>::kXyzzWords;
+    uint32_t w[K];
+    load_words<K>(base + idx * K, w);
     Xyzz<F> r;
-    r.X = fe_load<F>(p);
-    r.Y = fe_load<F>(p + F::N);
-    r.ZZ = fe_load<F>(p + 2 * F::N);
-    r.ZZZ = fe_load<F>(p + 3 * F::N);
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        r.X.v[i] = w[i];
+        r.Y.v[i] = w[F::N + i];
+        r.ZZ.v[i] = w[2 * F::N + i];
+        r.ZZZ.v[i] = w[3 * F::N + i];
+    }
     return r;
 }
 
 template <class F>
 KZ_DEV void xyzz_store(uint32_t* __restrict__ base, size_t idx, const Xyzz<F>& a) {
-    uint32_t* p = base + idx * (4 * F::N);
-    fe_store<F>(p, a.X);
-    fe_store<F>(p + F::N, a.Y);
-    fe_store<F>(p + 2 * F::N, a.ZZ);
-    fe_store<F>(p + 3 * F::N, a.ZZZ);
+    constexpr int K = Rec<F>::kXyzzWords;
+    uint32_t w[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) w[i] = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        w[i] = a.X.v[i];
+        w[F::N + i] = a.Y.v[i];
+        w[2 * F::N + i] = a.ZZ.v[i];
+        w[3 * F::N + i] = a.ZZZ.v[i];
+    }
+    store_words<K>(base + idx * K, w);
 }
 
 }  // namespace kz

@@ -1,84 +1,95 @@
-// Device finite-field arithmetic for gfx950: Montgomery form over 32-bit limbs.
+// Finite-field arithmetic for gfx950: Montgomery form over unsaturated
+// radix-2^29 limbs, host- and device-compilable (the host build is what the
+// CPU tests use to check the arithmetic against the oracle).
 //
-// Four fields, all with the top bit of the modulus free (p < 2^(32N-1) and the
-// top limb < 0x7FFFFFFE), which lets the CIOS product run without the extra
-// carry word: the interleaved "no-carry" CIOS (Koc-Acar-Kaliski CIOS with the
-// spare-bit shortcut).  Products are 32x32->64 multiply-adds, which hipcc lowers
-// to v_mad_u64_u32 chains; additions use v_add_co/v_addc_co carry chains.
+// Why radix 2^29: the product-scanning Montgomery multiply below accumulates a
+// whole column (up to 2N products of 29-bit limbs, < 2N * 2^58 < 2^63) in one
+// 64-bit register pair, so every partial product is a single v_mad_u64_u32
+// whose addend is the accumulator itself -- no carry handling inside a
+// column and no register shuffling to build 64-bit addends.  With saturated
+// 32-bit limbs the same multiply compiled to ~2 v_mov per v_mad.
 //
-// Values are kept fully reduced (< p) between operations, so every field
-// element has a unique representation and equality is limb equality.
+// Fields (N limbs, R = 2^(29N)):
+//   BLS12-381 Fp  N = 14  (381-bit p)   bls/cpp/bls12_381_fp.hpp:83-85
+//   BLS12-381 Fr  N = 9   (255-bit r)   kzg/cpp/kzg_oracle.hpp:66-71
+//   BN254 Fp, Fr  N = 9   (254-bit)     bn254/cpp/bn254_fp.hpp:75-81
+// All four satisfy 4p < R, so Montgomery products of values below p stay
+// below 2p before the final conditional subtraction.
 //
-// Field constants restate the reference's moduli:
-//   BLS12-381 Fp  bls/cpp/bls12_381_fp.hpp:83-98
-//   BLS12-381 Fr  kzg/cpp/kzg_oracle.hpp:66-90
-//   BN254 Fp, Fr  bn254/cpp/bn254_fp.hpp:75-97
-// split into 32-bit limbs (R = 2^(32N) is the same value as the reference's
-// R = 2^(64n) for n = N/2 u64 limbs, so Montgomery forms coincide).
+// Invariant: every Fe produced by the arithmetic has limbs < 2^29 and value
+// < p, so representations are unique and equality is limb equality.
+// Montgomery forms differ from the reference's (R = 2^384 / 2^256) but never
+// leave the device: inputs and outputs are canonical integers.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define KZ_DEV __device__ __forceinline__
-#define KZ_HD __host__ __device__ __forceinline__
+#define KZ_DEV __host__ __device__ __forceinline__
 
 namespace kz {
 
+constexpr int kLimbBits = 29;
+constexpr uint32_t kLimbMask = (1u << kLimbBits) - 1u;
+
 struct BlsFp {
-    static constexpr int N = 12;
-    static constexpr uint32_t PINV = 0xFFFCFFFDu;
-    static constexpr uint32_t P[12] = {0xFFFFAAABu, 0xB9FEFFFFu, 0xB153FFFFu, 0x1EABFFFEu,
-                                       0xF6B0F624u, 0x6730D2A0u, 0xF38512BFu, 0x64774B84u,
-                                       0x434BACD7u, 0x4B1BA7B6u, 0x397FE69Au, 0x1A0111EAu};
-    static constexpr uint32_t ONE[12] = {0x0002FFFDu, 0x76090000u, 0xC40C0002u, 0xEBF4000Bu,
-                                         0x53C758BAu, 0x5F489857u, 0x70525745u, 0x77CE5853u,
-                                         0xA256EC6Du, 0x5C071A97u, 0xFA80E493u, 0x15F65EC3u};
-    static constexpr uint32_t R2[12] = {0x1C341746u, 0xF4DF1F34u, 0x09D104F1u, 0x0A76E6A6u,
-                                        0x4C95B6D5u, 0x8DE5476Cu, 0x939D83C0u, 0x67EB88A9u,
-                                        0xB519952Du, 0x9A793E85u, 0x92CAE3AAu, 0x11988FE5u};
-    static constexpr uint32_t R3[12] = {0xD94CA1E0u, 0xED48AC6Bu, 0x03A7ADF8u, 0x315F831Eu,
-                                        0x615E29DDu, 0x9A53352Au, 0x921E1761u, 0x34C04E5Eu,
-                                        0x65724728u, 0x2512D435u, 0x91755D4Du, 0x0AA63460u};
+    static constexpr int N = 14;
+    static constexpr int BITS = 381;
+    static constexpr uint32_t PINV = 0x1FFCFFFDu;
+    static constexpr uint32_t P[14] = {0x1FFFAAABu, 0x0FF7FFFFu, 0x14FFFFEEu, 0x17FFFD62u, 0x0F6241EAu,
+                                       0x09507B58u, 0x0AFD9CC3u, 0x109E70A2u, 0x1764774Bu, 0x121A5D66u,
+                                       0x12C6E9EDu, 0x12FFCD34u, 0x00111EA3u, 0x0000000Du};
+    static constexpr uint32_t ONE[14] = {0x03A9FB84u, 0x0BA00690u, 0x071288F1u, 0x0F59BCC5u, 0x126CB614u,
+                                         0x0585BF36u, 0x1B85AC3Du, 0x1CF856FAu, 0x1891ECBDu, 0x1A7EEC05u,
+                                         0x155A88F0u, 0x0741AC6Du, 0x1317C30Fu, 0x00000009u};
+    static constexpr uint32_t R2[14] = {0x15BEF7AEu, 0x1031CD0Eu, 0x02DD93E8u, 0x09226323u, 0x0E6E2CD2u,
+                                        0x11684DAAu, 0x1170E5DBu, 0x088E25B1u, 0x1B366399u, 0x1C536F47u,
+                                        0x0D1F9CBCu, 0x0278B67Fu, 0x1EA66A2Bu, 0x0000000Cu};
+    static constexpr uint32_t R3[14] = {0x09217D6Au, 0x1D6118BAu, 0x1114B11Cu, 0x0126AEE7u, 0x0A55E2C4u,
+                                        0x04D63CE0u, 0x154FF87Du, 0x14555478u, 0x1D1BDC0Du, 0x161F98D4u,
+                                        0x1D74E921u, 0x09B4345Au, 0x1E5ECFB8u, 0x0000000Au};
 };
 
 struct BlsFr {
-    static constexpr int N = 8;
-    static constexpr uint32_t PINV = 0xFFFFFFFFu;
-    static constexpr uint32_t P[8] = {0x00000001u, 0xFFFFFFFFu, 0xFFFE5BFEu, 0x53BDA402u,
-                                      0x09A1D805u, 0x3339D808u, 0x299D7D48u, 0x73EDA753u};
-    static constexpr uint32_t ONE[8] = {0xFFFFFFFEu, 0x00000001u, 0x00034802u, 0x5884B7FAu,
-                                        0xECBC4FF5u, 0x998C4FEFu, 0xACC5056Fu, 0x1824B159u};
-    static constexpr uint32_t R2[8] = {0xF3F29C6Du, 0xC999E990u, 0x87925C23u, 0x2B6CEDCBu,
-                                       0x7254398Fu, 0x05D31496u, 0x9F59FF11u, 0x0748D9D9u};
-    static constexpr uint32_t R3[8] = {0x439B73AFu, 0xC62C1807u, 0x8CF06990u, 0x1B3E0D18u,
-                                       0xC7B5F418u, 0x73D13C71u, 0xC8DB33E9u, 0x6E2A5BB9u};
+    static constexpr int N = 9;
+    static constexpr int BITS = 255;
+    static constexpr uint32_t PINV = 0x1FFFFFFFu;
+    static constexpr uint32_t P[9] = {0x00000001u, 0x1FFFFFF8u, 0x1F96FFBFu, 0x1B4805FFu, 0x1D80553Bu,
+                                      0x0C0404D0u, 0x1520CCE7u, 0x0A6533AFu, 0x0073EDA7u};
+    static constexpr uint32_t ONE[9] = {0x1FFFFFBAu, 0x0000022Fu, 0x1CB61180u, 0x0A4E5C00u, 0x0EE8B1A2u,
+                                        0x16E6AEDFu, 0x1907F8BBu, 0x0853DDF7u, 0x004D043Fu};
+    static constexpr uint32_t R2[9] = {0x0A71B3C0u, 0x1D32207Eu, 0x1663D999u, 0x1C5ABC93u, 0x03B58C44u,
+                                       0x0BE37438u, 0x0829F771u, 0x1660139Eu, 0x0027FD91u};
+    static constexpr uint32_t R3[9] = {0x19D7065Du, 0x0020DB85u, 0x16122E43u, 0x0EDB1FF8u, 0x0FDA6124u,
+                                       0x0517AC72u, 0x12E6A522u, 0x19D54EDAu, 0x0009750Bu};
 };
 
 struct BnFp {
-    static constexpr int N = 8;
-    static constexpr uint32_t PINV = 0xE4866389u;
-    static constexpr uint32_t P[8] = {0xD87CFD47u, 0x3C208C16u, 0x6871CA8Du, 0x97816A91u,
-                                      0x8181585Du, 0xB85045B6u, 0xE131A029u, 0x30644E72u};
-    static constexpr uint32_t ONE[8] = {0xC58F0D9Du, 0xD35D438Du, 0xF5C70B3Du, 0x0A78EB28u,
-                                        0x7879462Cu, 0x666EA36Fu, 0x9A07DF2Fu, 0x0E0A77C1u};
-    static constexpr uint32_t R2[8] = {0x538AFA89u, 0xF32CFC5Bu, 0xD44501FBu, 0xB5E71911u,
-                                       0x0A417FF6u, 0x47AB1EFFu, 0xCAB8351Fu, 0x06D89F71u};
-    static constexpr uint32_t R3[8] = {0xDA1530DFu, 0xB1CD6DAFu, 0xA7283DB6u, 0x62F210E6u,
-                                       0x0ADA0AFBu, 0xEF7F0B0Cu, 0x2D592544u, 0x20FD6E90u};
+    static constexpr int N = 9;
+    static constexpr int BITS = 254;
+    static constexpr uint32_t PINV = 0x04866389u;
+    static constexpr uint32_t P[9] = {0x187CFD47u, 0x010460B6u, 0x1C72A34Fu, 0x02D522D0u, 0x1585D978u,
+                                      0x02DB40C0u, 0x00A6E141u, 0x0E5C2634u, 0x0030644Eu};
+    static constexpr uint32_t ONE[9] = {0x157CCC21u, 0x141C2758u, 0x185230D3u, 0x014C0419u, 0x0AA36FB9u,
+                                        0x1D4240CEu, 0x11D54C07u, 0x052AC7A8u, 0x000DC836u};
+    static constexpr uint32_t R2[9] = {0x059BAC10u, 0x0D1503A3u, 0x018016B8u, 0x10AB0CA8u, 0x02632639u,
+                                       0x02C0169Fu, 0x169BFD53u, 0x11869D4Cu, 0x002A11A6u};
+    static constexpr uint32_t R3[9] = {0x0E2312B2u, 0x16C05CA2u, 0x0BC84389u, 0x1CDF310Bu, 0x11ADAFDDu,
+                                       0x032E568Eu, 0x1D6AE48Cu, 0x10D4CD1Fu, 0x0026C2D2u};
 };
 
 struct BnFr {
-    static constexpr int N = 8;
-    static constexpr uint32_t PINV = 0xEFFFFFFFu;
-    static constexpr uint32_t P[8] = {0xF0000001u, 0x43E1F593u, 0x79B97091u, 0x2833E848u,
-                                      0x8181585Du, 0xB85045B6u, 0xE131A029u, 0x30644E72u};
-    static constexpr uint32_t ONE[8] = {0x4FFFFFFBu, 0xAC96341Cu, 0x9F60CD29u, 0x36FC7695u,
-                                        0x7879462Eu, 0x666EA36Fu, 0x9A07DF2Fu, 0x0E0A77C1u};
-    static constexpr uint32_t R2[8] = {0xAE216DA7u, 0x1BB8E645u, 0xE35C59E3u, 0x53FE3AB1u,
-                                       0x53BB8085u, 0x8C49833Du, 0x7F4E44A5u, 0x0216D0B1u};
-    static constexpr uint32_t R3[8] = {0xB4BF0040u, 0x5E94D8E1u, 0x1CFBB6B8u, 0x2A489CBEu,
-                                       0xA19FCFEDu, 0x893CC664u, 0x7FCC657Cu, 0x0CF8594Bu};
+    static constexpr int N = 9;
+    static constexpr int BITS = 254;
+    static constexpr uint32_t PINV = 0x0FFFFFFFu;
+    static constexpr uint32_t P[9] = {0x10000001u, 0x1F0FAC9Fu, 0x0E5C2450u, 0x07D090F3u, 0x1585D283u,
+                                      0x02DB40C0u, 0x00A6E141u, 0x0E5C2634u, 0x0030644Eu};
+    static constexpr uint32_t ONE[9] = {0x0FFFFF57u, 0x1EA70AB4u, 0x052C068Bu, 0x17504F49u, 0x0AA8075Bu,
+                                        0x1D4240CEu, 0x11D54C07u, 0x052AC7A8u, 0x000DC836u};
+    static constexpr uint32_t R2[9] = {0x05B69BD4u, 0x06170A5Au, 0x020CDDCEu, 0x1DB6310Bu, 0x0E54D0FFu,
+                                       0x1CF855E3u, 0x1C15E103u, 0x07D09161u, 0x000A054Au};
+    static constexpr uint32_t R3[9] = {0x001FDDB2u, 0x17D30B63u, 0x1A2600EEu, 0x09507C47u, 0x1496B29Bu,
+                                       0x0B00A268u, 0x15B645EBu, 0x1F9FCB3Du, 0x001BAA96u};
 };
 
 template <class F>
@@ -96,11 +107,16 @@ KZ_DEV Fe<F> fe_zero() {
 }
 
 template <class F>
-KZ_DEV Fe<F> fe_one() {
+KZ_DEV Fe<F> fe_const(const uint32_t (&c)[F::N]) {
     Fe<F> r;
 #pragma unroll
-    for (int i = 0; i < F::N; ++i) r.v[i] = F::ONE[i];
+    for (int i = 0; i < F::N; ++i) r.v[i] = c[i];
     return r;
+}
+
+template <class F>
+KZ_DEV Fe<F> fe_one() {
+    return fe_const<F>(F::ONE);
 }
 
 template <class F>
@@ -119,19 +135,28 @@ KZ_DEV bool fe_eq(const Fe<F>& a, const Fe<F>& b) {
     return acc == 0;
 }
 
-// r = a - p if a >= p else a (a < 2p).
+// a - p if a >= p, else a; a normalised (limbs < 2^29), a < 2^(29N).
 template <class F>
 KZ_DEV void fe_reduce_once(Fe<F>& a) {
     uint32_t d[F::N];
     uint32_t borrow = 0;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
-        uint64_t s = (uint64_t)a.v[i] - F::P[i] - borrow;
-        d[i] = (uint32_t)s;
-        borrow = (uint32_t)(s >> 32) & 1u;
+        const uint32_t t = a.v[i] - F::P[i] - borrow;
+        borrow = t >> 31;
+        d[i] = t & kLimbMask;
     }
 #pragma unroll
     for (int i = 0; i < F::N; ++i) a.v[i] = borrow ? a.v[i] : d[i];
+}
+
+// Is the normalised value a < p?
+template <class F>
+KZ_DEV bool fe_is_canonical(const Fe<F>& a) {
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) borrow = (a.v[i] - F::P[i] - borrow) >> 31;
+    return borrow != 0;
 }
 
 template <class F>
@@ -140,34 +165,34 @@ KZ_DEV Fe<F> fe_add(const Fe<F>& a, const Fe<F>& b) {
     uint32_t carry = 0;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
-        uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
-        s.v[i] = (uint32_t)t;
-        carry = (uint32_t)(t >> 32);
+        const uint32_t t = a.v[i] + b.v[i] + carry;
+        carry = t >> kLimbBits;
+        s.v[i] = t & kLimbMask;
     }
-    // a + b < 2p < 2^(32N): no carry out of the top limb.
+    // a + b < 2p < 2^(29N): no carry out of the top limb.
     fe_reduce_once<F>(s);
     return s;
 }
 
 template <class F>
 KZ_DEV Fe<F> fe_sub(const Fe<F>& a, const Fe<F>& b) {
-    Fe<F> d, e;
+    Fe<F> d;
     uint32_t borrow = 0;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
-        uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
-        d.v[i] = (uint32_t)t;
-        borrow = (uint32_t)(t >> 32) & 1u;
+        const uint32_t t = a.v[i] - b.v[i] - borrow;
+        borrow = t >> 31;
+        d.v[i] = t & kLimbMask;
     }
+    // on borrow the limbs hold a - b + 2^(29N); adding p wraps back into range
+    const uint32_t mask = 0u - borrow;
     uint32_t carry = 0;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
-        uint64_t t = (uint64_t)d.v[i] + F::P[i] + carry;
-        e.v[i] = (uint32_t)t;
-        carry = (uint32_t)(t >> 32);
+        const uint32_t t = d.v[i] + (F::P[i] & mask) + carry;
+        carry = t >> kLimbBits;
+        d.v[i] = t & kLimbMask;
     }
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) d.v[i] = borrow ? e.v[i] : d.v[i];
     return d;
 }
 
@@ -181,34 +206,38 @@ KZ_DEV Fe<F> fe_neg(const Fe<F>& a) {
     return fe_sub<F>(fe_zero<F>(), a);
 }
 
-// Montgomery product a*b*2^(-32N) mod p, no-carry CIOS.  a, b < p -> result < p.
+// Montgomery product a b 2^(-29N) mod p by product scanning: column k of
+// a*b + m*p accumulates in one 64-bit value; for k < N the digit m_k is chosen
+// to zero the column's low 29 bits.  2N^2 multiply-adds.
 template <class F>
 KZ_DEV Fe<F> fe_mul(const Fe<F>& a, const Fe<F>& b) {
     constexpr int N = F::N;
-    uint32_t t[N];
+    uint32_t m[N];
+    Fe<F> t;
+    uint64_t acc = 0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) t[j] = 0;
+    for (int k = 0; k < N; ++k) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const uint32_t bi = b.v[i];
-        uint64_t A = (uint64_t)a.v[0] * bi + t[0];
-        t[0] = (uint32_t)A;
-        const uint32_t m = t[0] * F::PINV;
-        uint64_t C = (uint64_t)m * F::P[0] + t[0];
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.v[i] * b.v[k - i];
 #pragma unroll
-        for (int j = 1; j < N; ++j) {
-            A = (uint64_t)a.v[j] * bi + t[j] + (A >> 32);
-            t[j] = (uint32_t)A;
-            C = (uint64_t)m * F::P[j] + t[j] + (C >> 32);
-            t[j - 1] = (uint32_t)C;
-        }
-        t[N - 1] = (uint32_t)(C >> 32) + (uint32_t)(A >> 32);
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * F::P[k - i];
+        m[k] = ((uint32_t)acc * F::PINV) & kLimbMask;
+        acc += (uint64_t)m[k] * F::P[0];
+        acc >>= kLimbBits;
     }
-    Fe<F> r;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.v[j] = t[j];
-    fe_reduce_once<F>(r);
-    return r;
+    for (int k = N; k < 2 * N - 1; ++k) {
+#pragma unroll
+        for (int i = k - N + 1; i < N; ++i) {
+            acc += (uint64_t)a.v[i] * b.v[k - i];
+            acc += (uint64_t)m[i] * F::P[k - i];
+        }
+        t.v[k - N] = (uint32_t)acc & kLimbMask;
+        acc >>= kLimbBits;
+    }
+    t.v[N - 1] = (uint32_t)acc;  // < 2p < 2^(29N): fits the top limb
+    fe_reduce_once<F>(t);
+    return t;
 }
 
 template <class F>
@@ -218,10 +247,7 @@ KZ_DEV Fe<F> fe_sqr(const Fe<F>& a) {
 
 template <class F>
 KZ_DEV Fe<F> fe_to_mont(const Fe<F>& a) {
-    Fe<F> r2;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) r2.v[i] = F::R2[i];
-    return fe_mul<F>(a, r2);
+    return fe_mul<F>(a, fe_const<F>(F::R2));
 }
 
 template <class F>
@@ -231,28 +257,78 @@ KZ_DEV Fe<F> fe_from_mont(const Fe<F>& a) {
     return fe_mul<F>(a, one);
 }
 
-// Is a (canonical) < p?
+// ------------------------------------------------------ integer conversions
+// Little-endian 32-bit words (nw words, value < 2^(29N)) to radix-2^29 limbs.
 template <class F>
-KZ_DEV bool fe_is_canonical(const Fe<F>& a) {
-    uint32_t borrow = 0;
+KZ_DEV Fe<F> fe_from_words(const uint32_t* w, int nw) {
+    Fe<F> r;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
-        uint64_t s = (uint64_t)a.v[i] - F::P[i] - borrow;
-        borrow = (uint32_t)(s >> 32) & 1u;
+        const int bit = kLimbBits * i;
+        const int wi = bit >> 5, sh = bit & 31;
+        uint32_t v = (wi < nw) ? (w[wi] >> sh) : 0u;
+        if (sh > 32 - kLimbBits && wi + 1 < nw) v |= w[wi + 1] << (32 - sh);
+        r.v[i] = v & kLimbMask;
     }
+    return r;
+}
+
+// Radix-2^29 limbs to nw little-endian 32-bit words.
+template <class F>
+KZ_DEV void fe_to_words(const Fe<F>& a, uint32_t* w, int nw) {
+    for (int k = 0; k < nw; ++k) w[k] = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int bit = kLimbBits * i;
+        const int wi = bit >> 5, sh = bit & 31;
+        if (wi < nw) w[wi] |= a.v[i] << sh;
+        if (sh > 32 - kLimbBits && wi + 1 < nw) w[wi + 1] |= a.v[i] >> (32 - sh);
+    }
+}
+
+// Big-endian bytes to limbs (value < 2^(29N)).
+template <class F>
+KZ_DEV Fe<F> fe_from_be_bytes(const uint8_t* b, int nbytes) {
+    uint32_t w[16];
+    const int nw = (nbytes + 3) / 4;
+    for (int k = 0; k < nw; ++k) w[k] = 0;
+    for (int k = 0; k < nbytes; ++k) {
+        const int bit = 8 * (nbytes - 1 - k);
+        w[bit >> 5] |= (uint32_t)b[k] << (bit & 31);
+    }
+    return fe_from_words<F>(w, nw);
+}
+
+template <class F>
+KZ_DEV void fe_to_be_bytes(const Fe<F>& a, uint8_t* b, int nbytes) {
+    uint32_t w[16];
+    const int nw = (nbytes + 3) / 4;
+    fe_to_words<F>(a, w, nw);
+    for (int k = 0; k < nbytes; ++k) {
+        const int bit = 8 * (nbytes - 1 - k);
+        b[k] = (uint8_t)(w[bit >> 5] >> (bit & 31));
+    }
+}
+
+// Integer comparison a > b of normalised values.
+template <class F>
+KZ_DEV bool fe_gt(const Fe<F>& a, const Fe<F>& b) {
+    uint32_t borrow = 0;  // computes b - a; borrow <=> b < a
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) borrow = (b.v[i] - a.v[i] - borrow) >> 31;
     return borrow != 0;
 }
 
 // ------------------------------------------------------------ inversion
-// Variable-time binary extended Euclid on the plain integer value of a
-// (HAC 14.61 specialised to an odd prime modulus).  Used only on a handful of
-// elements (final normalisation), where its ~2*bits short iterations beat a
-// ~1.5*bits-multiplication Fermat chain by a wide margin in latency.
+// Variable-time binary extended Euclid (HAC 14.61 for an odd prime modulus)
+// on the plain integer value; used only on a handful of elements (final
+// normalisation), where its ~2*bits short iterations beat a Fermat chain of
+// ~1.5*bits multiplications in latency.
 template <class F>
-KZ_DEV void mp_shr1(uint32_t* a, uint32_t top) {
+KZ_DEV void mp_shr1(uint32_t* a) {
 #pragma unroll
-    for (int i = 0; i < F::N - 1; ++i) a[i] = (a[i] >> 1) | (a[i + 1] << 31);
-    a[F::N - 1] = (a[F::N - 1] >> 1) | (top << 31);
+    for (int i = 0; i < F::N - 1; ++i) a[i] = (a[i] >> 1) | ((a[i + 1] & 1u) << (kLimbBits - 1));
+    a[F::N - 1] >>= 1;
 }
 
 template <class F>
@@ -263,114 +339,48 @@ KZ_DEV bool mp_is_one(const uint32_t* a) {
     return acc == 0;
 }
 
-template <class F>
-KZ_DEV bool mp_geq(const uint32_t* a, const uint32_t* b) {
-    uint32_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        uint64_t s = (uint64_t)a[i] - b[i] - borrow;
-        borrow = (uint32_t)(s >> 32) & 1u;
-    }
-    return borrow == 0;
-}
-
-template <class F>
-KZ_DEV void mp_sub_inplace(uint32_t* a, const uint32_t* b) {
-    uint32_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        uint64_t s = (uint64_t)a[i] - b[i] - borrow;
-        a[i] = (uint32_t)s;
-        borrow = (uint32_t)(s >> 32) & 1u;
-    }
-}
-
-// x <- x/2 mod p for x < p.
+// x <- x/2 mod p, x < p normalised.
 template <class F>
 KZ_DEV void fe_half_plain(uint32_t* x) {
     if (x[0] & 1u) {
         uint32_t carry = 0;
 #pragma unroll
         for (int i = 0; i < F::N; ++i) {
-            uint64_t t = (uint64_t)x[i] + F::P[i] + carry;
-            x[i] = (uint32_t)t;
-            carry = (uint32_t)(t >> 32);
+            const uint32_t t = x[i] + F::P[i] + carry;
+            carry = t >> kLimbBits;
+            x[i] = t & kLimbMask;
         }
-        mp_shr1<F>(x, carry);
-    } else {
-        mp_shr1<F>(x, 0);
+        // x + p < 2^(29N): no carry out
     }
+    mp_shr1<F>(x);
 }
 
-// Inverse of a Montgomery-form element (a != 0): returns a^-1 in Montgomery form.
+// Inverse of a Montgomery-form element (a != 0): a^-1 in Montgomery form.
 template <class F>
 KZ_DEV Fe<F> fe_inv(const Fe<F>& a) {
     if (fe_is_zero<F>(a)) return a;  // no inverse; callers never ask for it
-    uint32_t u[F::N], v[F::N], x1[F::N], x2[F::N];
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        u[i] = a.v[i];
-        v[i] = F::P[i];
-        x1[i] = 0;
-        x2[i] = 0;
-    }
-    x1[0] = 1;
-    while (!mp_is_one<F>(u) && !mp_is_one<F>(v)) {
-        while ((u[0] & 1u) == 0) {
-            mp_shr1<F>(u, 0);
-            fe_half_plain<F>(x1);
+    Fe<F> u = a, v = fe_const<F>(F::P), x1 = fe_zero<F>(), x2 = fe_zero<F>();
+    x1.v[0] = 1;
+    while (!mp_is_one<F>(u.v) && !mp_is_one<F>(v.v)) {
+        while ((u.v[0] & 1u) == 0) {
+            mp_shr1<F>(u.v);
+            fe_half_plain<F>(x1.v);
         }
-        while ((v[0] & 1u) == 0) {
-            mp_shr1<F>(v, 0);
-            fe_half_plain<F>(x2);
+        while ((v.v[0] & 1u) == 0) {
+            mp_shr1<F>(v.v);
+            fe_half_plain<F>(x2.v);
         }
-        if (mp_geq<F>(u, v)) {
-            mp_sub_inplace<F>(u, v);
-            Fe<F> t1, t2;
-#pragma unroll
-            for (int i = 0; i < F::N; ++i) { t1.v[i] = x1[i]; t2.v[i] = x2[i]; }
-            t1 = fe_sub<F>(t1, t2);
-#pragma unroll
-            for (int i = 0; i < F::N; ++i) x1[i] = t1.v[i];
+        if (!fe_gt<F>(v, u)) {  // u >= v
+            u = fe_sub<F>(u, v);  // no wrap: u - v >= 0
+            x1 = fe_sub<F>(x1, x2);
         } else {
-            mp_sub_inplace<F>(v, u);
-            Fe<F> t1, t2;
-#pragma unroll
-            for (int i = 0; i < F::N; ++i) { t1.v[i] = x1[i]; t2.v[i] = x2[i]; }
-            t2 = fe_sub<F>(t2, t1);
-#pragma unroll
-            for (int i = 0; i < F::N; ++i) x2[i] = t2.v[i];
+            v = fe_sub<F>(v, u);
+            x2 = fe_sub<F>(x2, x1);
         }
     }
-    Fe<F> r;
-    const bool uone = mp_is_one<F>(u);
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) r.v[i] = uone ? x1[i] : x2[i];
-    // r = (a R)^-1 as a plain value; times R^3 in Montgomery gives a^-1 R.
-    Fe<F> r3;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) r3.v[i] = F::R3[i];
-    return fe_mul<F>(r, r3);
-}
-
-// --------------------------------------------------------------- codecs
-// Big-endian bytes (N*4 of them, or nbytes < N*4 with leading limbs zero) to limbs.
-template <class F>
-KZ_DEV Fe<F> fe_from_be_bytes(const uint8_t* b, int nbytes) {
-    Fe<F> r = fe_zero<F>();
-    for (int k = 0; k < nbytes; ++k) {
-        const int bit = 8 * (nbytes - 1 - k);
-        r.v[bit >> 5] |= (uint32_t)b[k] << (bit & 31);
-    }
-    return r;
-}
-
-template <class F>
-KZ_DEV void fe_to_be_bytes(const Fe<F>& a, uint8_t* b, int nbytes) {
-    for (int k = 0; k < nbytes; ++k) {
-        const int bit = 8 * (nbytes - 1 - k);
-        b[k] = (uint8_t)(a.v[bit >> 5] >> (bit & 31));
-    }
+    const Fe<F> r = mp_is_one<F>(u.v) ? x1 : x2;
+    // r = (a R)^-1 as a plain value; its Montgomery product with R^3 is a^-1 R.
+    return fe_mul<F>(r, fe_const<F>(F::R3));
 }
 
 }  // namespace kz

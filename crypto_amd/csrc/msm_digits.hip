@@ -47,7 +47,7 @@ __device__ __forceinline__ void load_scalar(const uint32_t* __restrict__ scalars
 template <int CB, bool SCATTER>
 __global__ void __launch_bounds__(256)
 k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n,
-         uint32_t* __restrict__ counts_or_cursor, uint32_t* __restrict__ entries, uint32_t* __restrict__ ebucket) {
+         uint32_t* __restrict__ counts_or_cursor, uint32_t* __restrict__ entries) {
     using D = Digits<CB>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -70,7 +70,6 @@ k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, 
             } else {
                 const uint32_t pos = atomicAdd(&counts_or_cursor[gb], 1u);
                 entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
-                ebucket[pos] = gb;
             }
         }
     }
@@ -78,21 +77,21 @@ k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, 
 
 template <int CB>
 static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* counts,
-                   uint32_t* entries, uint32_t* ebucket, hipStream_t s) {
+                   uint32_t* entries, hipStream_t s) {
     const unsigned grid = ceil_div(n, 256);
     if (!scatter)
         hipLaunchKernelGGL((k_digits<CB, false>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, counts,
-                           nullptr, nullptr);
+                           nullptr);
     else
         hipLaunchKernelGGL((k_digits<CB, true>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, counts,
-                           entries, ebucket);
+                           entries);
 }
 
 void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
-                     uint32_t* counts_or_cursor, uint32_t* entries, uint32_t* ebucket, hipStream_t s) {
+                     uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s) {
     switch (c) {
 #define KZ_C(CB) \
-    case CB: launch<CB>(scatter, d_scalars, d_inf, n, counts_or_cursor, entries, ebucket, s); break;
+    case CB: launch<CB>(scatter, d_scalars, d_inf, n, counts_or_cursor, entries, s); break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15)
         KZ_C(16) KZ_C(17) KZ_C(18)
 #undef KZ_C

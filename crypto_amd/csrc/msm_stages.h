@@ -1,0 +1,28 @@
+// Launchers of the curve-templated MSM kernels.  Each stage lives in its own
+// translation unit per curve (msm_<stage>_<curve>.hip) so the builds run in
+// parallel; the host pipeline (msm_host.hip) calls them through these.
+#pragma once
+
+#include "common.h"
+
+namespace kz {
+
+#define KZ_DECLARE_STAGES(sfx)                                                                              \
+    void launch_points_from_wire_##sfx(const uint8_t* wire, uint32_t* pts, uint8_t* inf, uint32_t n,        \
+                                       hipStream_t s);                                                      \
+    void launch_compress_##sfx(const uint8_t* wire, uint8_t* out, hipStream_t s);                           \
+    void launch_accumulate_##sfx(unsigned grid, hipStream_t s, const uint32_t* entries,                     \
+                                 const uint32_t* offsets, uint32_t nb, uint32_t chunk, const uint32_t* pts, \
+                                 uint32_t* bsum, uint32_t* carry);                                          \
+    void launch_reduce_seg_##sfx(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,     \
+                                 const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA, \
+                                 uint32_t* segB);                                                           \
+    void launch_reduce_window_##sfx(unsigned nwin, hipStream_t s, const uint32_t* segA,                     \
+                                    const uint32_t* segB, uint32_t t1, uint32_t* winres);                   \
+    void launch_combine_##sfx(hipStream_t s, const uint32_t* winres, int nwin, int c, uint8_t* out);
+
+KZ_DECLARE_STAGES(bls)
+KZ_DECLARE_STAGES(bn)
+#undef KZ_DECLARE_STAGES
+
+}  // namespace kz

@@ -1,0 +1,15 @@
+// BN254 G1: window combine, normalisation and compression kernels (msm_impl.h).
+#include "msm_impl.h"
+#include "msm_stages.h"
+
+namespace kz {
+
+void launch_combine_bn(hipStream_t s, const uint32_t* winres, int nwin, int c, uint8_t* out) {
+    hipLaunchKernelGGL(k_combine<CurveBN254>, dim3(1), dim3(64), 0, s, winres, nwin, c, out);
+}
+
+void launch_compress_bn(const uint8_t* wire, uint8_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_compress_point<CurveBN254>, dim3(1), dim3(64), 0, s, wire, out);
+}
+
+}  // namespace kz

@@ -10,8 +10,9 @@ import torch
 from .lib import GPUKIT_OK, GPUKIT_STATUS, KinetError, point_bytes, require_device
 
 
-def fp_words(curve: int) -> int:
-    return 12 if curve == 2 else 8
+def affine_words(curve: int) -> int:
+    """Words per prepared (device-form) affine point record."""
+    return 28 if curve == 2 else 20
 
 
 def stream_handle(stream=None) -> int:
@@ -33,7 +34,7 @@ class PreparedPoints:
         pb = point_bytes(curve)
         d_wire = wire if isinstance(wire, torch.Tensor) else to_device(wire, device)
         self.n = d_wire.numel() // pb
-        self.pts = torch.empty(self.n * 2 * fp_words(curve) * 4 + 16, dtype=torch.uint8, device=device)
+        self.pts = torch.empty(self.n * affine_words(curve) * 4 + 16, dtype=torch.uint8, device=device)
         self.inf = torch.empty(self.n + 16, dtype=torch.uint8, device=device)
         rc = lib.kinet_hip_points_prepare(curve, d_wire.data_ptr(), self.n, self.pts.data_ptr(),
                                           self.inf.data_ptr(), stream_handle())

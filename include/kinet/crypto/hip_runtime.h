@@ -24,8 +24,8 @@ int kinet_hip_available(void);
 const char* kinet_hip_last_error(void);
 
 /* Wire points (BE affine, zero = identity) -> device form: Montgomery affine
- * (2 x 12 words BLS12-381, 2 x 8 words BN254 per point) and a byte per point
- * flagging the identity. */
+ * records of 28 (BLS12-381) or 20 (BN254) 32-bit words per point (radix-2^29
+ * limbs, 16-byte aligned), and a byte per point flagging the identity. */
 int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
                              void* stream);
 
