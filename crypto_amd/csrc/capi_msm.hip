@@ -127,10 +127,11 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 }
 
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
-                           size_t n, int window_bits, void* d_out, void* stream) {
+                           size_t n, int window_bits, int chunk, void* d_out, void* stream) {
     if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
     if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
+    if (chunk < 0 || chunk > 4096) return GPUKIT_ERR_BAD_SIZE;
     if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
     try {
         Engine& e = engine();
@@ -144,6 +145,7 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
         }
         MsmParams prm;
         prm.c = window_bits;
+        if (chunk > 0) prm.chunk = chunk;
         msm_device((int)curve, static_cast<const uint32_t*>(d_pts), static_cast<const uint8_t*>(d_inf),
                    static_cast<const uint32_t*>(d_scalars), n, prm, e.ws, s, static_cast<uint8_t*>(d_out));
         return GPUKIT_OK;

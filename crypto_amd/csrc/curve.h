@@ -41,6 +41,30 @@ struct Xyzz {
     Fe<F> X, Y, ZZ, ZZZ;
 };
 
+// Multiplication policy of the point operations.  MulInline expands every
+// field product in place (the hot accumulation loop); MulCall calls one
+// out-of-line copy, keeping latency-bound kernels (bucket reduction, window
+// combine) inside the instruction cache.  Operands and result travel in
+// VGPRs, so the call needs no scratch.
+template <class F>
+__host__ __device__ __noinline__ Fe<F> fe_mul_call(Fe<F> a, Fe<F> b) {
+    return fe_mul<F>(a, b);
+}
+
+struct MulInline {
+    template <class F>
+    static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
+        return fe_mul<F>(a, b);
+    }
+};
+
+struct MulCall {
+    template <class F>
+    static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
+        return fe_mul_call<F>(a, b);
+    }
+};
+
 template <class F>
 KZ_DEV Xyzz<F> xyzz_inf() {
     Xyzz<F> r;
@@ -74,99 +98,92 @@ KZ_DEV Xyzz<F> xyzz_neg(const Xyzz<F>& p) {
 }
 
 // 2*(x, y) for an affine point (mdbl-2008-s-1).
-template <class F>
+template <class F, class M = MulInline>
 KZ_DEV Xyzz<F> xyzz_dbl_affine(const Affine<F>& a) {
     Fe<F> U = fe_dbl<F>(a.y);
-    Fe<F> V = fe_sqr<F>(U);
-    Fe<F> W = fe_mul<F>(U, V);
-    Fe<F> S = fe_mul<F>(a.x, V);
-    Fe<F> xx = fe_sqr<F>(a.x);
-    Fe<F> M = fe_add<F>(fe_dbl<F>(xx), xx);
+    Fe<F> V = M::template mul<F>(U, U);
+    Fe<F> W = M::template mul<F>(U, V);
+    Fe<F> S = M::template mul<F>(a.x, V);
+    Fe<F> xx = M::template mul<F>(a.x, a.x);
+    Fe<F> xx3 = fe_add<F>(fe_dbl<F>(xx), xx);
     Xyzz<F> r;
-    r.X = fe_sub<F>(fe_sqr<F>(M), fe_dbl<F>(S));
-    r.Y = fe_sub<F>(fe_mul<F>(M, fe_sub<F>(S, r.X)), fe_mul<F>(W, a.y));
+    r.X = fe_sub<F>(M::template mul<F>(xx3, xx3), fe_dbl<F>(S));
+    r.Y = fe_sub<F>(M::template mul<F>(xx3, fe_sub<F>(S, r.X)), M::template mul<F>(W, a.y));
     r.ZZ = V;
     r.ZZZ = W;
     return r;
 }
 
 // 2*P (dbl-2008-s-1, a = 0).  Y = 0 gives V = 0, i.e. infinity.
-template <class F>
+template <class F, class M = MulInline>
 KZ_DEV Xyzz<F> xyzz_dbl(const Xyzz<F>& p) {
     Fe<F> U = fe_dbl<F>(p.Y);
-    Fe<F> V = fe_sqr<F>(U);
-    Fe<F> W = fe_mul<F>(U, V);
-    Fe<F> S = fe_mul<F>(p.X, V);
-    Fe<F> xx = fe_sqr<F>(p.X);
-    Fe<F> M = fe_add<F>(fe_dbl<F>(xx), xx);
+    Fe<F> V = M::template mul<F>(U, U);
+    Fe<F> W = M::template mul<F>(U, V);
+    Fe<F> S = M::template mul<F>(p.X, V);
+    Fe<F> xx = M::template mul<F>(p.X, p.X);
+    Fe<F> xx3 = fe_add<F>(fe_dbl<F>(xx), xx);
     Xyzz<F> r;
-    r.X = fe_sub<F>(fe_sqr<F>(M), fe_dbl<F>(S));
-    r.Y = fe_sub<F>(fe_mul<F>(M, fe_sub<F>(S, r.X)), fe_mul<F>(W, p.Y));
-    r.ZZ = fe_mul<F>(V, p.ZZ);
-    r.ZZZ = fe_mul<F>(W, p.ZZZ);
+    r.X = fe_sub<F>(M::template mul<F>(xx3, xx3), fe_dbl<F>(S));
+    r.Y = fe_sub<F>(M::template mul<F>(xx3, fe_sub<F>(S, r.X)), M::template mul<F>(W, p.Y));
+    r.ZZ = M::template mul<F>(V, p.ZZ);
+    r.ZZZ = M::template mul<F>(W, p.ZZZ);
     return r;
 }
 
 // acc + (x, y), (x, y) an affine point that is not infinity (madd-2008-s).
 // The rare equal-points branch stays inline: an out-of-line call there makes
 // the caller save registers around it and spill in the hot loop.
-template <class F>
+template <class F, class M = MulInline>
 KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
     if (xyzz_is_inf<F>(acc)) {
         acc = xyzz_from_affine<F>(q);
         return;
     }
-    Fe<F> U2 = fe_mul<F>(q.x, acc.ZZ);
-    Fe<F> S2 = fe_mul<F>(q.y, acc.ZZZ);
+    Fe<F> U2 = M::template mul<F>(q.x, acc.ZZ);
+    Fe<F> S2 = M::template mul<F>(q.y, acc.ZZZ);
     Fe<F> P = fe_sub<F>(U2, acc.X);
     Fe<F> R = fe_sub<F>(S2, acc.Y);
     if (fe_is_zero<F>(P)) {
-        if (fe_is_zero<F>(R)) acc = xyzz_dbl_affine<F>(q);
+        if (fe_is_zero<F>(R)) acc = xyzz_dbl_affine<F, M>(q);
         else acc = xyzz_inf<F>();
         return;
     }
-    Fe<F> PP = fe_sqr<F>(P);
-    Fe<F> PPP = fe_mul<F>(P, PP);
-    Fe<F> Q = fe_mul<F>(acc.X, PP);
-    Fe<F> X3 = fe_sub<F>(fe_sub<F>(fe_sqr<F>(R), PPP), fe_dbl<F>(Q));
-    Fe<F> Y3 = fe_sub<F>(fe_mul<F>(R, fe_sub<F>(Q, X3)), fe_mul<F>(acc.Y, PPP));
+    Fe<F> PP = M::template mul<F>(P, P);
+    Fe<F> PPP = M::template mul<F>(P, PP);
+    Fe<F> Q = M::template mul<F>(acc.X, PP);
+    Fe<F> X3 = fe_sub<F>(fe_sub<F>(M::template mul<F>(R, R), PPP), fe_dbl<F>(Q));
+    Fe<F> Y3 = fe_sub<F>(M::template mul<F>(R, fe_sub<F>(Q, X3)), M::template mul<F>(acc.Y, PPP));
     acc.X = X3;
     acc.Y = Y3;
-    acc.ZZ = fe_mul<F>(acc.ZZ, PP);
-    acc.ZZZ = fe_mul<F>(acc.ZZZ, PPP);
+    acc.ZZ = M::template mul<F>(acc.ZZ, PP);
+    acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
 }
 
 // a + b, general (add-2008-s), with the doubling and inverse special cases.
-template <class F>
+template <class F, class M = MulInline>
 KZ_DEV Xyzz<F> xyzz_add(const Xyzz<F>& a, const Xyzz<F>& b) {
     if (xyzz_is_inf<F>(a)) return b;
     if (xyzz_is_inf<F>(b)) return a;
-    Fe<F> U1 = fe_mul<F>(a.X, b.ZZ);
-    Fe<F> U2 = fe_mul<F>(b.X, a.ZZ);
-    Fe<F> S1 = fe_mul<F>(a.Y, b.ZZZ);
-    Fe<F> S2 = fe_mul<F>(b.Y, a.ZZZ);
+    Fe<F> U1 = M::template mul<F>(a.X, b.ZZ);
+    Fe<F> U2 = M::template mul<F>(b.X, a.ZZ);
+    Fe<F> S1 = M::template mul<F>(a.Y, b.ZZZ);
+    Fe<F> S2 = M::template mul<F>(b.Y, a.ZZZ);
     Fe<F> P = fe_sub<F>(U2, U1);
     Fe<F> R = fe_sub<F>(S2, S1);
     if (fe_is_zero<F>(P)) {
-        if (fe_is_zero<F>(R)) return xyzz_dbl<F>(a);
+        if (fe_is_zero<F>(R)) return xyzz_dbl<F, M>(a);
         return xyzz_inf<F>();
     }
-    Fe<F> PP = fe_sqr<F>(P);
-    Fe<F> PPP = fe_mul<F>(P, PP);
-    Fe<F> Q = fe_mul<F>(U1, PP);
+    Fe<F> PP = M::template mul<F>(P, P);
+    Fe<F> PPP = M::template mul<F>(P, PP);
+    Fe<F> Q = M::template mul<F>(U1, PP);
     Xyzz<F> r;
-    r.X = fe_sub<F>(fe_sub<F>(fe_sqr<F>(R), PPP), fe_dbl<F>(Q));
-    r.Y = fe_sub<F>(fe_mul<F>(R, fe_sub<F>(Q, r.X)), fe_mul<F>(S1, PPP));
-    r.ZZ = fe_mul<F>(fe_mul<F>(a.ZZ, b.ZZ), PP);
-    r.ZZZ = fe_mul<F>(fe_mul<F>(a.ZZZ, b.ZZZ), PPP);
+    r.X = fe_sub<F>(fe_sub<F>(M::template mul<F>(R, R), PPP), fe_dbl<F>(Q));
+    r.Y = fe_sub<F>(M::template mul<F>(R, fe_sub<F>(Q, r.X)), M::template mul<F>(S1, PPP));
+    r.ZZ = M::template mul<F>(M::template mul<F>(a.ZZ, b.ZZ), PP);
+    r.ZZZ = M::template mul<F>(M::template mul<F>(a.ZZZ, b.ZZZ), PPP);
     return r;
-}
-
-// Out-of-line multiply for the single-thread affine normalisation, which
-// keeps that rarely-run code small.
-template <class F>
-__host__ __device__ __noinline__ Fe<F> fe_mul_ol(Fe<F> a, Fe<F> b) {
-    return fe_mul<F>(a, b);
 }
 
 // Affine normalisation (x = X/ZZ, y = Y/ZZZ) with one inversion of ZZ*ZZZ.
@@ -178,11 +195,11 @@ KZ_DEV void xyzz_to_affine(const Xyzz<F>& p, Affine<F>& out, bool& inf) {
         out.y = fe_zero<F>();
         return;
     }
-    Fe<F> I = fe_inv<F>(fe_mul_ol<F>(p.ZZ, p.ZZZ));   // Z^-5
-    Fe<F> zz_inv = fe_mul_ol<F>(I, p.ZZZ);             // Z^-2
-    Fe<F> zzz_inv = fe_mul_ol<F>(I, p.ZZ);             // Z^-3
-    out.x = fe_mul_ol<F>(p.X, zz_inv);
-    out.y = fe_mul_ol<F>(p.Y, zzz_inv);
+    Fe<F> I = fe_inv<F>(fe_mul_call<F>(p.ZZ, p.ZZZ));   // Z^-5
+    Fe<F> zz_inv = fe_mul_call<F>(I, p.ZZZ);             // Z^-2
+    Fe<F> zzz_inv = fe_mul_call<F>(I, p.ZZ);             // Z^-3
+    out.x = fe_mul_call<F>(p.X, zz_inv);
+    out.y = fe_mul_call<F>(p.Y, zzz_inv);
 }
 
 // ------------------------------------------------------------ memory I/O

@@ -207,37 +207,67 @@ KZ_DEV Fe<F> fe_neg(const Fe<F>& a) {
 }
 
 // Montgomery product a b 2^(-29N) mod p by product scanning: column k of
-// a*b + m*p accumulates in one 64-bit value; for k < N the digit m_k is chosen
-// to zero the column's low 29 bits.  2N^2 multiply-adds.
-template <class F>
-KZ_DEV Fe<F> fe_mul(const Fe<F>& a, const Fe<F>& b) {
+// a*b + m*p accumulates in 64-bit values; for k < N the digit m_k is chosen to
+// zero the column's low 29 bits.  2N^2 multiply-adds.
+//
+// K independent accumulators per column (summed at the column's end, K-1
+// 64-bit adds) give K-way instruction-level parallelism: a single dependent
+// chain of v_mad_u64_u32 stalls on the instruction's latency whenever few
+// waves share a SIMD.  Each accumulator holds at most N products of 29-bit
+// limbs plus the column carry, far below 2^64.
+template <class F, int K>
+KZ_DEV Fe<F> fe_mul_k(const Fe<F>& a, const Fe<F>& b) {
     constexpr int N = F::N;
     uint32_t m[N];
     Fe<F> t;
-    uint64_t acc = 0;
+    uint64_t acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
 #pragma unroll
-        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.v[i] * b.v[k - i];
+        for (int i = 0; i <= k; ++i) acc[i % K] += (uint64_t)a.v[i] * b.v[k - i];
 #pragma unroll
-        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * F::P[k - i];
-        m[k] = ((uint32_t)acc * F::PINV) & kLimbMask;
-        acc += (uint64_t)m[k] * F::P[0];
-        acc >>= kLimbBits;
+        for (int i = 0; i < k; ++i) acc[(k + 1 + i) % K] += (uint64_t)m[i] * F::P[k - i];
+        uint64_t s = acc[0];
+#pragma unroll
+        for (int q = 1; q < K; ++q) s += acc[q];
+        m[k] = ((uint32_t)s * F::PINV) & kLimbMask;
+        s += (uint64_t)m[k] * F::P[0];
+        acc[0] = s >> kLimbBits;
+#pragma unroll
+        for (int q = 1; q < K; ++q) acc[q] = 0;
     }
 #pragma unroll
     for (int k = N; k < 2 * N - 1; ++k) {
+        int q = 0;
 #pragma unroll
         for (int i = k - N + 1; i < N; ++i) {
-            acc += (uint64_t)a.v[i] * b.v[k - i];
-            acc += (uint64_t)m[i] * F::P[k - i];
+            acc[q] += (uint64_t)a.v[i] * b.v[k - i];
+            q = (q + 1) % K;
+            acc[q] += (uint64_t)m[i] * F::P[k - i];
+            q = (q + 1) % K;
         }
-        t.v[k - N] = (uint32_t)acc & kLimbMask;
-        acc >>= kLimbBits;
+        uint64_t s = acc[0];
+#pragma unroll
+        for (int r = 1; r < K; ++r) s += acc[r];
+        t.v[k - N] = (uint32_t)s & kLimbMask;
+        acc[0] = s >> kLimbBits;
+#pragma unroll
+        for (int r = 1; r < K; ++r) acc[r] = 0;
     }
-    t.v[N - 1] = (uint32_t)acc;  // < 2p < 2^(29N): fits the top limb
+    t.v[N - 1] = (uint32_t)acc[0];  // < 2p < 2^(29N): fits the top limb
     fe_reduce_once<F>(t);
     return t;
+}
+
+#ifndef KZ_MUL_ILP
+#define KZ_MUL_ILP 1
+#endif
+
+template <class F>
+KZ_DEV Fe<F> fe_mul(const Fe<F>& a, const Fe<F>& b) {
+    return fe_mul_k<F, KZ_MUL_ILP>(a, b);
 }
 
 template <class F>

@@ -9,7 +9,7 @@ namespace kz {
 struct MsmWorkspace {
     DevBuf counts, offsets, cursor, block_sums, total;
     DevBuf entries;
-    DevBuf bsum, carry, segA, segB, winres;
+    DevBuf bsum, carry, run_bucket, segA, segB, winres;
     DevBuf out;
 };
 

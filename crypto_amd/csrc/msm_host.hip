@@ -23,14 +23,15 @@ constexpr uint32_t kSegBuckets = 16;  // must match kSeg in msm_impl.h
 
 struct Stages {
     decltype(&launch_accumulate_bls) accumulate;
+    decltype(&launch_fold_bls) fold;
     decltype(&launch_reduce_seg_bls) reduce_seg;
     decltype(&launch_reduce_window_bls) reduce_window;
     decltype(&launch_combine_bls) combine;
     int xyzz_words;
 };
 
-const Stages kBls = {launch_accumulate_bls, launch_reduce_seg_bls, launch_reduce_window_bls, launch_combine_bls, 56};
-const Stages kBn = {launch_accumulate_bn, launch_reduce_seg_bn, launch_reduce_window_bn, launch_combine_bn, 36};
+const Stages kBls = {launch_accumulate_bls, launch_fold_bls, launch_reduce_seg_bls, launch_reduce_window_bls, launch_combine_bls, 56};
+const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_window_bn, launch_combine_bn, 36};
 
 void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars,
                   size_t n, const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
@@ -55,6 +56,7 @@ void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf,
     ws.entries.ensure((max_entries + 1) * sizeof(uint32_t));
     ws.bsum.ensure(NB * pw);
     ws.carry.ensure((size_t)(max_chunks + 1) * pw);
+    ws.run_bucket.ensure((size_t)(max_chunks + 1) * sizeof(uint32_t));
     ws.segA.ensure((size_t)nseg * pw);
     ws.segB.ensure((size_t)nseg * pw);
     ws.winres.ensure((size_t)W * pw);
@@ -68,6 +70,11 @@ void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf,
     digits_dispatch(c, true, d_scalars, d_inf, (uint32_t)n, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
     st.accumulate(ceil_div(max_chunks, 64), s, ws.entries.as<uint32_t>(), offsets, NB, chunk, d_pts,
                   ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>());
+    // A bucket holds at most n entries (one per point), so its carry run is at
+    // most ceil(n / chunk) long: that many tree levels always suffice.
+    uint32_t levels = 0;
+    while (((size_t)1 << levels) < (n + chunk - 1) / chunk + 1) ++levels;
+    st.fold(s, offsets, NB, chunk, max_chunks, levels, ws.run_bucket.as<uint32_t>(), ws.carry.as<uint32_t>());
     st.reduce_seg(ceil_div(nseg, 64), s, offsets, chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(), nseg,
                   ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
     st.reduce_window((unsigned)W, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), t1, ws.winres.as<uint32_t>());

@@ -17,9 +17,10 @@
 //   k_reduce_window    one workgroup per window: LDS suffix scan + tree
 //   k_combine          Horner over windows, affine normalisation, compression
 //
-// Every point operation is inlined (out-of-line calls spill to scratch, which
-// throttles wide launches); kernels keep the number of inlined call sites
-// small with `#pragma unroll 1` loops over a single site.
+// The accumulation inlines its field products (throughput-bound, one compact
+// loop); the latency-bound reduction and combine kernels call one out-of-line
+// field multiply (MulCall) so their code stays inside the instruction cache,
+// and keep few point-operation sites with `#pragma unroll 1` loops.
 #pragma once
 
 #include "curve.h"
@@ -111,11 +112,54 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
     else xyzz_store<F>(bsum, g, acc);
 }
 
+// ------------------------------------------------------------ carry fold
+// The carries of a bucket that spans chunks j0..j1 (j0 = the chunk holding its
+// first entry) sit in carry[j0+1..j1], a contiguous run.  Runs are long when a
+// bucket is huge -- e.g. the top window covers only the few scalar bits above
+// the last full window, so its handful of buckets hold most points -- and a
+// serial fold would put the whole run on one lane.  Instead the runs are
+// folded by a pairwise tree: level L adds carry[j + 2^L] into carry[j] for run
+// positions r = 0 mod 2^(L+1); after ceil(log2(longest run)) levels the run's
+// head carry[j0+1] holds its total.
+
+// run_bucket[j] = bucket whose run chunk j belongs to, or ~0 if chunk j starts
+// a bucket (or lies past the end).  (Templated only so that each per-curve
+// translation unit owns its copy.)
+template <class F>
+__global__ void __launch_bounds__(256)
+k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
+             uint32_t* __restrict__ run_bucket) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nchunks) return;
+    const uint32_t start = j * chunk;
+    uint32_t rb = 0xFFFFFFFFu;
+    if (start < offsets[nb]) {
+        const uint32_t g = bucket_of(offsets, nb, start);
+        if (offsets[g] < start) rb = g;
+    }
+    run_bucket[j] = rb;
+}
+
+template <class F>
+__global__ void __launch_bounds__(64)
+k_fold_level(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ run_bucket,
+             uint32_t nchunks, uint32_t level, uint32_t* __restrict__ carry) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nchunks) return;
+    const uint32_t g = run_bucket[j];
+    if (g == 0xFFFFFFFFu) return;
+    const uint32_t head = offsets[g] / chunk + 1;
+    const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
+    const uint32_t r = j - head, s = 1u << level;
+    if ((r & (2 * s - 1)) != 0 || r + s >= len) return;
+    xyzz_store<F>(carry, j, xyzz_add<F, MulCall>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s)));
+}
+
 // ---------------------------------------------------------- bucket reduce
 constexpr int kSeg = 16;  // buckets per first-level segment
 
-// Segment t (over all windows): folds carries into its 16 bucket sums, then
-//   A = sum_l S_l,  Bw = sum_l (l+1) S_l  by running sums from the top bucket.
+// Segment t (over all windows): adds each bucket's folded carry run to its
+// sum, then A = sum_l S_l,  Bw = sum_l (l+1) S_l  by running sums from the top.
 template <class F>
 __global__ void __launch_bounds__(64)
 k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ bsum,
@@ -131,15 +175,14 @@ k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
         Xyzz<F> S = xyzz_inf<F>();
         if (end > off) {
             S = xyzz_load<F>(bsum, gb);
-            const uint32_t j1 = (end - 1) / chunk;
-#pragma unroll 1
-            for (uint32_t j = off / chunk + 1; j <= j1; ++j) S = xyzz_add<F>(S, xyzz_load<F>(carry, j));
+            const uint32_t head = off / chunk + 1;
+            if ((end - 1) / chunk >= head) S = xyzz_add<F, MulCall>(S, xyzz_load<F>(carry, head));
         }
 #pragma unroll 1
         for (int op = 0; op < 2; ++op) {
             const Xyzz<F> a = op == 0 ? run : tot;
             const Xyzz<F> b = op == 0 ? S : run;
-            const Xyzz<F> r = xyzz_add<F>(a, b);
+            const Xyzz<F> r = xyzz_add<F, MulCall>(a, b);
             if (op == 0) run = r;
             else tot = r;
         }
@@ -181,7 +224,7 @@ k_reduce_window(const uint32_t* __restrict__ segA, const uint32_t* __restrict__ 
         if (op == 0) { a = run; b = xyzz_load<F>(segA, base + t); }
         else if (op == 1) { a = u; b = run; }
         else { a = bs; b = xyzz_load<F>(segB, base + t); }
-        const Xyzz<F> r = xyzz_add<F>(a, b);
+        const Xyzz<F> r = xyzz_add<F, MulCall>(a, b);
         if (op == 0) run = r;
         else if (op == 1) u = r;
         else bs = r;
@@ -194,7 +237,7 @@ k_reduce_window(const uint32_t* __restrict__ segA, const uint32_t* __restrict__ 
     for (int s = 1; s < kReduceThreads; s <<= 1) {
         const Xyzz<F> other = (k + s < kReduceThreads) ? sh[k + s] : xyzz_inf<F>();
         __syncthreads();
-        R = xyzz_add<F>(R, other);
+        R = xyzz_add<F, MulCall>(R, other);
         sh[k] = R;
         __syncthreads();
     }
@@ -204,14 +247,14 @@ k_reduce_window(const uint32_t* __restrict__ segA, const uint32_t* __restrict__ 
     const int first_add = (k == 0 ? 0 : log_l2);
 #pragma unroll 1
     for (int st = 0; st < steps; ++st) {
-        if (st == first_add || st == steps - 1) cur = xyzz_add<F>(cur, st == first_add ? u : bs);
-        else cur = xyzz_dbl<F>(cur);
+        if (st == first_add || st == steps - 1) cur = xyzz_add<F, MulCall>(cur, st == first_add ? u : bs);
+        else cur = xyzz_dbl<F, MulCall>(cur);
     }
     sh[k] = cur;
     __syncthreads();
 #pragma unroll 1
     for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
-        if (k < s) sh[k] = xyzz_add<F>(sh[k], sh[k + s]);
+        if (k < s) sh[k] = xyzz_add<F, MulCall>(sh[k], sh[k + s]);
         __syncthreads();
     }
     if (k == 0) xyzz_store<F>(winres, w, sh[0]);
@@ -261,9 +304,9 @@ __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ win
     for (int w = nwin - 2; w >= 0; --w) {
         if (!xyzz_is_inf<F>(r)) {
 #pragma unroll 1
-            for (int i = 0; i < c; ++i) r = xyzz_dbl<F>(r);
+            for (int i = 0; i < c; ++i) r = xyzz_dbl<F, MulCall>(r);
         }
-        r = xyzz_add<F>(r, xyzz_load<F>(winres, w));
+        r = xyzz_add<F, MulCall>(r, xyzz_load<F>(winres, w));
     }
     write_output<C>(r, out);
 }

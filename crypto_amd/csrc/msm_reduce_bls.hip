@@ -4,6 +4,15 @@
 
 namespace kz {
 
+void launch_fold_bls(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
+                      uint32_t levels, uint32_t* run_bucket, uint32_t* carry) {
+    hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div(nchunks, 256)), dim3(256), 0, s, offsets, nb, chunk, nchunks,
+                       run_bucket);
+    for (uint32_t level = 0; level < levels; ++level)
+        hipLaunchKernelGGL(k_fold_level<CurveBLS12381::Fp>, dim3(ceil_div(nchunks, 64)), dim3(64), 0, s, offsets, chunk,
+                           run_bucket, nchunks, level, carry);
+}
+
 void launch_reduce_seg_bls(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,
                             const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA,
                             uint32_t* segB) {

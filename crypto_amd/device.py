@@ -44,7 +44,7 @@ class PreparedPoints:
 
 
 def msm_prepared(pp: PreparedPoints, d_scalars: torch.Tensor, n: int | None = None, window_bits: int = 0,
-                 out: torch.Tensor | None = None, stream=None, sync: bool = True) -> torch.Tensor:
+                 out: torch.Tensor | None = None, stream=None, sync: bool = True, chunk: int = 0) -> torch.Tensor:
     """MSM over prepared points; returns the device output record (wire || compressed)."""
     lib = require_device()
     n = pp.n if n is None else n
@@ -52,7 +52,7 @@ def msm_prepared(pp: PreparedPoints, d_scalars: torch.Tensor, n: int | None = No
     if out is None:
         out = torch.empty(pb + pb // 2, dtype=torch.uint8, device=d_scalars.device)
     rc = lib.kinet_hip_msm_prepared(pp.curve, d_scalars.data_ptr(), pp.pts.data_ptr(), pp.inf.data_ptr(), n,
-                                    window_bits, out.data_ptr(), stream_handle(stream))
+                                    window_bits, chunk, out.data_ptr(), stream_handle(stream))
     if rc != GPUKIT_OK:
         raise KinetError("kinet_hip_msm_prepared", rc, GPUKIT_STATUS)
     if sync:

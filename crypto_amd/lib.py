@@ -63,7 +63,7 @@ def library():
             getattr(lib, name).argtypes = [u32, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p]
             getattr(lib, name).restype = i32
         lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
-        lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, vp, vp]
+        lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         _lib = lib
     return _lib
 

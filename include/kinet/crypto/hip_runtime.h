@@ -30,10 +30,11 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
                              void* stream);
 
 /* MSM over prepared points.  d_scalars: n x 32 bytes LE.  window_bits: 0 picks
- * from n, else 5..18.  Writes the affine wire result (64/96 bytes) followed by
+ * from n, else 5..18.  chunk: sorted bucket entries per accumulation thread,
+ * 0 for the default.  Writes the affine wire result (64/96 bytes) followed by
  * its compressed form (32/48 bytes) at d_out. */
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
-                           size_t n, int window_bits, void* d_out, void* stream);
+                           size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
 /* Measured VALU throughput of the current device, in instructions per second
  * summed over all lanes: op 0 v_mad_u64_u32 (the roofline denominator of the
