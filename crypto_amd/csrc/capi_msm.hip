@@ -155,4 +155,16 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
     }
 }
 
+void kinet_hip_profile_enable(int on) { msm_set_profiling(on); }
+
+int kinet_hip_profile_collect(double* stage_ms, int* stage_calls, int nstages) {
+    if (!stage_ms || !stage_calls) return GPUKIT_ERR_NULL_ARG;
+    try {
+        return msm_collect_profile(stage_ms, stage_calls, nstages);
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
 }  // extern "C"

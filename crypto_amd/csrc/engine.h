@@ -18,6 +18,7 @@ struct Engine {
     MsmWorkspace ws;
     DevBuf h2d_a, h2d_b, pts, inf, out;
     DevBuf ntt_a, ntt_b, ntt_tw;
+    DevBuf gen_table[3];  // fixed-base tables, indexed by curve id
 };
 
 // Returns the engine, initialising HIP on first use; throws HipError when no

@@ -47,6 +47,11 @@ void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out);
 
+// Opt-in per-stage timing of msm_device (HIP events on the MSM's stream).
+enum MsmStage { kStageSort = 0, kStageAccumulate = 1, kStageReduce = 2, kStageCombine = 3, kStageCount = 4 };
+void msm_set_profiling(int on);
+int msm_collect_profile(double* sums_ms, int* counts, int nstages);
+
 // Compress an affine wire point on the device (one point).
 void compress_on_device(int curve, const uint8_t* d_wire, uint8_t* d_out, hipStream_t s);
 

@@ -1,5 +1,8 @@
 // Host side of the Pippenger MSM: workspace sizing, the launch sequence and
 // curve dispatch.  Kernels are in msm_impl.h / msm_digits.hip / scan.hip.
+#include <atomic>
+#include <vector>
+
 #include "msm_internal.h"
 #include "msm_stages.h"
 #include "scan.h"
@@ -15,6 +18,63 @@ int msm_default_window_bits(size_t n) {
     if (c < 5) c = 5;
     if (c > 16) c = 16;
     return c;
+}
+
+// ---------------------------------------------------------------- profiler
+// Opt-in per-stage timing: HIP events recorded on the MSM's own stream around
+// each stage, summed when collected.  Off by default (no events recorded).
+namespace {
+
+std::atomic<int> g_profiling{0};
+std::mutex g_prof_mu;
+struct StageEvents {
+    int stage;
+    hipEvent_t a, b;
+};
+std::vector<StageEvents> g_prof_events;
+
+struct StageTimer {
+    hipStream_t s;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageTimer(hipStream_t s_, int stage_) : s(s_), stage(stage_) {
+        if (!g_profiling.load()) return;
+        KZ_HIP(hipEventCreate(&a));
+        KZ_HIP(hipEventRecord(a, s));
+    }
+    ~StageTimer() {
+        if (!a) return;
+        hipEvent_t b;
+        if (hipEventCreate(&b) != hipSuccess) return;
+        (void)hipEventRecord(b, s);
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        g_prof_events.push_back({stage, a, b});
+    }
+};
+
+}  // namespace
+
+void msm_set_profiling(int on) { g_profiling.store(on); }
+
+int msm_collect_profile(double* sums_ms, int* counts, int nstages) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (int i = 0; i < nstages; ++i) {
+        sums_ms[i] = 0;
+        counts[i] = 0;
+    }
+    for (auto& ev : g_prof_events) {
+        KZ_HIP(hipEventSynchronize(ev.b));
+        float ms = 0;
+        KZ_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+        if (ev.stage < nstages) {
+            sums_ms[ev.stage] += ms;
+            counts[ev.stage] += 1;
+        }
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    g_prof_events.clear();
+    return 0;
 }
 
 namespace {
@@ -63,22 +123,36 @@ void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf,
 
     uint32_t* counts = ws.counts.as<uint32_t>();
     uint32_t* offsets = ws.offsets.as<uint32_t>();
-    KZ_HIP(hipMemsetAsync(counts, 0, NB * sizeof(uint32_t), s));
-    digits_dispatch(c, false, d_scalars, d_inf, (uint32_t)n, counts, nullptr, s);
-    device_exclusive_scan(counts, offsets, ws.cursor.as<uint32_t>(), NB, ws.block_sums.as<uint32_t>(),
-                          ws.total.as<uint32_t>(), s);
-    digits_dispatch(c, true, d_scalars, d_inf, (uint32_t)n, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
-    st.accumulate(ceil_div(max_chunks, 64), s, ws.entries.as<uint32_t>(), offsets, NB, chunk, d_pts,
-                  ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>());
+    {
+        StageTimer t(s, kStageSort);
+        KZ_HIP(hipMemsetAsync(counts, 0, NB * sizeof(uint32_t), s));
+        digits_dispatch(c, false, d_scalars, d_inf, (uint32_t)n, counts, nullptr, s);
+        device_exclusive_scan(counts, offsets, ws.cursor.as<uint32_t>(), NB, ws.block_sums.as<uint32_t>(),
+                              ws.total.as<uint32_t>(), s);
+        digits_dispatch(c, true, d_scalars, d_inf, (uint32_t)n, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>(),
+                        s);
+    }
+    {
+        StageTimer t(s, kStageAccumulate);
+        st.accumulate(ceil_div(max_chunks, 64), s, ws.entries.as<uint32_t>(), offsets, NB, chunk, d_pts,
+                      ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>());
+    }
     // A bucket holds at most n entries (one per point), so its carry run is at
     // most ceil(n / chunk) long: that many tree levels always suffice.
     uint32_t levels = 0;
     while (((size_t)1 << levels) < (n + chunk - 1) / chunk + 1) ++levels;
-    st.fold(s, offsets, NB, chunk, max_chunks, levels, ws.run_bucket.as<uint32_t>(), ws.carry.as<uint32_t>());
-    st.reduce_seg(ceil_div(nseg, 64), s, offsets, chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(), nseg,
-                  ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
-    st.reduce_window((unsigned)W, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), t1, ws.winres.as<uint32_t>());
-    st.combine(s, ws.winres.as<uint32_t>(), W, c, d_out);
+    {
+        StageTimer t(s, kStageReduce);
+        st.fold(s, offsets, NB, chunk, max_chunks, levels, ws.run_bucket.as<uint32_t>(), ws.carry.as<uint32_t>());
+        st.reduce_seg(ceil_div(nseg, 64), s, offsets, chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(), nseg,
+                      ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
+        st.reduce_window((unsigned)W, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), t1,
+                         ws.winres.as<uint32_t>());
+    }
+    {
+        StageTimer t(s, kStageCombine);
+        st.combine(s, ws.winres.as<uint32_t>(), W, c, d_out);
+    }
     KZ_HIP(hipGetLastError());
 }
 

@@ -64,6 +64,11 @@ def library():
             getattr(lib, name).restype = i32
         lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
         lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
+        lib.kinet_hip_g1_fixed_base_mul.argtypes = [u32, vp, sz, vp, vp]
+        lib.kinet_hip_valu_peak.argtypes = [i32, ctypes.POINTER(ctypes.c_double)]
+        lib.kinet_hip_profile_enable.argtypes = [i32]
+        lib.kinet_hip_profile_enable.restype = None
+        lib.kinet_hip_profile_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), i32]
         _lib = lib
     return _lib
 

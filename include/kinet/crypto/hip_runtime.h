@@ -36,6 +36,21 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                            size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
+/* Fixed-base batch scalar multiplication: out[i] = k_i * G for n scalars
+ * (32-byte LE, any 256-bit value) on the device, written in the point wire
+ * format (BE affine x || y, identity = zero bytes).  The table of multiples of
+ * G is built on first use per curve and stays resident.  Synthetic MSM inputs
+ * and SRS points [tau^i] G come from here.  gpukit_status codes. */
+int kinet_hip_g1_fixed_base_mul(uint32_t curve, const void* d_scalars, size_t n, void* d_out_wire, void* stream);
+
+/* Opt-in stage profiler of the MSM pipeline: while enabled, each MSM records
+ * HIP events on its own stream around its stages (0 digit sort, 1 bucket
+ * accumulation, 2 bucket reduction, 3 window combine + output).  Collect sums
+ * the elapsed milliseconds and call counts per stage (waiting for the events)
+ * and forgets them. */
+void kinet_hip_profile_enable(int on);
+int kinet_hip_profile_collect(double* stage_ms, int* stage_calls, int nstages);
+
 /* Measured VALU throughput of the current device, in instructions per second
  * summed over all lanes: op 0 v_mad_u64_u32 (the roofline denominator of the
  * MSM and NTT kernels), 1 v_mul_lo_u32 + add, 2 v_mul_hi_u32 + xor,

@@ -128,3 +128,24 @@ def test_msm_adversarial_buckets(ca, curve):
     assert ca.multi_pippenger(curve, sc * 2, g + negg) == bytes(pb)
     assert ca.multi_pippenger(curve, sc * 4, g * 4) == O.g1_mul_gen(
         curve, (4 * int.from_bytes(sc, "little") % r).to_bytes(32, "little"))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_fixed_base_generator(ca, curve):
+    """kinet_hip_g1_fixed_base_mul against the oracle's k * G."""
+    import torch
+    from crypto_amd import device as D
+
+    r = M.CURVES[curve]["r"]
+    rnd = random.Random(17 + curve)
+    ks = [0, 1, 2, 255, 256, r - 1, r, r + 1, (1 << 256) - 1] + [rnd.getrandbits(256) for _ in range(40)]
+    kb = b"".join(k.to_bytes(32, "little") for k in ks)
+    pb = ca.point_bytes(curve)
+    out = torch.empty(pb * len(ks), dtype=torch.uint8, device="cuda")
+    lib = ca.library()
+    assert lib.kinet_hip_g1_fixed_base_mul(curve, D.to_device(kb).data_ptr(), len(ks), out.data_ptr(),
+                                           D.stream_handle()) == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().tobytes()
+    for i, k in enumerate(ks):
+        assert got[pb * i: pb * (i + 1)] == O.g1_mul_gen(curve, k.to_bytes(32, "little")), (i, k)

@@ -333,3 +333,17 @@ def test_kzg_commit_closed_form(curve):
     ft = sum(e[j] * pow(w, j, r) * (pow(t, n, r) - 1) * pow(n * (t - pow(w, j, r)), r - 2, r) for j in range(n)) % r
     P = M.mul(curve, M.CURVES[curve]["g"], ft)
     assert c1 == M.compress(curve, P)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_package_scalar_generator_matches_oracle(curve):
+    """crypto_amd.inputs (numpy, used by the bench) draws the same scalars as
+    the oracle's C generator."""
+    from crypto_amd import inputs
+
+    for seed, n in ((0, 1), (7, 1000), (2 ** 63 + 5, 333)):
+        assert inputs.scalars_bytes(curve, seed, n) == O.gen_scalars(curve, seed, n)
+    x = inputs.scalars(curve, 3, 50)
+    r = M.CURVES[curve]["r"]
+    assert all(0 <= v < r for v in inputs.to_int(x))
+    assert inputs.dot_mod(curve, x, x) == sum(v * v for v in inputs.to_int(x)) % r
