@@ -41,6 +41,9 @@ def parse_args():
     ap.add_argument("--cpu-log2n", type=int, default=18, help="CPU-baseline sample size (log2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the BN254 and one-shot extra figures")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 256 MiB device copy "
+                         "(the HBM-counter calibration kernel), no JSON line")
     return ap.parse_args()
 
 
@@ -192,6 +195,15 @@ def main():
     peak = ctypes.c_double()
     lib.kinet_hip_valu_peak(0, ctypes.byref(peak))
     p_mad = peak.value
+
+    if args.profile_run:
+        timed_msm(lib, torch, D, curve, n, 20261203, 2, 0, args.window_bits, profile=False)
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("profile run done: 2 MSMs + 256 MiB copy", flush=True)
+        return
 
     ms, stage, result, check, ext = timed_msm(lib, torch, D, curve, n, 20261203, args.steps, args.warmup,
                                               args.window_bits, profile=True)

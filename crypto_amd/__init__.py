@@ -15,6 +15,7 @@ from .lib import (  # noqa: F401
     library,
     library_path,
     multi_pippenger,
+    ntt_fr,
     point_bytes,
     compressed_bytes,
 )

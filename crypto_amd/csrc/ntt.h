@@ -1,0 +1,21 @@
+// Host interface of the scalar-field NTT (BLS12-381 Fr, BN254 Fr).
+//
+// Domain: X_k = sum_j x_j w^(jk), w the primitive n-th root g^((r-1)/n) with
+// g = 7 (BLS12-381 Fr) or 5 (BN254 Fr); natural order in and out; the inverse
+// scales by n^-1.  Elements are 32-byte little-endian canonical integers, in
+// and out, which is also the MSM scalar wire format.
+#pragma once
+
+#include "common.h"
+
+namespace kz {
+
+constexpr int kNttMaxLog = 26;  // largest supported transform: 2^26 elements
+
+// Forward (inverse = false) or inverse transform of n = 2^logn elements, from
+// d_in to d_out (may alias) through the scratch buffer d_tmp (n * 32 bytes).
+// d_err (one int, zeroed by the caller) gets nonzero if an input is >= r.
+void ntt_device(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
+                int* d_err, hipStream_t s);
+
+}  // namespace kz

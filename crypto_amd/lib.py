@@ -66,6 +66,10 @@ def library():
         lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         lib.kinet_hip_g1_fixed_base_mul.argtypes = [u32, vp, sz, vp, vp]
         lib.kinet_hip_valu_peak.argtypes = [i32, ctypes.POINTER(ctypes.c_double)]
+        for name in ("ntt_fr_forward", "ntt_fr_inverse"):
+            getattr(lib, name).argtypes = [u32, ctypes.c_void_p, sz]
+            getattr(lib, name).restype = i32
+        lib.kinet_hip_ntt_fr.argtypes = [u32, i32, vp, vp, vp, sz, vp, vp]
         lib.kinet_hip_profile_enable.argtypes = [i32]
         lib.kinet_hip_profile_enable.restype = None
         lib.kinet_hip_profile_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), i32]
@@ -93,6 +97,18 @@ def point_bytes(curve: int) -> int:
 
 def compressed_bytes(curve: int) -> int:
     return 48 if curve == CURVE_BLS12_381_G1 else 32
+
+
+def ntt_fr(field: int, elems: bytes, inverse: bool = False) -> bytes:
+    """ntt_fr_forward / ntt_fr_inverse on 32-byte LE canonical elements."""
+    lib = require_device()
+    n = len(elems) // 32
+    buf = ctypes.create_string_buffer(bytes(elems), len(elems))
+    fn = lib.ntt_fr_inverse if inverse else lib.ntt_fr_forward
+    rc = fn(field, ctypes.cast(buf, ctypes.c_void_p), n)
+    if rc != 0:
+        raise KinetError(fn.__name__, rc, CRYPTO_STATUS)
+    return buf.raw[: len(elems)]
 
 
 def multi_pippenger(curve: int, scalars: bytes, points: bytes, backend: str = "dispatch") -> bytes:
