@@ -12,6 +12,8 @@ from .lib import (  # noqa: F401
     FIELD_BN254_FR,
     HipUnavailable,
     KinetError,
+    KzgSrs,
+    kzg_commit_batch,
     library,
     library_path,
     multi_pippenger,

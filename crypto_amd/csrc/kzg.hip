@@ -1,0 +1,291 @@
+// KZG commitments C = MSM(SRS, iNTT(evals)) on the device, and the SRS /
+// context handles of include/kinet/crypto/kzg.h.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "engine.h"
+#include "field.h"
+#include "kinet/crypto/hip_runtime.h"
+#include "kinet/crypto/kzg.h"
+#include "msm.h"
+#include "ntt.h"
+
+namespace kz {
+void fixed_base_mul(int curve, const uint32_t* d_scalars, size_t n, uint8_t* d_out_wire, hipStream_t s);
+}
+
+struct kzg_srs {
+    int curve = 0;
+    size_t n = 0;
+    kz::DevBuf pts;  // Montgomery affine records
+    kz::DevBuf inf;  // identity flags
+    kz::DevBuf wire; // wire points (kept for export)
+};
+
+struct kzg_ctx {
+    hipStream_t stream = nullptr;
+    kz::MsmWorkspace ws;
+    kz::DevBuf coeffs, tmp;
+};
+
+namespace kz {
+
+// out[i] = tau^i mod r (canonical, 8 LE words), i < n.
+template <class F>
+__global__ void __launch_bounds__(256) k_powers(const uint32_t* __restrict__ tau_words, uint32_t n,
+                                                uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Fe<F> base = fe_to_mont<F>(fe_from_words<F>(tau_words, 8)), acc = fe_one<F>();
+    for (uint32_t e = i; e; e >>= 1) {
+        if (e & 1u) acc = fe_mul<F>(acc, base);
+        base = fe_mul<F>(base, base);
+    }
+    fe_to_words<F>(fe_from_mont<F>(acc), out + (size_t)i * 8, 8);
+}
+
+static bool canonical_fr(int curve, const uint8_t* le32) {
+    // r of BLS12-381 / BN254 as big-endian bytes
+    static const uint8_t r_bls[32] = {0x73, 0xED, 0xA7, 0x53, 0x29, 0x9D, 0x7D, 0x48, 0x33, 0x39, 0xD8,
+                                      0x08, 0x09, 0xA1, 0xD8, 0x05, 0x53, 0xBD, 0xA4, 0x02, 0xFF, 0xFE,
+                                      0x5B, 0xFE, 0xFF, 0xFF, 0xFF, 0xFF, 0x00, 0x00, 0x00, 0x01};
+    static const uint8_t r_bn[32] = {0x30, 0x64, 0x4E, 0x72, 0xE1, 0x31, 0xA0, 0x29, 0xB8, 0x50, 0x45,
+                                     0xB6, 0x81, 0x81, 0x58, 0x5D, 0x28, 0x33, 0xE8, 0x48, 0x79, 0xB9,
+                                     0x70, 0x91, 0x43, 0xE1, 0xF5, 0x93, 0xF0, 0x00, 0x00, 0x01};
+    const uint8_t* r = curve == 2 ? r_bls : r_bn;
+    for (int k = 0; k < 32; ++k) {
+        const uint8_t b = le32[31 - k];
+        if (b != r[k]) return b < r[k];
+    }
+    return false;
+}
+
+static int log2_exact(size_t n) {
+    int l = 0;
+    while (((size_t)1 << l) < n) ++l;
+    return ((size_t)1 << l) == n ? l : -1;
+}
+
+static void srs_upload(kzg_srs* srs, const uint8_t* d_wire_src, bool src_on_host, hipStream_t s) {
+    const size_t pb = point_wire_bytes(srs->curve);
+    srs->wire.ensure(pb * srs->n);
+    srs->pts.ensure((size_t)affine_words(srs->curve) * 4 * srs->n);
+    srs->inf.ensure(srs->n);
+    if (src_on_host)
+        KZ_HIP(hipMemcpyAsync(srs->wire.ptr, d_wire_src, pb * srs->n, hipMemcpyHostToDevice, s));
+    points_from_wire(srs->curve, srs->wire.as<uint8_t>(), srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), srs->n,
+                     s);
+}
+
+// iNTT of the evaluations into ctx->coeffs, then the MSM against the SRS.
+static void commit_device(kzg_ctx* ctx, const kzg_srs* srs, const uint32_t* d_evals, size_t n, int window_bits,
+                          uint8_t* d_out, int* d_err) {
+    const int logn = log2_exact(n);
+    uint32_t* coeffs = static_cast<uint32_t*>(ctx->coeffs.ensure(32 * n));
+    uint32_t* tmp = static_cast<uint32_t*>(ctx->tmp.ensure(32 * n));
+    const int field = srs->curve == 2 ? 0 : 1;
+    ntt_device(field, logn, true, d_evals, coeffs, tmp, d_err, ctx->stream);
+    MsmParams prm;
+    prm.c = window_bits;
+    msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), coeffs, n, prm, ctx->ws, ctx->stream,
+               d_out);
+}
+
+static kzg_ctx* default_ctx() {
+    Engine& e = engine();
+    static kzg_ctx* ctx = [&e] {
+        kzg_ctx* c = new kzg_ctx();
+        c->stream = e.stream;
+        return c;
+    }();
+    return ctx;
+}
+
+}  // namespace kz
+
+using namespace kz;
+
+extern "C" {
+
+int kzg_srs_create(uint32_t curve, const uint8_t* points, size_t n, kzg_srs** out) {
+    if (!out || (curve != 1 && curve != 2) || n == 0 || !points) return KZ_CRYPTO_ERR_INPUT;
+    *out = nullptr;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        auto srs = std::make_unique<kzg_srs>();
+        srs->curve = (int)curve;
+        srs->n = n;
+        srs_upload(srs.get(), points, true, e.stream);
+        KZ_HIP(hipStreamSynchronize(e.stream));
+        *out = srs.release();
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+int kzg_srs_generate(uint32_t curve, const uint8_t tau[32], size_t n, kzg_srs** out) {
+    if (!out || !tau || (curve != 1 && curve != 2) || n == 0 || n > ((size_t)1 << 28)) return KZ_CRYPTO_ERR_INPUT;
+    if (!canonical_fr((int)curve, tau)) return KZ_CRYPTO_ERR_INPUT;
+    *out = nullptr;
+    try {
+        Engine& e = engine();
+        hipStream_t s = e.stream;
+        auto srs = std::make_unique<kzg_srs>();
+        srs->curve = (int)curve;
+        srs->n = n;
+        DevBuf d_tau, d_pow;
+        uint32_t tw[8];
+        std::memcpy(tw, tau, 32);
+        d_tau.ensure(32);
+        d_pow.ensure(32 * n);
+        KZ_HIP(hipMemcpyAsync(d_tau.ptr, tw, 32, hipMemcpyHostToDevice, s));
+        if (curve == 2)
+            hipLaunchKernelGGL(k_powers<BlsFr>, dim3(ceil_div(n, 256)), dim3(256), 0, s, d_tau.as<uint32_t>(),
+                               (uint32_t)n, d_pow.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_powers<BnFr>, dim3(ceil_div(n, 256)), dim3(256), 0, s, d_tau.as<uint32_t>(),
+                               (uint32_t)n, d_pow.as<uint32_t>());
+        KZ_HIP(hipGetLastError());
+        srs->wire.ensure(point_wire_bytes((int)curve) * n);
+        fixed_base_mul((int)curve, d_pow.as<uint32_t>(), n, srs->wire.as<uint8_t>(), s);
+        {
+            std::lock_guard<std::mutex> g(e.mu);
+            srs_upload(srs.get(), nullptr, false, s);
+        }
+        KZ_HIP(hipStreamSynchronize(s));
+        *out = srs.release();
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+void kzg_srs_destroy(kzg_srs* srs) { delete srs; }
+
+size_t kzg_srs_size(const kzg_srs* srs) { return srs ? srs->n : 0; }
+
+uint32_t kzg_srs_curve(const kzg_srs* srs) { return srs ? (uint32_t)srs->curve : 0; }
+
+int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* points_out) {
+    if (!srs || !points_out || first + count > srs->n) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        const size_t pb = point_wire_bytes(srs->curve);
+        KZ_HIP(hipMemcpy(points_out, srs->wire.as<uint8_t>() + pb * first, pb * count, hipMemcpyDeviceToHost));
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+static int commit_host(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out) {
+    if (!srs || !evals || !commit_out) return KZ_CRYPTO_ERR_INPUT;
+    if (n == 0 || n > srs->n || log2_exact(n) < 0 || log2_exact(n) > (srs->curve == 2 ? 32 : 28))
+        return KZ_CRYPTO_ERR_INPUT;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        kzg_ctx* ctx = default_ctx();
+        hipStream_t s = ctx->stream;
+        uint32_t* d_ev = static_cast<uint32_t*>(e.h2d_a.ensure(32 * n));
+        uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
+        int* d_err = reinterpret_cast<int*>(d_out + msm_out_bytes(srs->curve));
+        KZ_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+        KZ_HIP(hipMemcpyAsync(d_ev, evals, 32 * n, hipMemcpyHostToDevice, s));
+        commit_device(ctx, srs, d_ev, n, 0, d_out, d_err);
+        uint8_t host[96 + 48 + 16];
+        KZ_HIP(hipMemcpyAsync(host, d_out, msm_out_bytes(srs->curve) + sizeof(int), hipMemcpyDeviceToHost, s));
+        KZ_HIP(hipStreamSynchronize(s));
+        int err;
+        std::memcpy(&err, host + msm_out_bytes(srs->curve), sizeof(int));
+        if (err) return KZ_CRYPTO_ERR_INPUT;
+        std::memcpy(commit_out, host + point_wire_bytes(srs->curve), compressed_bytes(srs->curve));
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+int kzg_commit(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out) {
+    return commit_host(srs, evals, n, commit_out);
+}
+
+int kzg_commit_batch(const kzg_srs* const* srs, const uint8_t* const* evals, const size_t* n, size_t count,
+                     uint8_t* commits_out) {
+    if (count == 0) return KZ_CRYPTO_OK;
+    if (!srs || !evals || !n || !commits_out || !srs[0]) return KZ_CRYPTO_ERR_INPUT;
+    const int curve = srs[0]->curve;
+    const size_t cb = compressed_bytes(curve);
+    for (size_t i = 0; i < count; ++i)
+        if (!srs[i] || srs[i]->curve != curve) return KZ_CRYPTO_ERR_INPUT;
+    for (size_t i = 0; i < count; ++i) {
+        const int rc = commit_host(srs[i], evals[i], n[i], commits_out + cb * i);
+        if (rc != KZ_CRYPTO_OK) return rc;
+    }
+    return KZ_CRYPTO_OK;
+}
+
+int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* result) {
+    if (!srs || !result || (n > 0 && !scalars) || n > srs->n) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        const size_t pb = point_wire_bytes(srs->curve);
+        if (n == 0) {
+            std::memset(result, 0, pb);
+            return KZ_CRYPTO_OK;
+        }
+        hipStream_t s = e.stream;
+        uint32_t* d_sc = static_cast<uint32_t*>(e.h2d_a.ensure(32 * n));
+        uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
+        KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));
+        MsmParams prm;
+        msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
+        KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
+        KZ_HIP(hipStreamSynchronize(s));
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+int kzg_ctx_create(void* stream, kzg_ctx** out) {
+    if (!out) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        (void)engine();
+        kzg_ctx* c = new kzg_ctx();
+        c->stream = static_cast<hipStream_t>(stream);
+        *out = c;
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+void kzg_ctx_destroy(kzg_ctx* ctx) { delete ctx; }
+
+int kzg_commit_async(kzg_ctx* ctx, const kzg_srs* srs, const void* d_evals, size_t n, int window_bits, void* d_out,
+                     int* d_err) {
+    if (!ctx || !srs || !d_evals || !d_out || !d_err) return KZ_CRYPTO_ERR_INPUT;
+    if (n == 0 || n > srs->n || log2_exact(n) < 0 || log2_exact(n) > (srs->curve == 2 ? 32 : 28))
+        return KZ_CRYPTO_ERR_INPUT;
+    if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        commit_device(ctx, srs, static_cast<const uint32_t*>(d_evals), n, window_bits, static_cast<uint8_t*>(d_out),
+                      d_err);
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+}  // extern "C"

@@ -70,6 +70,24 @@ def library():
             getattr(lib, name).argtypes = [u32, ctypes.c_void_p, sz]
             getattr(lib, name).restype = i32
         lib.kinet_hip_ntt_fr.argtypes = [u32, i32, vp, vp, vp, sz, vp, vp]
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        lib.kzg_srs_create.argtypes = [u32, ctypes.c_char_p, sz, pp]
+        lib.kzg_srs_generate.argtypes = [u32, ctypes.c_char_p, sz, pp]
+        lib.kzg_srs_destroy.argtypes = [vp]
+        lib.kzg_srs_destroy.restype = None
+        lib.kzg_srs_size.argtypes = [vp]
+        lib.kzg_srs_size.restype = sz
+        lib.kzg_srs_curve.argtypes = [vp]
+        lib.kzg_srs_curve.restype = u32
+        lib.kzg_srs_export.argtypes = [vp, sz, sz, ctypes.c_char_p]
+        lib.kzg_commit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p]
+        lib.kzg_commit_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(sz), sz,
+                                         ctypes.c_char_p]
+        lib.kzg_msm.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p]
+        lib.kzg_ctx_create.argtypes = [vp, pp]
+        lib.kzg_ctx_destroy.argtypes = [vp]
+        lib.kzg_ctx_destroy.restype = None
+        lib.kzg_commit_async.argtypes = [vp, vp, vp, sz, i32, vp, vp]
         lib.kinet_hip_profile_enable.argtypes = [i32]
         lib.kinet_hip_profile_enable.restype = None
         lib.kinet_hip_profile_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), i32]
@@ -109,6 +127,79 @@ def ntt_fr(field: int, elems: bytes, inverse: bool = False) -> bytes:
     if rc != 0:
         raise KinetError(fn.__name__, rc, CRYPTO_STATUS)
     return buf.raw[: len(elems)]
+
+
+class KzgSrs:
+    """Device-resident SRS (kzg_srs_create / kzg_srs_generate)."""
+
+    def __init__(self, handle, curve: int, n: int):
+        self.handle, self.curve, self.n = handle, curve, n
+
+    @classmethod
+    def from_points(cls, curve: int, points: bytes) -> "KzgSrs":
+        lib = require_device()
+        n = len(points) // point_bytes(curve)
+        h = ctypes.c_void_p()
+        rc = lib.kzg_srs_create(curve, points, n, ctypes.byref(h))
+        if rc != 0:
+            raise KinetError("kzg_srs_create", rc, CRYPTO_STATUS)
+        return cls(h, curve, n)
+
+    @classmethod
+    def generate(cls, curve: int, tau: int, n: int) -> "KzgSrs":
+        lib = require_device()
+        h = ctypes.c_void_p()
+        rc = lib.kzg_srs_generate(curve, tau.to_bytes(32, "little"), n, ctypes.byref(h))
+        if rc != 0:
+            raise KinetError("kzg_srs_generate", rc, CRYPTO_STATUS)
+        return cls(h, curve, n)
+
+    def export(self, first: int = 0, count: int | None = None) -> bytes:
+        count = self.n - first if count is None else count
+        out = ctypes.create_string_buffer(point_bytes(self.curve) * count)
+        rc = library().kzg_srs_export(self.handle, first, count, out)
+        if rc != 0:
+            raise KinetError("kzg_srs_export", rc, CRYPTO_STATUS)
+        return out.raw
+
+    def commit(self, evals: bytes) -> bytes:
+        out = ctypes.create_string_buffer(compressed_bytes(self.curve))
+        rc = library().kzg_commit(self.handle, evals, len(evals) // 32, out)
+        if rc != 0:
+            raise KinetError("kzg_commit", rc, CRYPTO_STATUS)
+        return out.raw
+
+    def msm(self, scalars: bytes) -> bytes:
+        out = ctypes.create_string_buffer(point_bytes(self.curve))
+        rc = library().kzg_msm(self.handle, scalars, len(scalars) // 32, out)
+        if rc != 0:
+            raise KinetError("kzg_msm", rc, CRYPTO_STATUS)
+        return out.raw
+
+    def close(self):
+        if self.handle:
+            library().kzg_srs_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kzg_commit_batch(srs_list, evals_list) -> list[bytes]:
+    lib = require_device()
+    count = len(srs_list)
+    arr_srs = (ctypes.c_void_p * count)(*[s.handle for s in srs_list])
+    arr_ev = (ctypes.c_char_p * count)(*evals_list)
+    arr_n = (ctypes.c_size_t * count)(*[len(e) // 32 for e in evals_list])
+    cb = compressed_bytes(srs_list[0].curve) if count else 0
+    out = ctypes.create_string_buffer(max(1, cb * count))
+    rc = lib.kzg_commit_batch(arr_srs, arr_ev, arr_n, count, out)
+    if rc != 0:
+        raise KinetError("kzg_commit_batch", rc, CRYPTO_STATUS)
+    return [out.raw[cb * i: cb * (i + 1)] for i in range(count)]
 
 
 def multi_pippenger(curve: int, scalars: bytes, points: bytes, backend: str = "dispatch") -> bytes:

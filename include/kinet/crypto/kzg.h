@@ -1,0 +1,69 @@
+/*
+ * KZG polynomial commitments on AMD MI355X (gfx950): C = [f(tau)] G with
+ * f given by its evaluations, computed as C = MSM(SRS, iNTT(evals)).
+ *
+ * The reference declares kzg_blob_to_commit (c-abi/crypto.h:238), which
+ * delegates to c-kzg-4844 (absent, and its translation unit does not compile:
+ * kzg/cpp/kzg_blob.cpp:18).  Its documentation describes, but does not
+ * implement, an SRS-handle API (docs/content/docs/api/kzg.mdx:289-331:
+ * kzg_srs_t, kzg_commit).  These entries implement that API on the device;
+ * names and conventions follow it and c-abi/crypto.h.
+ *
+ * Curves: 1 = BN254 G1, 2 = BLS12-381 G1 (gpukit_curve numbering).
+ * SRS points: monomial basis [tau^i] G, wire format of gpukit_multi_pippenger
+ * (BE affine x || y, 64 or 96 bytes).  They stay resident on the device.
+ * Evaluations: n x 32 bytes LE canonical Fr (< r), f(w^j) for the n-th root of
+ * unity w of ntt.h (natural order), n a power of two <= SRS size.
+ * Commitments: compressed G1 -- BLS12-381 48 bytes (ZCash/IETF format: 0x80
+ * compressed flag, 0x40 infinity, 0x20 larger y); BN254 32 bytes (big-endian
+ * x; top bits 0b10 smaller y, 0b11 larger y, 0b01 infinity).
+ * Returns CRYPTO_OK (0) or CRYPTO_ERR_* (c-abi/crypto.h:60-66): INPUT for bad
+ * arguments or non-canonical evaluations, BACKEND for device failures.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kzg_srs kzg_srs; /* opaque: device-resident SRS points */
+typedef struct kzg_ctx kzg_ctx; /* opaque: a stream plus its scratch */
+
+/* SRS from n wire points (host memory). */
+int kzg_srs_create(uint32_t curve, const uint8_t* points, size_t n, kzg_srs** out);
+/* Synthetic SRS [tau^i] G, i < n, generated on the device (tau: 32-byte LE, < r). */
+int kzg_srs_generate(uint32_t curve, const uint8_t tau[32], size_t n, kzg_srs** out);
+void kzg_srs_destroy(kzg_srs* srs);
+size_t kzg_srs_size(const kzg_srs* srs);
+uint32_t kzg_srs_curve(const kzg_srs* srs);
+/* Copies points [first, first + count) back to the host in wire format. */
+int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* points_out);
+
+/* Commitment to the polynomial with the given evaluations (host memory, synchronous). */
+int kzg_commit(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out);
+/* count independent commitments, each against its own SRS; commits_out holds
+ * count compressed points back to back (48 or 32 bytes each; every SRS of one
+ * call must be on the same curve). */
+int kzg_commit_batch(const kzg_srs* const* srs, const uint8_t* const* evals, const size_t* n, size_t count,
+                     uint8_t* commits_out);
+/* MSM of n scalars (32-byte LE, host memory) against the first n SRS points,
+ * result in wire format (64 or 96 bytes). */
+int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* result);
+
+/* Contexts for device-resident, asynchronous use (batching, serving): each
+ * owns scratch and runs on `stream` (hipStream_t; NULL = null stream).  Calls
+ * on one context are ordered; different contexts may run concurrently. */
+int kzg_ctx_create(void* stream, kzg_ctx** out);
+void kzg_ctx_destroy(kzg_ctx* ctx);
+/* d_evals: n x 32 bytes on the device; d_out receives the wire point followed
+ * by its compressed form (96 + 48 or 64 + 32 bytes).  *d_err (device int,
+ * zeroed by the caller) becomes nonzero if an evaluation is >= r.
+ * window_bits: 0 = default. */
+int kzg_commit_async(kzg_ctx* ctx, const kzg_srs* srs, const void* d_evals, size_t n, int window_bits,
+                     void* d_out, int* d_err);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

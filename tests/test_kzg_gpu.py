@@ -1,0 +1,88 @@
+"""GPU parity of the KZG commit path (iNTT -> MSM -> compress) against the
+oracle and the closed form [f(tau)] G."""
+import random
+
+import pytest
+
+import oracle_lib as O
+import pymath as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ca():
+    import crypto_amd
+
+    crypto_amd.lib.require_device()
+    return crypto_amd
+
+
+def _evals(curve, n, seed):
+    return O.gen_scalars(curve, seed, n)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_srs_generate_matches_oracle(ca, curve):
+    tau = int.from_bytes(O.gen_scalars(curve, 5, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, 64)
+    want = O.gen_points(curve, O.powers(curve, tau.to_bytes(32, "little"), 64))
+    assert srs.export() == want
+    assert srs.export(10, 5) == want[10 * len(want) // 64: 15 * len(want) // 64]
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("n", [1, 2, 64, 1024, 4096])
+def test_commit_matches_oracle(ca, curve, n):
+    tau = int.from_bytes(O.gen_scalars(curve, 11, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, 4096)
+    ev = _evals(curve, n, 1000 + n)
+    print(f"kzg curve={curve} n={n}", flush=True)
+    got = srs.commit(ev)
+    assert got == O.kzg_commit(curve, srs.export(0, n), ev)
+    assert got == O.kzg_commit_closed_form(curve, tau.to_bytes(32, "little"), ev)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_commit_closed_form_large(ca, curve):
+    n = 1 << 16
+    tau = int.from_bytes(O.gen_scalars(curve, 12, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, n)
+    ev = _evals(curve, n, 77)
+    assert srs.commit(ev) == O.kzg_commit_closed_form(curve, tau.to_bytes(32, "little"), ev)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_commit_special_polynomials(ca, curve):
+    n = 256
+    tau = int.from_bytes(O.gen_scalars(curve, 13, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, n)
+    g = O.generator(curve)
+    # constant polynomial 1: commitment = G; zero polynomial: infinity
+    assert srs.commit((1).to_bytes(32, "little") * n) == O.compress(curve, g)
+    assert srs.commit(bytes(32 * n)) == O.compress(curve, bytes(len(g)))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_batch_and_msm(ca, curve):
+    tau = int.from_bytes(O.gen_scalars(curve, 14, 1), "little")
+    a = ca.KzgSrs.generate(curve, tau, 1024)
+    b = ca.KzgSrs.generate(curve, tau + 1, 512)
+    evs = [_evals(curve, 1024, 1), _evals(curve, 512, 2), _evals(curve, 128, 3)]
+    got = ca.kzg_commit_batch([a, b, a], evs)
+    assert got == [a.commit(evs[0]), b.commit(evs[1]), a.commit(evs[2])]
+    sc = O.gen_scalars(curve, 15, 300)
+    assert a.msm(sc) == O.msm(curve, sc, a.export(0, 300))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_commit_rejects_bad_input(ca, curve):
+    srs = ca.KzgSrs.generate(curve, 3, 64)
+    r = M.CURVES[curve]["r"]
+    bad = r.to_bytes(32, "little") + bytes(32 * 63)
+    for ev in (bad, bytes(32 * 3), bytes(32 * 128)):
+        with pytest.raises(ca.KinetError) as e:
+            srs.commit(ev)
+        assert e.value.code == -1
+    with pytest.raises(ca.KinetError):
+        ca.KzgSrs.generate(curve, r, 8)
