@@ -167,6 +167,8 @@ int kzg_srs_generate(uint32_t curve, const uint8_t tau[32], size_t n, kzg_srs** 
 
 void kzg_srs_destroy(kzg_srs* srs) { delete srs; }
 
+void kzg_srs_free(kzg_srs* srs) { delete srs; }
+
 size_t kzg_srs_size(const kzg_srs* srs) { return srs ? srs->n : 0; }
 
 uint32_t kzg_srs_curve(const kzg_srs* srs) { return srs ? (uint32_t)srs->curve : 0; }
@@ -248,6 +250,29 @@ int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* resul
         MsmParams prm;
         msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
+        KZ_HIP(hipStreamSynchronize(s));
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+int kzg_commit_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, uint8_t* commit_out) {
+    if (!srs || !coeffs || !commit_out || n == 0 || n > srs->n) return KZ_CRYPTO_ERR_INPUT;
+    for (size_t i = 0; i < n; ++i)
+        if (!canonical_fr(srs->curve, coeffs + 32 * i)) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        hipStream_t s = e.stream;
+        uint32_t* d_sc = static_cast<uint32_t*>(e.h2d_a.ensure(32 * n));
+        uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
+        KZ_HIP(hipMemcpyAsync(d_sc, coeffs, 32 * n, hipMemcpyHostToDevice, s));
+        MsmParams prm;
+        msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
+        KZ_HIP(hipMemcpyAsync(commit_out, d_out + point_wire_bytes(srs->curve), compressed_bytes(srs->curve),
+                              hipMemcpyDeviceToHost, s));
         KZ_HIP(hipStreamSynchronize(s));
         return KZ_CRYPTO_OK;
     } catch (const std::exception& ex) {

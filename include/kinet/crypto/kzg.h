@@ -43,6 +43,13 @@ int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* poin
 
 /* Commitment to the polynomial with the given evaluations (host memory, synchronous). */
 int kzg_commit(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out);
+/* Commitment from coefficients c_i (f(X) = sum c_i X^i): MSM + compression
+ * only, the form documented for the reference's planned C API
+ * (docs/content/docs/api/kzg.mdx:289-331, `kzg_commit(srs, coeffs, ...)`). */
+int kzg_commit_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, uint8_t* commit_out);
+/* Alias of kzg_srs_destroy under the documented name (kzg.mdx: kzg_srs_free). */
+void kzg_srs_free(kzg_srs* srs);
+
 /* count independent commitments, each against its own SRS; commits_out holds
  * count compressed points back to back (48 or 32 bytes each; every SRS of one
  * call must be on the same curve). */

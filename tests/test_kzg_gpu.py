@@ -86,3 +86,18 @@ def test_commit_rejects_bad_input(ca, curve):
         assert e.value.code == -1
     with pytest.raises(ca.KinetError):
         ca.KzgSrs.generate(curve, r, 8)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_commit_coeffs_matches_evals_path(ca, curve):
+    import ctypes
+
+    n = 512
+    srs = ca.KzgSrs.generate(curve, 99, n)
+    ev = _evals(curve, n, 5)
+    coeffs = O.ntt(0 if curve == 2 else 1, ev, inverse=True)
+    out = ctypes.create_string_buffer(ca.compressed_bytes(curve))
+    lib = ca.library()
+    lib.kzg_commit_coeffs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p]
+    assert lib.kzg_commit_coeffs(srs.handle, coeffs, n, out) == 0
+    assert out.raw == srs.commit(ev)
