@@ -41,6 +41,7 @@ def parse_args():
     ap.add_argument("--cpu-log2n", type=int, default=18, help="CPU-baseline sample size (log2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the BN254 and one-shot extra figures")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-versus-plain job set")
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 256 MiB device copy "
                          "(the HBM-counter calibration kernel), no JSON line")
@@ -165,6 +166,138 @@ def cpu_baseline(curve, scalars, points_wire, log2n):
             "seconds": round(dt, 2)}
 
 
+def batched_set(lib, torch, D, reps: int, seed: int = 4242):
+    """The batched workload: 256 BN254 MSMs with sizes floor(2^U), U uniform on
+    [10, 18], over one shared 2^18-point SRS prefix with their own scalars, plus
+    256 BN254-Fr NTTs of sizes 2^K, K uniform on {12..20}.  Runs the whole set
+    (a) plainly -- one MSM pipeline per MSM, one transform per NTT, in
+    submission order -- and (b) batched -- one batch pipeline for all MSMs
+    (replayed from a HIP graph) and one launch sequence per NTT size.  Checks
+    both give identical bytes; returns set throughput and per-job completion
+    latency (from the set's start) for each."""
+    import random
+
+    import numpy as np
+
+    from crypto_amd import inputs
+
+    curve, field = 1, 1
+    rnd = random.Random(seed)
+    msm_n = [int(2 ** rnd.uniform(10, 18)) for _ in range(256)]
+    ntt_log = [rnd.randint(12, 20) for _ in range(256)]
+    npts = 1 << 18
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ks = inputs.scalars(curve, seed, npts)
+        d_k = torch.from_numpy(ks.view("u1").reshape(-1).copy()).to("cuda")
+        pp = D.PreparedPoints(curve, gpu_fixed_base(lib, torch, curve, d_k, npts, stream))
+        total = sum(msm_n)
+        sc = inputs.scalars(curve, seed + 1, total)
+        d_sc = torch.from_numpy(sc.view("u1").reshape(-1).copy()).to("cuda")
+        sc_off = np.concatenate([[0], np.cumsum(msm_n)[:-1]]).tolist()
+        rec = 96
+        out_plain = torch.zeros(256 * rec, dtype=torch.uint8, device="cuda")
+        out_batch = torch.zeros_like(out_plain)
+        batch = D.MsmBatch(curve, msm_n)
+        # NTT inputs: canonical random elements (top byte < 0x20, so < 2^253 < r),
+        # jobs of one size contiguous so that a size group is one batch
+        order = sorted(range(256), key=lambda j: ntt_log[j])
+        off_of, off = {}, 0
+        for j in order:
+            off_of[j] = off
+            off += 1 << ntt_log[j]
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        d_in = torch.randint(0, 256, (off * 32,), dtype=torch.uint8, device="cuda", generator=gen)
+        d_in.view(-1, 32)[:, 31] &= 0x1F
+        d_out_plain = torch.empty_like(d_in)
+        d_out_batch = torch.empty_like(d_in)
+        d_tmp = torch.empty_like(d_in)
+        d_err = torch.zeros(1, dtype=torch.int32, device="cuda")
+        groups = {}
+        for j in order:
+            groups.setdefault(ntt_log[j], []).append(j)
+        sh = stream.cuda_stream
+
+        def run_plain(events=None):
+            for j in range(256):
+                rc = lib.kinet_hip_msm_prepared(curve, d_sc.data_ptr() + 32 * sc_off[j], pp.pts.data_ptr(),
+                                                pp.inf.data_ptr(), msm_n[j], 0, 0,
+                                                out_plain.data_ptr() + rec * j, sh)
+                assert rc == 0, rc
+                if events is not None:
+                    events[("msm", j)].record(stream)
+                o = 32 * off_of[j]
+                rc = lib.kinet_hip_ntt_fr(field, 0, d_in.data_ptr() + o, d_out_plain.data_ptr() + o,
+                                          d_tmp.data_ptr() + o, 1 << ntt_log[j], d_err.data_ptr(), sh)
+                assert rc == 0, rc
+                if events is not None:
+                    events[("ntt", j)].record(stream)
+
+        def run_batched(events=None):
+            batch.run(pp, d_sc, out=out_batch, use_graph=True, stream=stream, sync=False)
+            if events is not None:
+                events["msm"].record(stream)
+            for lg, jobs in sorted(groups.items()):
+                o = 32 * off_of[jobs[0]]
+                rc = lib.kinet_hip_ntt_fr_batch(field, 0, d_in.data_ptr() + o, d_out_batch.data_ptr() + o,
+                                                d_tmp.data_ptr() + o, 1 << lg, len(jobs), d_err.data_ptr(), sh)
+                assert rc == 0, rc
+                if events is not None:
+                    events[("ntt", lg)].record(stream)
+
+        def timed(fn, keys):
+            best = None
+            for _ in range(reps):
+                ev = {k: torch.cuda.Event(enable_timing=True) for k in keys}
+                e0 = torch.cuda.Event(enable_timing=True)
+                stream.synchronize()
+                e0.record(stream)
+                fn(ev)
+                stream.synchronize()
+                lat = {k: e0.elapsed_time(e) for k, e in ev.items()}
+                tot = max(lat.values())
+                if best is None or tot < best[0]:
+                    best = (tot, lat)
+            return best
+
+        run_plain()
+        run_batched()  # warm up: twiddle tables, graph capture
+        stream.synchronize()
+        plain_keys = [("msm", j) for j in range(256)] + [("ntt", j) for j in range(256)]
+        batch_keys = ["msm"] + [("ntt", lg) for lg in groups]
+        t_plain, lat_plain = timed(run_plain, plain_keys)
+        t_batch, lat_batch = timed(run_batched, batch_keys)
+        same = bool(torch.equal(out_plain, out_batch)) and bool(torch.equal(d_out_plain, d_out_batch))
+        err = int(d_err.item())
+    per_job_plain = sorted(lat_plain.values())
+    per_job_batch = sorted([lat_batch["msm"]] * 256 + [lat_batch[("ntt", ntt_log[j])] for j in range(256)])
+
+    def pct(v, p):
+        return v[min(len(v) - 1, int(round(p / 100 * (len(v) - 1))))]
+
+    batch.close()
+    return {
+        "workload": "256 BN254 G1 MSMs, n = floor(2^U), U~U[10,18], shared 2^18-point SRS prefix "
+                    f"({total} points total) + 256 BN254-Fr NTTs, n = 2^K, K~U{{12..20}} ({off} elements total)",
+        "jobs": 512,
+        "plain_ms": round(t_plain, 3),
+        "batched_ms": round(t_batch, 3),
+        "plain_jobs_per_s": round(512 / t_plain * 1e3, 1),
+        "batched_jobs_per_s": round(512 / t_batch * 1e3, 1),
+        "speedup": round(t_plain / t_batch, 3),
+        "msm_part_ms": {"plain": round(sum(lat_plain[("msm", j)] - (lat_plain[("ntt", j - 1)] if j else 0.0)
+                                           for j in range(256)), 3),
+                        "batched": round(lat_batch["msm"], 3)},
+        "ntt_part_ms": {"plain": round(sum(lat_plain[("ntt", j)] - lat_plain[("msm", j)] for j in range(256)), 3),
+                        "batched": round(t_batch - lat_batch["msm"], 3)},
+        "p50_ms": {"plain": round(pct(per_job_plain, 50), 3), "batched": round(pct(per_job_batch, 50), 3)},
+        "p99_ms": {"plain": round(pct(per_job_plain, 99), 3), "batched": round(pct(per_job_batch, 99), 3)},
+        "bytes_identical": same and err == 0,
+        "window_bits": "per job from the library's window table (default c = floor(log2 n) - 4, clamped to [5, 16])",
+    }
+
+
 def load_traffic(curve, log2n):
     """HBM bytes per accumulation launch from the committed rocprof summary."""
     path = os.path.join(ROOT, "profiles", f"msm_accumulate_{CURVE_NAMES[curve]}_2p{log2n}_traffic.txt")
@@ -260,6 +393,8 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(curve, ext["scalars"], ext["points_wire"], args.cpu_log2n)
+    if not args.no_batched:
+        line["batched"] = batched_set(lib, torch, D, reps=3)
     if not args.no_extras:
         extras = {}
         other = 1 if curve == 2 else 2

@@ -1,6 +1,8 @@
 // Host interface of the HIP Pippenger MSM pipeline.
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 namespace kz {
@@ -16,6 +18,21 @@ struct MsmWorkspace {
 struct MsmParams {
     int c = 0;       // window bits; 0 = choose from n
     int chunk = 32;  // sorted entries per accumulation thread
+};
+
+// Device descriptor of one MSM inside a batch.  A batch runs every job's
+// stages together: global bucket ids, windows and outputs are the jobs'
+// ranges laid end to end.
+struct MsmJob {
+    uint64_t sc_base;   // first scalar (index into the batch's scalar array)
+    uint32_t n;         // points
+    uint32_t pt_base;   // first point (index into the point array)
+    uint32_t c;         // window bits
+    uint32_t nwin;      // windows
+    uint32_t win_base;  // first global window
+    uint32_t gb_base;   // first global bucket
+    uint32_t blk_base;  // first block of the digit kernels
+    uint32_t pad;
 };
 
 int msm_default_window_bits(size_t n);
@@ -46,6 +63,42 @@ void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t
 // d_scalars: n x 32-byte LE.  Writes msm_out_bytes(curve) bytes at d_out.
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out);
+
+// ------------------------------------------------------------------ batches
+// A batch of independent MSMs over one point array and one scalar array, run
+// as one pipeline: every stage is launched once for all jobs (about 12
+// launches whatever the job count), and the per-job window combines -- each
+// a serial chain of ~256 doublings -- run side by side.
+struct MsmBatchPlan {
+    int curve = 0;
+    uint32_t njobs = 0;
+    uint32_t nb = 0;          // global buckets
+    uint32_t nwin = 0;        // global windows
+    uint32_t nseg = 0;        // first-level reduction segments
+    uint32_t dig_blocks = 0;  // blocks of the digit kernels
+    uint32_t chunk = 32;
+    uint32_t levels = 0;      // carry-fold tree levels
+    size_t max_entries = 0;
+    std::vector<MsmJob> jobs;  // host copy of the descriptors
+    DevBuf d_jobs, d_win_seg;
+};
+
+// Window bits used for an n-point MSM: the installed tuning table when one is
+// set for (curve, batched), else msm_default_window_bits(n).
+int msm_window_bits(int curve, size_t n, bool batched);
+// Installs a tuning table: c_by_log[k] = window bits for 2^k <= n < 2^(k+1)
+// (0 = default rule), k < count <= 32.  count = 0 clears it.
+void msm_set_window_table(int curve, bool batched, const int* c_by_log, int count);
+
+// Builds a plan (synchronous upload of the descriptor tables).  Each spec
+// gives n, pt_base, sc_base and c (0 = msm_window_bits(curve, n, true)).
+void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, MsmBatchPlan& plan);
+// Grows ws for the plan; after this, msm_batch_run allocates nothing and
+// issues only kernel launches (so it can be captured in a HIP graph).
+void msm_batch_reserve(const MsmBatchPlan& plan, MsmWorkspace& ws);
+// Job j's output record (msm_out_bytes) goes to d_out + j * msm_out_bytes.
+void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_t* d_inf,
+                   const uint32_t* d_scalars, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out);
 
 // Opt-in per-stage timing of msm_device (HIP events on the MSM's stream).
 enum MsmStage { kStageSort = 0, kStageAccumulate = 1, kStageReduce = 2, kStageCombine = 3, kStageCount = 4 };

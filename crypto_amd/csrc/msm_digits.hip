@@ -87,6 +87,84 @@ static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf
                            entries);
 }
 
+// Batch form: the window size is a per-job runtime value, so the windows are
+// read by streaming the scalar's words through a 64-bit bit buffer (static
+// word indices only: no scratch).  Block b serves job j = last job with
+// blk_base <= b; its global bucket ids start at the job's gb_base and its
+// entries carry the point's index in the batch's point array.
+template <bool SCATTER>
+__global__ void __launch_bounds__(256)
+k_digits_multi(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf,
+               const MsmJob* __restrict__ jobs, uint32_t njobs, uint32_t* __restrict__ counts_or_cursor,
+               uint32_t* __restrict__ entries) {
+    __shared__ MsmJob sj;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = njobs;  // invariant: jobs[lo].blk_base <= blockIdx.x
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].blk_base <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        sj = jobs[lo];
+    }
+    __syncthreads();
+    const MsmJob jd = sj;
+    const uint32_t i = (blockIdx.x - jd.blk_base) * blockDim.x + threadIdx.x;
+    if (i >= jd.n) return;
+    const uint32_t p = jd.pt_base + i;
+    if (inf && inf[p]) return;
+    uint32_t s[8];
+    load_scalar(scalars + jd.sc_base * 8, i, s);
+    const uint32_t c = jd.c, B = 1u << (c - 1), mask = (1u << c) - 1u;
+    uint32_t w = 0, carry = 0;
+    auto emit = [&](uint32_t v) {
+        int32_t d = (int32_t)(v + carry);
+        carry = 0;
+        if (d > (int32_t)B) {
+            d -= (int32_t)(1u << c);
+            carry = 1;
+        }
+        if (d != 0) {
+            const uint32_t gb = jd.gb_base + w * B + (uint32_t)(d > 0 ? d : -d) - 1u;
+            if (!SCATTER) {
+                atomicAdd(&counts_or_cursor[gb], 1u);
+            } else {
+                const uint32_t pos = atomicAdd(&counts_or_cursor[gb], 1u);
+                entries[pos] = p | (d < 0 ? 0x80000000u : 0u);
+            }
+        }
+        ++w;
+    };
+    uint64_t buf = 0;
+    uint32_t nb = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        buf |= (uint64_t)s[k] << nb;
+        nb += 32;
+        while (nb >= c) {
+            emit((uint32_t)buf & mask);
+            buf >>= c;
+            nb -= c;
+        }
+    }
+    while (w < jd.nwin) {  // the remaining < c bits, then the carry window
+        emit((uint32_t)buf & mask);
+        buf >>= c;
+    }
+}
+
+void digits_multi_dispatch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, const MsmJob* d_jobs,
+                           uint32_t njobs, uint32_t nblocks, uint32_t* counts_or_cursor, uint32_t* entries,
+                           hipStream_t s) {
+    if (nblocks == 0) return;
+    if (!scatter)
+        hipLaunchKernelGGL(k_digits_multi<false>, dim3(nblocks), dim3(256), 0, s, d_scalars, d_inf, d_jobs, njobs,
+                           counts_or_cursor, nullptr);
+    else
+        hipLaunchKernelGGL(k_digits_multi<true>, dim3(nblocks), dim3(256), 0, s, d_scalars, d_inf, d_jobs, njobs,
+                           counts_or_cursor, entries);
+}
+
 void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
                      uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s) {
     switch (c) {

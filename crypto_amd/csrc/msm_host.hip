@@ -1,5 +1,6 @@
-// Host side of the Pippenger MSM: workspace sizing, the launch sequence and
-// curve dispatch.  Kernels are in msm_impl.h / msm_digits.hip / scan.hip.
+// Host side of the Pippenger MSM: workspace sizing, the launch sequence, curve
+// dispatch, batches of MSMs and the window-size tables.  Kernels are in
+// msm_impl.h / msm_digits.hip / scan.hip.
 #include <atomic>
 #include <vector>
 
@@ -18,6 +19,34 @@ int msm_default_window_bits(size_t n) {
     if (c < 5) c = 5;
     if (c > 16) c = 16;
     return c;
+}
+
+// ------------------------------------------------------------ window tables
+namespace {
+
+std::mutex g_table_mu;
+int g_table[3][2][32];  // [curve][batched][floor(log2 n)], 0 = default rule
+
+}  // namespace
+
+int msm_window_bits(int curve, size_t n, bool batched) {
+    int lg = 0;
+    while (((size_t)1 << (lg + 1)) <= n) ++lg;
+    if (curve >= 1 && curve <= 2 && lg < 32) {
+        std::lock_guard<std::mutex> g(g_table_mu);
+        const int c = g_table[curve][batched ? 1 : 0][lg];
+        if (c) return c;
+    }
+    return msm_default_window_bits(n);
+}
+
+void msm_set_window_table(int curve, bool batched, const int* c_by_log, int count) {
+    if (curve < 1 || curve > 2 || count < 0 || count > 32) throw std::invalid_argument("bad window table");
+    for (int k = 0; k < count; ++k)
+        if (c_by_log[k] != 0 && (c_by_log[k] < 5 || c_by_log[k] > 18))
+            throw std::invalid_argument("window bits out of range [5, 18]");
+    std::lock_guard<std::mutex> g(g_table_mu);
+    for (int k = 0; k < 32; ++k) g_table[curve][batched ? 1 : 0][k] = k < count ? c_by_log[k] : 0;
 }
 
 // ---------------------------------------------------------------- profiler
@@ -39,6 +68,8 @@ struct StageTimer {
     hipEvent_t a = nullptr;
     StageTimer(hipStream_t s_, int stage_) : s(s_), stage(stage_) {
         if (!g_profiling.load()) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
         KZ_HIP(hipEventCreate(&a));
         KZ_HIP(hipEventRecord(a, s));
     }
@@ -77,6 +108,7 @@ int msm_collect_profile(double* sums_ms, int* counts, int nstages) {
     return 0;
 }
 
+// ---------------------------------------------------------------- pipeline
 namespace {
 
 constexpr uint32_t kSegBuckets = 16;  // must match kSeg in msm_impl.h
@@ -90,68 +122,85 @@ struct Stages {
     int xyzz_words;
 };
 
-const Stages kBls = {launch_accumulate_bls, launch_fold_bls, launch_reduce_seg_bls, launch_reduce_window_bls, launch_combine_bls, 56};
-const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_window_bn, launch_combine_bn, 36};
+const Stages kBls = {launch_accumulate_bls, launch_fold_bls,     launch_reduce_seg_bls,
+                     launch_reduce_window_bls, launch_combine_bls, 56};
+const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_window_bn,
+                    launch_combine_bn,    36};
 
-void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars,
-                  size_t n, const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
-    const int c = prm.c ? prm.c : msm_default_window_bits(n);
-    if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
-    const int W = msm_num_windows(c);
-    const uint32_t B = 1u << (c - 1);
-    const uint32_t NB = (uint32_t)W * B;
-    const uint32_t chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
-    const size_t max_entries = n * (size_t)W;
-    if (max_entries >= 0x80000000ull) throw std::invalid_argument("MSM too large for 31-bit entry indices");
-    const uint32_t max_chunks = ceil_div(max_entries, chunk);
+const Stages& stages_of(int curve) { return curve == 2 ? kBls : kBn; }
+
+// Sizes of one pipeline run, single MSM or batch.
+struct Shape {
+    uint32_t nb;        // global buckets
+    uint32_t nwin;      // global windows
+    uint32_t t1;        // segments per window (single MSM), 0 = per-window table
+    uint32_t nseg;      // first-level segments
+    uint32_t chunk;     // sorted entries per accumulation thread
+    uint32_t levels;    // carry-fold tree levels
+    size_t max_entries;
+    uint32_t max_chunks() const { return ceil_div(max_entries, chunk); }
+};
+
+// A bucket holds at most n entries (one per point), so its carry run is at
+// most ceil(n / chunk) long: that many tree levels always suffice.
+uint32_t fold_levels(size_t max_n, uint32_t chunk) {
+    uint32_t levels = 0;
+    while (((size_t)1 << levels) < (max_n + chunk - 1) / chunk + 1) ++levels;
+    return levels;
+}
+
+void reserve(const Stages& st, const Shape& sh, MsmWorkspace& ws) {
+    if (sh.max_entries >= 0x80000000ull) throw std::invalid_argument("MSM too large for 31-bit entry indices");
     const size_t pw = (size_t)st.xyzz_words * sizeof(uint32_t);  // XYZZ record bytes
-    const uint32_t t1 = B / kSegBuckets;
-    const uint32_t nseg = (uint32_t)W * t1;
-
-    ws.counts.ensure(NB * sizeof(uint32_t));
-    ws.offsets.ensure((NB + 1) * sizeof(uint32_t));
-    ws.cursor.ensure(NB * sizeof(uint32_t));
-    ws.block_sums.ensure(scan_block_count(NB) * sizeof(uint32_t));
+    const uint32_t max_chunks = sh.max_chunks();
+    ws.counts.ensure((size_t)sh.nb * sizeof(uint32_t));
+    ws.offsets.ensure(((size_t)sh.nb + 1) * sizeof(uint32_t));
+    ws.cursor.ensure((size_t)sh.nb * sizeof(uint32_t));
+    ws.block_sums.ensure((size_t)scan_block_count(sh.nb) * sizeof(uint32_t));
     ws.total.ensure(sizeof(uint32_t));
-    ws.entries.ensure((max_entries + 1) * sizeof(uint32_t));
-    ws.bsum.ensure(NB * pw);
+    ws.entries.ensure((sh.max_entries + 1) * sizeof(uint32_t));
+    ws.bsum.ensure((size_t)sh.nb * pw);
     ws.carry.ensure((size_t)(max_chunks + 1) * pw);
     ws.run_bucket.ensure((size_t)(max_chunks + 1) * sizeof(uint32_t));
-    ws.segA.ensure((size_t)nseg * pw);
-    ws.segB.ensure((size_t)nseg * pw);
-    ws.winres.ensure((size_t)W * pw);
+    ws.segA.ensure((size_t)sh.nseg * pw);
+    ws.segB.ensure((size_t)sh.nseg * pw);
+    ws.winres.ensure((size_t)sh.nwin * pw);
+}
 
+// Sort: clear the histogram, count, scan, scatter.  `digits(scatter, counts,
+// entries)` launches the count or scatter pass.
+template <class Digits>
+void run_stages(const Stages& st, const Shape& sh, Digits&& digits, const uint2* d_win_seg,
+                const MsmJob* d_jobs, const MsmJob& one, uint32_t njobs, const uint32_t* d_pts, MsmWorkspace& ws,
+                hipStream_t s, uint8_t* d_out) {
     uint32_t* counts = ws.counts.as<uint32_t>();
     uint32_t* offsets = ws.offsets.as<uint32_t>();
+    const uint32_t max_chunks = sh.max_chunks();
     {
         StageTimer t(s, kStageSort);
-        KZ_HIP(hipMemsetAsync(counts, 0, NB * sizeof(uint32_t), s));
-        digits_dispatch(c, false, d_scalars, d_inf, (uint32_t)n, counts, nullptr, s);
-        device_exclusive_scan(counts, offsets, ws.cursor.as<uint32_t>(), NB, ws.block_sums.as<uint32_t>(),
+        device_fill_u32(counts, 0, sh.nb, s);
+        digits(false, counts, nullptr);
+        device_exclusive_scan(counts, offsets, ws.cursor.as<uint32_t>(), sh.nb, ws.block_sums.as<uint32_t>(),
                               ws.total.as<uint32_t>(), s);
-        digits_dispatch(c, true, d_scalars, d_inf, (uint32_t)n, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>(),
-                        s);
+        digits(true, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>());
     }
     {
         StageTimer t(s, kStageAccumulate);
-        st.accumulate(ceil_div(max_chunks, 64), s, ws.entries.as<uint32_t>(), offsets, NB, chunk, d_pts,
+        st.accumulate(ceil_div(max_chunks, 64), s, ws.entries.as<uint32_t>(), offsets, sh.nb, sh.chunk, d_pts,
                       ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>());
     }
-    // A bucket holds at most n entries (one per point), so its carry run is at
-    // most ceil(n / chunk) long: that many tree levels always suffice.
-    uint32_t levels = 0;
-    while (((size_t)1 << levels) < (n + chunk - 1) / chunk + 1) ++levels;
     {
         StageTimer t(s, kStageReduce);
-        st.fold(s, offsets, NB, chunk, max_chunks, levels, ws.run_bucket.as<uint32_t>(), ws.carry.as<uint32_t>());
-        st.reduce_seg(ceil_div(nseg, 64), s, offsets, chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(), nseg,
-                      ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
-        st.reduce_window((unsigned)W, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), t1,
+        st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, ws.run_bucket.as<uint32_t>(),
+                ws.carry.as<uint32_t>());
+        st.reduce_seg(ceil_div(sh.nseg, 64), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(),
+                      sh.nseg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
+        st.reduce_window(sh.nwin, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), sh.t1, d_win_seg,
                          ws.winres.as<uint32_t>());
     }
     {
         StageTimer t(s, kStageCombine);
-        st.combine(s, ws.winres.as<uint32_t>(), W, c, d_out);
+        st.combine(s, ws.winres.as<uint32_t>(), d_jobs, one, njobs, d_out);
     }
     KZ_HIP(hipGetLastError());
 }
@@ -160,9 +209,106 @@ void run_pipeline(const Stages& st, const uint32_t* d_pts, const uint8_t* d_inf,
 
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
-    run_pipeline(curve == 2 ? kBls : kBn, d_pts, d_inf, d_scalars, n, prm, ws, s, d_out);
+    const Stages& st = stages_of(curve);
+    const int c = prm.c ? prm.c : msm_window_bits(curve, n, false);
+    if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
+    const int W = msm_num_windows(c);
+    const uint32_t B = 1u << (c - 1);
+    Shape sh;
+    sh.nb = (uint32_t)W * B;
+    sh.nwin = (uint32_t)W;
+    sh.t1 = B / kSegBuckets;
+    sh.nseg = (uint32_t)W * sh.t1;
+    sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
+    sh.levels = fold_levels(n, sh.chunk);
+    sh.max_entries = n * (size_t)W;
+    reserve(st, sh, ws);
+    MsmJob one{};
+    one.n = (uint32_t)n;
+    one.c = (uint32_t)c;
+    one.nwin = (uint32_t)W;
+    auto digits = [&](bool scatter, uint32_t* cc, uint32_t* entries) {
+        digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)n, cc, entries, s);
+    };
+    run_stages(st, sh, digits, nullptr, nullptr, one, 1, d_pts, ws, s, d_out);
 }
 
+// ------------------------------------------------------------------ batches
+namespace {
+
+Shape batch_shape(const MsmBatchPlan& p) {
+    Shape sh;
+    sh.nb = p.nb;
+    sh.nwin = p.nwin;
+    sh.t1 = 0;
+    sh.nseg = p.nseg;
+    sh.chunk = p.chunk;
+    sh.levels = p.levels;
+    sh.max_entries = p.max_entries;
+    return sh;
+}
+
+}  // namespace
+
+void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, MsmBatchPlan& plan) {
+    if (curve != 1 && curve != 2) throw std::invalid_argument("curve");
+    if (njobs == 0 || njobs > (1u << 24)) throw std::invalid_argument("job count out of range");
+    plan.curve = curve;
+    plan.njobs = (uint32_t)njobs;
+    plan.chunk = (uint32_t)(chunk > 0 ? chunk : 32);
+    plan.jobs.assign(specs, specs + njobs);
+    std::vector<uint2> win_seg;
+    uint64_t nb = 0, nwin = 0, nseg = 0, blocks = 0;
+    size_t entries = 0, max_n = 0;
+    for (auto& j : plan.jobs) {
+        const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, true);
+        if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
+        const uint32_t W = (uint32_t)msm_num_windows(c), B = 1u << (c - 1), t1 = B / kSegBuckets;
+        j.c = (uint32_t)c;
+        j.nwin = W;
+        j.win_base = (uint32_t)nwin;
+        j.gb_base = (uint32_t)nb;
+        j.blk_base = (uint32_t)blocks;
+        j.pad = 0;
+        for (uint32_t w = 0; w < W; ++w) win_seg.push_back(make_uint2((uint32_t)(nseg + (uint64_t)w * t1), t1));
+        nb += (uint64_t)W * B;
+        nwin += W;
+        nseg += (uint64_t)W * t1;
+        blocks += (j.n + 255) / 256;
+        entries += (size_t)j.n * W;
+        if (j.n > max_n) max_n = j.n;
+        if ((uint64_t)j.pt_base + j.n > 0x80000000ull) throw std::invalid_argument("point index beyond 2^31");
+    }
+    if (nb >= 0xFFFFFFFFull || blocks >= 0x80000000ull) throw std::invalid_argument("batch too large");
+    plan.nb = (uint32_t)nb;
+    plan.nwin = (uint32_t)nwin;
+    plan.nseg = (uint32_t)nseg;
+    plan.dig_blocks = (uint32_t)blocks;
+    plan.max_entries = entries;
+    plan.levels = fold_levels(max_n, plan.chunk);
+    plan.d_jobs.ensure(njobs * sizeof(MsmJob));
+    plan.d_win_seg.ensure(win_seg.size() * sizeof(uint2));
+    KZ_HIP(hipMemcpy(plan.d_jobs.ptr, plan.jobs.data(), njobs * sizeof(MsmJob), hipMemcpyHostToDevice));
+    KZ_HIP(hipMemcpy(plan.d_win_seg.ptr, win_seg.data(), win_seg.size() * sizeof(uint2), hipMemcpyHostToDevice));
+}
+
+void msm_batch_reserve(const MsmBatchPlan& plan, MsmWorkspace& ws) {
+    reserve(stages_of(plan.curve), batch_shape(plan), ws);
+}
+
+void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_t* d_inf,
+                   const uint32_t* d_scalars, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
+    const Stages& st = stages_of(plan.curve);
+    const Shape sh = batch_shape(plan);
+    reserve(st, sh, ws);
+    const MsmJob* d_jobs = plan.d_jobs.as<MsmJob>();
+    auto digits = [&](bool scatter, uint32_t* cc, uint32_t* entries) {
+        digits_multi_dispatch(scatter, d_scalars, d_inf, d_jobs, plan.njobs, plan.dig_blocks, cc, entries, s);
+    };
+    run_stages(st, sh, digits, plan.d_win_seg.as<uint2>(), d_jobs, MsmJob{}, plan.njobs, d_pts, ws, s, d_out);
+}
+
+// ------------------------------------------------------------ conversions
 void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t* d_inf, size_t n,
                       hipStream_t s) {
     if (n == 0) return;

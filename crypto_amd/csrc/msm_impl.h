@@ -23,7 +23,7 @@
 // and keep few point-operation sites with `#pragma unroll 1` loops.
 #pragma once
 
-#include "curve.h"
+#include "curve_quad.h"
 #include "msm_internal.h"
 
 namespace kz {
@@ -201,17 +201,24 @@ constexpr int kReduceThreads = 256;
 //   U - V = sum_k u_k + L2 sum_k R_k - (L2 + 1) R_0,
 // hence window = sum_k [b_k + 16 (u_k + X_k)] with X_k = L2 R_k for k > 0 and
 // X_0 = -R_0, a single tree sum.
+// Windows of one MSM all have t1 segments at w * t1; a batch passes win_seg
+// (per window: first segment, segment count) instead.
 template <class F>
 __global__ void __launch_bounds__(kReduceThreads)
 k_reduce_window(const uint32_t* __restrict__ segA, const uint32_t* __restrict__ segB, uint32_t t1,
-                uint32_t* __restrict__ winres) {
+                const uint2* __restrict__ win_seg, uint32_t* __restrict__ winres) {
     __shared__ Xyzz<F> sh[kReduceThreads];
     const int k = threadIdx.x;
     const uint32_t w = blockIdx.x;
+    size_t base = (size_t)w * t1;
+    if (win_seg) {
+        const uint2 ws = win_seg[w];
+        base = ws.x;
+        t1 = ws.y;
+    }
     const uint32_t l2 = t1 >= kReduceThreads ? t1 / kReduceThreads : 1u;
     int log_l2 = 0;
     while ((1u << log_l2) < l2) ++log_l2;
-    const size_t base = (size_t)w * t1;
 
     // local running sums over this thread's segments (one add site)
     Xyzz<F> run = xyzz_inf<F>(), u = xyzz_inf<F>(), bs = xyzz_inf<F>();
@@ -293,22 +300,29 @@ __device__ void write_output(const Xyzz<typename C::Fp>& r, uint8_t* __restrict_
     write_compressed<C>(x, y, out + 2 * C::FB);
 }
 
-// Horner over the window sums, MSW first: r = 2^c r + S_w.
+// Horner over the window sums, MSW first: r = 2^c r + S_w.  One quad of lanes
+// per MSM (quad-cooperative point operations: the chain of ~256 doublings is
+// the critical path of the whole MSM).  jobs = null: a single MSM described
+// by `one`; else njobs MSMs, job j's record at out + j * 3 * FB.
 template <class C>
-__global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ winres, int nwin, int c,
+__global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ winres,
+                                                const MsmJob* __restrict__ jobs, MsmJob one, uint32_t njobs,
                                                 uint8_t* __restrict__ out) {
     using F = typename C::Fp;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    Xyzz<F> r = xyzz_load<F>(winres, nwin - 1);
+    const uint32_t job = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    if (job >= njobs) return;
+    const MsmJob jd = jobs ? jobs[job] : one;
+    const int c = (int)jd.c;
+    Xyzz<F> r = xyzz_load<F>(winres, jd.win_base + jd.nwin - 1);
 #pragma unroll 1
-    for (int w = nwin - 2; w >= 0; --w) {
+    for (int w = (int)jd.nwin - 2; w >= 0; --w) {
         if (!xyzz_is_inf<F>(r)) {
 #pragma unroll 1
-            for (int i = 0; i < c; ++i) r = xyzz_dbl<F, MulCall>(r);
+            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F>(r);
         }
-        r = xyzz_add<F, MulCall>(r, xyzz_load<F>(winres, w));
+        r = xyzz_add_quad<F>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
     }
-    write_output<C>(r, out);
+    if (quad_lane() == 0) write_output<C>(r, out + (size_t)job * 3 * C::FB);
 }
 
 template <class C>

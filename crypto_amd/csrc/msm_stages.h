@@ -3,7 +3,7 @@
 // parallel; the host pipeline (msm_host.hip) calls them through these.
 #pragma once
 
-#include "common.h"
+#include "msm.h"
 
 namespace kz {
 
@@ -20,8 +20,10 @@ namespace kz {
                                  const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA, \
                                  uint32_t* segB);                                                           \
     void launch_reduce_window_##sfx(unsigned nwin, hipStream_t s, const uint32_t* segA,                     \
-                                    const uint32_t* segB, uint32_t t1, uint32_t* winres);                   \
-    void launch_combine_##sfx(hipStream_t s, const uint32_t* winres, int nwin, int c, uint8_t* out);
+                                    const uint32_t* segB, uint32_t t1, const uint2* win_seg,                \
+                                    uint32_t* winres);                                                      \
+    void launch_combine_##sfx(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,  \
+                              uint32_t njobs, uint8_t* out);
 
 KZ_DECLARE_STAGES(bls)
 KZ_DECLARE_STAGES(bn)

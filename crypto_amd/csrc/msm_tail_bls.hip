@@ -4,8 +4,10 @@
 
 namespace kz {
 
-void launch_combine_bls(hipStream_t s, const uint32_t* winres, int nwin, int c, uint8_t* out) {
-    hipLaunchKernelGGL(k_combine<CurveBLS12381>, dim3(1), dim3(64), 0, s, winres, nwin, c, out);
+void launch_combine_bls(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,
+                        uint32_t njobs, uint8_t* out) {
+    hipLaunchKernelGGL(k_combine<CurveBLS12381>, dim3(ceil_div((size_t)njobs * 4, 64)), dim3(64), 0, s, winres,
+                       jobs, one, njobs, out);
 }
 
 void launch_compress_bls(const uint8_t* wire, uint8_t* out, hipStream_t s) {

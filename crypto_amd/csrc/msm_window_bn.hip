@@ -5,8 +5,9 @@
 namespace kz {
 
 void launch_reduce_window_bn(unsigned nwin, hipStream_t s, const uint32_t* segA, const uint32_t* segB,
-                               uint32_t t1, uint32_t* winres) {
-    hipLaunchKernelGGL(k_reduce_window<CurveBN254::Fp>, dim3(nwin), dim3(kReduceThreads), 0, s, segA, segB, t1, winres);
+                             uint32_t t1, const uint2* win_seg, uint32_t* winres) {
+    hipLaunchKernelGGL(k_reduce_window<CurveBN254::Fp>, dim3(nwin), dim3(kReduceThreads), 0, s, segA, segB, t1,
+                       win_seg, winres);
 }
 
 }  // namespace kz

@@ -18,4 +18,13 @@ constexpr int kNttMaxLog = 26;  // largest supported transform: 2^26 elements
 void ntt_device(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
                 int* d_err, hipStream_t s);
 
+// count transforms of the same size laid back to back (d_in, d_out, d_tmp:
+// count * n * 32 bytes), one launch per pass for all of them.
+void ntt_device_batch(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
+                      size_t count, int* d_err, hipStream_t s);
+
+// Builds the twiddle tables of (field, 2^logn, direction) now (allocates and
+// synchronises), so later transforms only launch kernels.
+void ntt_prepare(int field, int logn, bool inverse, hipStream_t s);
+
 }  // namespace kz

@@ -67,6 +67,10 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
     static_assert(F::N == 9, "Fr limbs");
     __shared__ uint32_t sh[kNttTile * F::N];
     __shared__ uint32_t shtw[(kNttTile / 2) * F::N];
+    // blockIdx.y selects one transform of a batch laid back to back (n words x 8)
+    const size_t xform = (size_t)blockIdx.y * ((size_t)n_mask + 1) * 8;
+    in += xform;
+    out += xform;
     const uint32_t L = 1u << d.logL, half_L = L >> 1, tile = d.T * L;
     const uint32_t q_start = blockIdx.x * d.T;
     const uint32_t tid = threadIdx.x;
@@ -263,10 +267,18 @@ void plan(int logn, bool inverse, NttPass* passes, int& np) {
 
 }  // namespace
 
-void ntt_device(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
-                int* d_err, hipStream_t s) {
-    if (logn < 0 || logn > kNttMaxLog || logn > kTwoAdicity[field])
+void ntt_prepare(int field, int logn, bool inverse, hipStream_t s) {
+    if (field < 0 || field > 1 || logn < 0 || logn > kNttMaxLog || logn > kTwoAdicity[field])
         throw std::invalid_argument("NTT size out of range");
+    (void)tables(field, logn, inverse, s);
+}
+
+void ntt_device_batch(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
+                      size_t count, int* d_err, hipStream_t s) {
+    if (field < 0 || field > 1 || logn < 0 || logn > kNttMaxLog || logn > kTwoAdicity[field])
+        throw std::invalid_argument("NTT size out of range");
+    if (count == 0) return;
+    if (count > 65535) throw std::invalid_argument("NTT batch larger than 65535 transforms");
     const NttTables& t = tables(field, logn, inverse, s);
     NttPass passes[3];
     int np = 0;
@@ -277,13 +289,18 @@ void ntt_device(int field, int logn, bool inverse, const uint32_t* d_in, uint32_
         uint32_t* dst = (i == np - 1) ? d_out : d_tmp;
         const unsigned grid = (n >> passes[i].logL) / passes[i].T;  // exact: powers of two, T <= n / L
         if (field == 0)
-            hipLaunchKernelGGL(k_ntt_pass<BlsFr>, dim3(grid), dim3(kNttThreads), 0, s, src, dst,
+            hipLaunchKernelGGL(k_ntt_pass<BlsFr>, dim3(grid, (unsigned)count), dim3(kNttThreads), 0, s, src, dst,
                                t.tw.as<uint32_t>(), t.ninv.as<uint32_t>(), passes[i], n - 1, d_err);
         else
-            hipLaunchKernelGGL(k_ntt_pass<BnFr>, dim3(grid), dim3(kNttThreads), 0, s, src, dst,
+            hipLaunchKernelGGL(k_ntt_pass<BnFr>, dim3(grid, (unsigned)count), dim3(kNttThreads), 0, s, src, dst,
                                t.tw.as<uint32_t>(), t.ninv.as<uint32_t>(), passes[i], n - 1, d_err);
     }
     KZ_HIP(hipGetLastError());
+}
+
+void ntt_device(int field, int logn, bool inverse, const uint32_t* d_in, uint32_t* d_out, uint32_t* d_tmp,
+                int* d_err, hipStream_t s) {
+    ntt_device_batch(field, logn, inverse, d_in, d_out, d_tmp, 1, d_err, s);
 }
 
 }  // namespace kz
@@ -304,6 +321,27 @@ int kinet_hip_ntt_fr(uint32_t field, int inverse, const void* d_in, void* d_out,
         (void)engine();
         ntt_device((int)field, logn, inverse != 0, static_cast<const uint32_t*>(d_in), static_cast<uint32_t*>(d_out),
                    static_cast<uint32_t*>(d_tmp), d_err, static_cast<hipStream_t>(stream));
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+int kinet_hip_ntt_fr_batch(uint32_t field, int inverse, const void* d_in, void* d_out, void* d_tmp, size_t n,
+                           size_t count, int* d_err, void* stream) {
+    if (field > 1) return KZ_CRYPTO_ERR_INPUT;
+    if (!d_in || !d_out || !d_tmp || !d_err) return KZ_CRYPTO_ERR_INPUT;
+    int logn = 0;
+    while (((size_t)1 << logn) < n) ++logn;
+    if (n == 0 || ((size_t)1 << logn) != n || logn > kNttMaxLog || logn > (field == 0 ? 32 : 28))
+        return KZ_CRYPTO_ERR_INPUT;
+    if (count > 65535) return KZ_CRYPTO_ERR_INPUT;
+    try {
+        (void)engine();
+        ntt_device_batch((int)field, logn, inverse != 0, static_cast<const uint32_t*>(d_in),
+                         static_cast<uint32_t*>(d_out), static_cast<uint32_t*>(d_tmp), count, d_err,
+                         static_cast<hipStream_t>(stream));
         return KZ_CRYPTO_OK;
     } catch (const std::exception& ex) {
         set_last_error(ex.what());

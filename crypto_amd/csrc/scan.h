@@ -23,4 +23,8 @@ void device_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, uint32_t* d_cu
 
 inline uint32_t scan_block_count(uint32_t n) { return (n + kScanTile - 1) / kScanTile; }
 
+// p[0..n) = v by a kernel (not hipMemsetAsync, so the clear is safe to record
+// in a HIP graph and replay).
+void device_fill_u32(uint32_t* d_p, uint32_t v, size_t n, hipStream_t s);
+
 }  // namespace kz

@@ -83,5 +83,14 @@ void device_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, uint32_t* d_cu
                        d_total);
 }
 
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void device_fill_u32(uint32_t* d_p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_p, v, n);
+}
 
 }  // namespace kz

@@ -58,3 +58,79 @@ def msm_prepared(pp: PreparedPoints, d_scalars: torch.Tensor, n: int | None = No
     if sync:
         (stream or torch.cuda.current_stream()).synchronize()
     return out
+
+
+class MsmBatch:
+    """A plan of many MSMs run as one pipeline (kinet_hip_msm_batch_*).
+
+    sizes[j] points per job; sc_off / pt_off as in the C header (None: scalars
+    back to back, every job from point 0)."""
+
+    def __init__(self, curve: int, sizes, sc_off=None, pt_off=None, window_bits: int = 0):
+        import ctypes
+
+        lib = require_device()
+        self.curve = curve
+        self.count = len(sizes)
+        n_arr = (ctypes.c_size_t * self.count)(*sizes)
+        sc_arr = (ctypes.c_uint64 * self.count)(*sc_off) if sc_off is not None else None
+        pt_arr = (ctypes.c_uint32 * self.count)(*pt_off) if pt_off is not None else None
+        h = ctypes.c_void_p()
+        rc = lib.kinet_hip_msm_batch_create(curve, n_arr, sc_arr, pt_arr, self.count, window_bits, ctypes.byref(h))
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_msm_batch_create", rc, GPUKIT_STATUS)
+        self.handle = h.value
+        self.record = point_bytes(curve) + point_bytes(curve) // 2
+
+    def window_bits(self, job: int) -> int:
+        return require_device().kinet_hip_msm_batch_window_bits(self.handle, job)
+
+    def run(self, pp: PreparedPoints, d_scalars: torch.Tensor, out: torch.Tensor | None = None,
+            use_graph: bool = False, stream=None, sync: bool = True) -> torch.Tensor:
+        """Runs every job; returns the device records (count x record bytes)."""
+        lib = require_device()
+        if out is None:
+            out = torch.empty(self.count * self.record, dtype=torch.uint8, device=d_scalars.device)
+        rc = lib.kinet_hip_msm_batch_run(self.handle, d_scalars.data_ptr(), pp.pts.data_ptr(), pp.inf.data_ptr(),
+                                         out.data_ptr(), 1 if use_graph else 0, stream_handle(stream))
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_msm_batch_run", rc, GPUKIT_STATUS)
+        if sync:
+            (stream or torch.cuda.current_stream()).synchronize()
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            require_device().kinet_hip_msm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ntt_batch(field: int, d_in: torch.Tensor, n: int, count: int, inverse: bool = False,
+              d_out: torch.Tensor | None = None, d_tmp: torch.Tensor | None = None, d_err: torch.Tensor | None = None,
+              stream=None, sync: bool = True) -> torch.Tensor:
+    """count same-size Fr transforms back to back (kinet_hip_ntt_fr_batch)."""
+    from .lib import CRYPTO_STATUS
+
+    lib = require_device()
+    if d_out is None:
+        d_out = torch.empty(count * n * 32, dtype=torch.uint8, device=d_in.device)
+    if d_tmp is None:
+        d_tmp = torch.empty(count * n * 32, dtype=torch.uint8, device=d_in.device)
+    own_err = d_err is None
+    if own_err:
+        d_err = torch.zeros(1, dtype=torch.int32, device=d_in.device)
+    rc = lib.kinet_hip_ntt_fr_batch(field, 1 if inverse else 0, d_in.data_ptr(), d_out.data_ptr(), d_tmp.data_ptr(),
+                                    n, count, d_err.data_ptr(), stream_handle(stream))
+    if rc != 0:
+        raise KinetError("kinet_hip_ntt_fr_batch", rc, CRYPTO_STATUS)
+    if sync:
+        (stream or torch.cuda.current_stream()).synchronize()
+        if own_err and int(d_err.item()) != 0:
+            raise KinetError("kinet_hip_ntt_fr_batch", -1, CRYPTO_STATUS)
+    return d_out

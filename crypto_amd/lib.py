@@ -91,6 +91,15 @@ def library():
         lib.kinet_hip_profile_enable.argtypes = [i32]
         lib.kinet_hip_profile_enable.restype = None
         lib.kinet_hip_profile_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), i32]
+        lib.kinet_hip_ntt_fr_batch.argtypes = [u32, i32, vp, vp, vp, sz, sz, vp, vp]
+        lib.kinet_hip_msm_batch_create.argtypes = [u32, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_uint64),
+                                                   ctypes.POINTER(u32), sz, i32, pp]
+        lib.kinet_hip_msm_batch_destroy.argtypes = [vp]
+        lib.kinet_hip_msm_batch_destroy.restype = None
+        lib.kinet_hip_msm_batch_window_bits.argtypes = [vp, sz]
+        lib.kinet_hip_msm_batch_run.argtypes = [vp, vp, vp, vp, vp, i32, vp]
+        lib.kinet_hip_msm_set_window_table.argtypes = [u32, i32, ctypes.POINTER(i32), i32]
+        lib.kinet_hip_msm_window_bits.argtypes = [u32, sz, i32]
         _lib = lib
     return _lib
 

@@ -36,6 +36,40 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                            size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
+/* ------------------------------------------------------------- batches
+ * Many independent MSMs as one pipeline: every stage is launched once for
+ * all jobs (bucket ids, windows and outputs of the jobs laid end to end), so
+ * the per-MSM launch overhead and the serial window-combine chain (~256
+ * doublings per MSM) are paid once per batch instead of once per job. */
+typedef struct kinet_msm_batch kinet_msm_batch;
+
+/* Plan for `count` MSMs over one prepared point array and one scalar array.
+ * Job j: n[j] points from point pt_off[j] (pt_off NULL: every job starts at
+ * point 0, e.g. prefixes of one SRS) and n[j] scalars from scalar sc_off[j]
+ * (sc_off NULL: back to back in job order).  window_bits 0 = per job from the
+ * installed tuning table (else the default rule), or 5..18 for every job.
+ * The plan owns its scratch: different plans may run concurrently on
+ * different streams; calls on one plan must not overlap.  gpukit codes. */
+int kinet_hip_msm_batch_create(uint32_t curve, const size_t* n, const uint64_t* sc_off, const uint32_t* pt_off,
+                               size_t count, int window_bits, kinet_msm_batch** out);
+void kinet_hip_msm_batch_destroy(kinet_msm_batch* batch);
+/* Window bits the plan uses for job `job` (-1 if out of range). */
+int kinet_hip_msm_batch_window_bits(const kinet_msm_batch* batch, size_t job);
+/* Runs every job; job j's record (wire point, then compressed: 96 + 48 or
+ * 64 + 32 bytes) goes to d_out + j * record.  use_graph = 1 captures the
+ * launch sequence in a HIP graph on first use with these pointers and this
+ * stream and replays the graph afterwards (stream must not be NULL);
+ * use_graph = 0 launches the kernels directly. */
+int kinet_hip_msm_batch_run(kinet_msm_batch* batch, const void* d_scalars, const void* d_pts, const void* d_inf,
+                            void* d_out, int use_graph, void* stream);
+
+/* Window-size tuning table per curve and mode (batched = 0: single MSMs,
+ * 1: batches): c_by_log[k] = window bits for 2^k <= n < 2^(k+1), 0 = default
+ * rule, count <= 32 (0 clears).  The library default is c = floor(log2 n) - 4
+ * clamped to [5, 16]. */
+int kinet_hip_msm_set_window_table(uint32_t curve, int batched, const int* c_by_log, int count);
+int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int batched);
+
 /* Fixed-base batch scalar multiplication: out[i] = k_i * G for n scalars
  * (32-byte LE, any 256-bit value) on the device, written in the point wire
  * format (BE affine x || y, identity = zero bytes).  The table of multiples of

@@ -37,6 +37,12 @@ int ntt_fr_inverse(uint32_t field, uint8_t* elems, size_t n);
 int kinet_hip_ntt_fr(uint32_t field, int inverse, const void* d_in, void* d_out, void* d_tmp, size_t n, int* d_err,
                      void* stream);
 
+/* `count` transforms of the same size n laid back to back (d_in, d_out, d_tmp:
+ * count x n x 32 bytes), one launch per pass for all of them; count <= 65535.
+ * Same element format, domain and error flag as kinet_hip_ntt_fr. */
+int kinet_hip_ntt_fr_batch(uint32_t field, int inverse, const void* d_in, void* d_out, void* d_tmp, size_t n,
+                           size_t count, int* d_err, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
