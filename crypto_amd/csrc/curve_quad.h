@@ -3,11 +3,12 @@
 // A lone serial chain of point operations (the window combine's 256
 // doublings, a running sum) runs at one wave's instruction issue rate, so its
 // time is (field products on the critical path) x (one product's latency).
-// Here the four lanes of a DPP quad hold the same point(s) and evaluate one
-// operation together: at each level of the formula's dependency graph lane q
-// computes one field product with operands picked by q, and DPP quad_perm
-// broadcasts the four products back across the quad.  The lanes execute in
-// lock step, so four products cost one product's latency:
+// Here the four lanes of a quad (lanes 4k..4k+3) hold the same point(s) and
+// evaluate one operation together: at each level of the formula's dependency
+// graph lane q computes one field product with operands picked by q, and a
+// quad-permute swizzle broadcasts the four products back across the quad.
+// The lanes execute in lock step, so four products cost one product's
+// latency:
 //
 //   doubling (dbl-2008-s-1)  9 products -> 3 levels
 //   addition (add-2008-s)   14 products -> 4 levels
@@ -57,53 +58,53 @@ __device__ __forceinline__ Fe<F> quad_pick(int q, const Fe<F>& a0, const Fe<F>& 
 }
 
 // 2P, three product levels.
-template <class F>
+template <class F, class M = MulCall>
 __device__ Xyzz<F> xyzz_dbl_quad(const Xyzz<F>& p) {
     const int q = quad_lane();
     const Fe<F> U = fe_dbl<F>(p.Y);
     // level 1: V = U^2, XX = X^2
-    Fe<F> m = fe_mul_call<F>(quad_pick<F>(q, U, p.X, U, p.X), quad_pick<F>(q, U, p.X, U, p.X));
+    Fe<F> m = M::template mul<F>(quad_pick<F>(q, U, p.X, U, p.X), quad_pick<F>(q, U, p.X, U, p.X));
     const Fe<F> V = quad_read<0>(m), XX = quad_read<1>(m);
-    const Fe<F> M = fe_add<F>(fe_dbl<F>(XX), XX);
-    // level 2: W = U V, S = X V, ZZ' = V ZZ, M^2
-    m = fe_mul_call<F>(quad_pick<F>(q, U, p.X, V, M), quad_pick<F>(q, V, V, p.ZZ, M));
+    const Fe<F> X3 = fe_add<F>(fe_dbl<F>(XX), XX);  // 3 X^2
+    // level 2: W = U V, S = X V, ZZ' = V ZZ, (3 X^2)^2
+    m = M::template mul<F>(quad_pick<F>(q, U, p.X, V, X3), quad_pick<F>(q, V, V, p.ZZ, X3));
     const Fe<F> W = quad_read<0>(m), S = quad_read<1>(m);
     Xyzz<F> r;
     r.ZZ = quad_read<2>(m);
     r.X = fe_sub<F>(quad_read<3>(m), fe_dbl<F>(S));
-    // level 3: M (S - X'), W Y, W ZZZ
-    m = fe_mul_call<F>(quad_pick<F>(q, M, W, W, W), quad_pick<F>(q, fe_sub<F>(S, r.X), p.Y, p.ZZZ, p.ZZZ));
+    // level 3: 3 X^2 (S - X'), W Y, W ZZZ
+    m = M::template mul<F>(quad_pick<F>(q, X3, W, W, W), quad_pick<F>(q, fe_sub<F>(S, r.X), p.Y, p.ZZZ, p.ZZZ));
     r.Y = fe_sub<F>(quad_read<0>(m), quad_read<1>(m));
     r.ZZZ = quad_read<2>(m);
     return r;
 }
 
 // a + b with the doubling and inverse special cases, four product levels.
-template <class F>
+template <class F, class M = MulCall>
 __device__ Xyzz<F> xyzz_add_quad(const Xyzz<F>& a, const Xyzz<F>& b) {
     if (xyzz_is_inf<F>(a)) return b;
     if (xyzz_is_inf<F>(b)) return a;
     const int q = quad_lane();
     // level 1: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1
-    Fe<F> m = fe_mul_call<F>(quad_pick<F>(q, a.X, b.X, a.Y, b.Y), quad_pick<F>(q, b.ZZ, a.ZZ, b.ZZZ, a.ZZZ));
+    Fe<F> m = M::template mul<F>(quad_pick<F>(q, a.X, b.X, a.Y, b.Y), quad_pick<F>(q, b.ZZ, a.ZZ, b.ZZZ, a.ZZZ));
     const Fe<F> U1 = quad_read<0>(m), S1 = quad_read<2>(m);
     const Fe<F> P = fe_sub<F>(quad_read<1>(m), U1);
     const Fe<F> R = fe_sub<F>(quad_read<3>(m), S1);
     if (fe_is_zero<F>(P)) {
-        if (fe_is_zero<F>(R)) return xyzz_dbl_quad<F>(a);
+        if (fe_is_zero<F>(R)) return xyzz_dbl_quad<F, M>(a);
         return xyzz_inf<F>();
     }
     // level 2: PP = P^2, RR = R^2, ZZ1 ZZ2, ZZZ1 ZZZ2
-    m = fe_mul_call<F>(quad_pick<F>(q, P, R, a.ZZ, a.ZZZ), quad_pick<F>(q, P, R, b.ZZ, b.ZZZ));
+    m = M::template mul<F>(quad_pick<F>(q, P, R, a.ZZ, a.ZZZ), quad_pick<F>(q, P, R, b.ZZ, b.ZZZ));
     const Fe<F> PP = quad_read<0>(m), RR = quad_read<1>(m), Z12 = quad_read<2>(m), ZZZ12 = quad_read<3>(m);
     // level 3: PPP = P PP, Q = U1 PP, ZZ' = ZZ1 ZZ2 PP
-    m = fe_mul_call<F>(quad_pick<F>(q, P, U1, Z12, Z12), quad_pick<F>(q, PP, PP, PP, PP));
+    m = M::template mul<F>(quad_pick<F>(q, P, U1, Z12, Z12), quad_pick<F>(q, PP, PP, PP, PP));
     const Fe<F> PPP = quad_read<0>(m), Q = quad_read<1>(m);
     Xyzz<F> r;
     r.ZZ = quad_read<2>(m);
     r.X = fe_sub<F>(fe_sub<F>(RR, PPP), fe_dbl<F>(Q));
     // level 4: R (Q - X'), S1 PPP, ZZZ' = ZZZ1 ZZZ2 PPP
-    m = fe_mul_call<F>(quad_pick<F>(q, R, S1, ZZZ12, ZZZ12), quad_pick<F>(q, fe_sub<F>(Q, r.X), PPP, PPP, PPP));
+    m = M::template mul<F>(quad_pick<F>(q, R, S1, ZZZ12, ZZZ12), quad_pick<F>(q, fe_sub<F>(Q, r.X), PPP, PPP, PPP));
     r.Y = fe_sub<F>(quad_read<0>(m), quad_read<1>(m));
     r.ZZZ = quad_read<2>(m);
     return r;

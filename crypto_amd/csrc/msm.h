@@ -12,6 +12,7 @@ struct MsmWorkspace {
     DevBuf counts, offsets, cursor, block_sums, total;
     DevBuf entries;
     DevBuf bsum, carry, run_bucket, segA, segB, winres;
+    DevBuf lvA[2], lvB[2];  // reduction levels (ping-pong)
     DevBuf out;
 };
 
@@ -33,6 +34,16 @@ struct MsmJob {
     uint32_t gb_base;   // first global bucket
     uint32_t blk_base;  // first block of the digit kernels
     uint32_t pad;
+};
+
+// Inputs per workgroup (one part) of a bucket-reduction level.
+constexpr uint32_t kReduceFanIn = 128;
+
+// One window's slice of a bucket-reduction level: inputs [in_base, in_base +
+// n_in), outputs [out_base, out_base + ceil(n_in / kReduceFanIn)), served by
+// the workgroups from wg_base on.
+struct RedLevel {
+    uint32_t in_base, n_in, out_base, wg_base;
 };
 
 int msm_default_window_bits(size_t n);
@@ -80,7 +91,11 @@ struct MsmBatchPlan {
     uint32_t levels = 0;      // carry-fold tree levels
     size_t max_entries = 0;
     std::vector<MsmJob> jobs;  // host copy of the descriptors
-    DevBuf d_jobs, d_win_seg;
+    // reduction levels: level l's per-window table at d_levels + level_off[l]
+    // (nwin entries), launched on level_grid[l] workgroups
+    std::vector<uint32_t> level_off, level_grid;
+    uint32_t level_max_out = 0;  // largest output count of a non-final level
+    DevBuf d_jobs, d_levels;
 };
 
 // Window bits used for an n-point MSM: the installed tuning table when one is

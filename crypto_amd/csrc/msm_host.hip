@@ -117,14 +117,14 @@ struct Stages {
     decltype(&launch_accumulate_bls) accumulate;
     decltype(&launch_fold_bls) fold;
     decltype(&launch_reduce_seg_bls) reduce_seg;
-    decltype(&launch_reduce_window_bls) reduce_window;
+    decltype(&launch_reduce_level_bls) reduce_level;
     decltype(&launch_combine_bls) combine;
     int xyzz_words;
 };
 
-const Stages kBls = {launch_accumulate_bls, launch_fold_bls,     launch_reduce_seg_bls,
-                     launch_reduce_window_bls, launch_combine_bls, 56};
-const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_window_bn,
+const Stages kBls = {launch_accumulate_bls, launch_fold_bls,    launch_reduce_seg_bls,
+                     launch_reduce_level_bls, launch_combine_bls, 56};
+const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_level_bn,
                     launch_combine_bn,    36};
 
 const Stages& stages_of(int curve) { return curve == 2 ? kBls : kBn; }
@@ -133,11 +133,12 @@ const Stages& stages_of(int curve) { return curve == 2 ? kBls : kBn; }
 struct Shape {
     uint32_t nb;        // global buckets
     uint32_t nwin;      // global windows
-    uint32_t t1;        // segments per window (single MSM), 0 = per-window table
+    uint32_t t1;        // segments per window (single MSM), 0 = per-window tables
     uint32_t nseg;      // first-level segments
     uint32_t chunk;     // sorted entries per accumulation thread
     uint32_t levels;    // carry-fold tree levels
     size_t max_entries;
+    size_t level_out;   // largest output count of a reduction level
     uint32_t max_chunks() const { return ceil_div(max_entries, chunk); }
 };
 
@@ -165,12 +166,19 @@ void reserve(const Stages& st, const Shape& sh, MsmWorkspace& ws) {
     ws.segA.ensure((size_t)sh.nseg * pw);
     ws.segB.ensure((size_t)sh.nseg * pw);
     ws.winres.ensure((size_t)sh.nwin * pw);
+    const size_t lv = (sh.level_out > sh.nwin ? sh.level_out : sh.nwin) * pw;
+    for (int i = 0; i < 2; ++i) {
+        ws.lvA[i].ensure(lv);
+        ws.lvB[i].ensure(lv);
+    }
 }
+
+uint32_t parts_of(uint32_t n_in) { return (n_in + kReduceFanIn - 1) / kReduceFanIn; }
 
 // Sort: clear the histogram, count, scan, scatter.  `digits(scatter, counts,
 // entries)` launches the count or scatter pass.
 template <class Digits>
-void run_stages(const Stages& st, const Shape& sh, Digits&& digits, const uint2* d_win_seg,
+void run_stages(const Stages& st, const Shape& sh, Digits&& digits, const MsmBatchPlan* plan,
                 const MsmJob* d_jobs, const MsmJob& one, uint32_t njobs, const uint32_t* d_pts, MsmWorkspace& ws,
                 hipStream_t s, uint8_t* d_out) {
     uint32_t* counts = ws.counts.as<uint32_t>();
@@ -193,10 +201,30 @@ void run_stages(const Stages& st, const Shape& sh, Digits&& digits, const uint2*
         StageTimer t(s, kStageReduce);
         st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, ws.run_bucket.as<uint32_t>(),
                 ws.carry.as<uint32_t>());
-        st.reduce_seg(ceil_div(sh.nseg, 64), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(), ws.carry.as<uint32_t>(),
-                      sh.nseg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
-        st.reduce_window(sh.nwin, s, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>(), sh.t1, d_win_seg,
-                         ws.winres.as<uint32_t>());
+        st.reduce_seg(ceil_div((size_t)sh.nseg * 4, 256), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(),
+                      ws.carry.as<uint32_t>(), sh.nseg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
+        // levels: segment pairs carry weight m = 16 (lm = 4); each level
+        // multiplies it by kReduceFanIn = 2^7
+        const uint32_t* inA = ws.segA.as<uint32_t>();
+        const uint32_t* inB = ws.segB.as<uint32_t>();
+        uint32_t lm = 4, n_in = sh.t1;
+        const size_t nlev = plan ? plan->level_grid.size() : 0;
+        for (size_t lv = 0;; ++lv) {
+            const uint32_t parts = plan ? 0 : parts_of(n_in);
+            const bool final = plan ? lv + 1 == nlev : parts == 1;
+            uint32_t* oA = ws.lvA[lv & 1].as<uint32_t>();
+            uint32_t* oB = final ? ws.winres.as<uint32_t>() : ws.lvB[lv & 1].as<uint32_t>();
+            if (plan)
+                st.reduce_level(plan->level_grid[lv], s, inA, inB, plan->d_levels.as<RedLevel>() + plan->level_off[lv],
+                                sh.nwin, 0, 0, lm, oA, oB);
+            else
+                st.reduce_level(sh.nwin * parts, s, inA, inB, nullptr, 0, n_in, parts, lm, oA, oB);
+            if (final) break;
+            inA = oA;
+            inB = oB;
+            n_in = parts;
+            lm += 7;
+        }
     }
     {
         StageTimer t(s, kStageCombine);
@@ -222,6 +250,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
     sh.levels = fold_levels(n, sh.chunk);
     sh.max_entries = n * (size_t)W;
+    sh.level_out = (size_t)W * parts_of(sh.t1);
     reserve(st, sh, ws);
     MsmJob one{};
     one.n = (uint32_t)n;
@@ -245,6 +274,7 @@ Shape batch_shape(const MsmBatchPlan& p) {
     sh.chunk = p.chunk;
     sh.levels = p.levels;
     sh.max_entries = p.max_entries;
+    sh.level_out = p.level_max_out;
     return sh;
 }
 
@@ -257,7 +287,7 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     plan.njobs = (uint32_t)njobs;
     plan.chunk = (uint32_t)(chunk > 0 ? chunk : 32);
     plan.jobs.assign(specs, specs + njobs);
-    std::vector<uint2> win_seg;
+    std::vector<uint32_t> lv_base, lv_n;  // per window: first segment, segment count
     uint64_t nb = 0, nwin = 0, nseg = 0, blocks = 0;
     size_t entries = 0, max_n = 0;
     for (auto& j : plan.jobs) {
@@ -270,7 +300,10 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
         j.gb_base = (uint32_t)nb;
         j.blk_base = (uint32_t)blocks;
         j.pad = 0;
-        for (uint32_t w = 0; w < W; ++w) win_seg.push_back(make_uint2((uint32_t)(nseg + (uint64_t)w * t1), t1));
+        for (uint32_t w = 0; w < W; ++w) {
+            lv_base.push_back((uint32_t)(nseg + (uint64_t)w * t1));
+            lv_n.push_back(t1);
+        }
         nb += (uint64_t)W * B;
         nwin += W;
         nseg += (uint64_t)W * t1;
@@ -286,10 +319,37 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     plan.dig_blocks = (uint32_t)blocks;
     plan.max_entries = entries;
     plan.levels = fold_levels(max_n, plan.chunk);
+    // reduction levels until every window is down to one part; the last level
+    // writes window w's sum at index w
+    std::vector<RedLevel> tab;
+    plan.level_off.clear();
+    plan.level_grid.clear();
+    plan.level_max_out = 0;
+    for (;;) {
+        bool final = true;
+        for (uint32_t n : lv_n) final = final && parts_of(n) == 1;
+        plan.level_off.push_back((uint32_t)tab.size());
+        uint32_t wg = 0;
+        for (size_t w = 0; w < lv_n.size(); ++w) {
+            const uint32_t parts = parts_of(lv_n[w]);
+            RedLevel d;
+            d.in_base = lv_base[w];
+            d.n_in = lv_n[w];
+            d.out_base = final ? (uint32_t)w : wg;
+            d.wg_base = wg;
+            tab.push_back(d);
+            lv_base[w] = d.out_base;
+            lv_n[w] = parts;
+            wg += parts;
+        }
+        plan.level_grid.push_back(wg);
+        if (final) break;
+        if (wg > plan.level_max_out) plan.level_max_out = wg;
+    }
     plan.d_jobs.ensure(njobs * sizeof(MsmJob));
-    plan.d_win_seg.ensure(win_seg.size() * sizeof(uint2));
+    plan.d_levels.ensure(tab.size() * sizeof(RedLevel));
     KZ_HIP(hipMemcpy(plan.d_jobs.ptr, plan.jobs.data(), njobs * sizeof(MsmJob), hipMemcpyHostToDevice));
-    KZ_HIP(hipMemcpy(plan.d_win_seg.ptr, win_seg.data(), win_seg.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    KZ_HIP(hipMemcpy(plan.d_levels.ptr, tab.data(), tab.size() * sizeof(RedLevel), hipMemcpyHostToDevice));
 }
 
 void msm_batch_reserve(const MsmBatchPlan& plan, MsmWorkspace& ws) {
@@ -305,7 +365,7 @@ void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_
     auto digits = [&](bool scatter, uint32_t* cc, uint32_t* entries) {
         digits_multi_dispatch(scatter, d_scalars, d_inf, d_jobs, plan.njobs, plan.dig_blocks, cc, entries, s);
     };
-    run_stages(st, sh, digits, plan.d_win_seg.as<uint2>(), d_jobs, MsmJob{}, plan.njobs, d_pts, ws, s, d_out);
+    run_stages(st, sh, digits, &plan, d_jobs, MsmJob{}, plan.njobs, d_pts, ws, s, d_out);
 }
 
 // ------------------------------------------------------------ conversions

@@ -14,13 +14,15 @@
 //                      continue an earlier bucket writes them as its carry
 //   k_reduce_seg       per 16-bucket segment: folds the carries into the
 //                      bucket sums, then (sum, weighted sum) by running sums
-//   k_reduce_window    one workgroup per window: LDS suffix scan + tree
+//   k_reduce_level     log-depth levels (LDS suffix scan + tree) down to one
+//                      sum per window
 //   k_combine          Horner over windows, affine normalisation, compression
 //
 // The accumulation inlines its field products (throughput-bound, one compact
-// loop); the latency-bound reduction and combine kernels call one out-of-line
-// field multiply (MulCall) so their code stays inside the instruction cache,
-// and keep few point-operation sites with `#pragma unroll 1` loops.
+// loop).  The latency-bound reduction and combine kernels use quad-cooperative
+// point operations (curve_quad.h); the reduction calls one out-of-line field
+// multiply (MulCall) so its code stays inside the instruction cache, and keeps
+// few point-operation sites with `#pragma unroll 1` loops.
 #pragma once
 
 #include "curve_quad.h"
@@ -156,17 +158,36 @@ k_fold_level(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
 }
 
 // ---------------------------------------------------------- bucket reduce
-constexpr int kSeg = 16;  // buckets per first-level segment
+// Window sum W = sum_b (b+1) S_b over the window's bucket sums, in levels.
+//
+//   k_reduce_seg    per segment t of kSeg = 16 buckets (one quad each): adds
+//                   each bucket's folded carry run to its sum, then by running
+//                   sums from the top  A_t = sum_l S_{16t+l},
+//                   B_t = sum_l (l+1) S_{16t+l},  so  W = sum_t [B_t + 16 t A_t].
+//   k_reduce_level  generic step.  A window's inputs (A_i, B_i), i < n, with
+//                   W = sum_i [B_i + m i A_i]  (m = 2^lm)  are cut into parts of
+//                   kLevelQuads = 128 consecutive inputs; part g yields
+//                   A'_g = sum_k A_{128g+k},  B'_g = sum_k [B_{128g+k} + m k A_{128g+k}],
+//                   and W = sum_g [B'_g + 128 m g A'_g]: the same form with 128x
+//                   fewer inputs.  Levels repeat until one part is left, whose
+//                   B' is W.
+// Inside a part (one workgroup, one quad per input): sum_k k A_k =
+// sum_{k>=1} R_k with the suffix sums R_k = sum_{j>=k} A_j (LDS scan), so
+// B' = sum_k term_k with term_0 = B_0, term_k = B_k + m R_k (LDS tree), and
+// A' = R_0.  The running sums cost 3 point additions per bucket; the levels,
+// which see 16x and more fewer inputs, trade extra additions for depth
+// log2(128) per scan/tree instead of a serial pass over a window.
+constexpr int kSeg = 16;
+constexpr int kLevelQuads = (int)kReduceFanIn;
+constexpr int kLevelThreads = 4 * kLevelQuads;
 
-// Segment t (over all windows): adds each bucket's folded carry run to its
-// sum, then A = sum_l S_l,  Bw = sum_l (l+1) S_l  by running sums from the top.
 template <class F>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ bsum,
              const uint32_t* __restrict__ carry, uint32_t nseg, uint32_t* __restrict__ segA,
              uint32_t* __restrict__ segB) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nseg) return;
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    if (t >= nseg) return;  // whole quads leave together
     Xyzz<F> run = xyzz_inf<F>(), tot = xyzz_inf<F>();
 #pragma unroll 1
     for (int l = kSeg - 1; l >= 0; --l) {
@@ -176,95 +197,90 @@ k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
         if (end > off) {
             S = xyzz_load<F>(bsum, gb);
             const uint32_t head = off / chunk + 1;
-            if ((end - 1) / chunk >= head) S = xyzz_add<F, MulCall>(S, xyzz_load<F>(carry, head));
+            if ((end - 1) / chunk >= head) S = xyzz_add_quad<F>(S, xyzz_load<F>(carry, head));
         }
-#pragma unroll 1
-        for (int op = 0; op < 2; ++op) {
-            const Xyzz<F> a = op == 0 ? run : tot;
-            const Xyzz<F> b = op == 0 ? S : run;
-            const Xyzz<F> r = xyzz_add<F, MulCall>(a, b);
-            if (op == 0) run = r;
-            else tot = r;
-        }
+        run = xyzz_add_quad<F>(run, S);
+        tot = xyzz_add_quad<F>(tot, run);
     }
-    xyzz_store<F>(segA, t, run);
-    xyzz_store<F>(segB, t, tot);
+    if (quad_lane() == 0) {
+        xyzz_store<F>(segA, t, run);
+        xyzz_store<F>(segB, t, tot);
+    }
 }
 
-constexpr int kReduceThreads = 256;
-
-// One workgroup per window over its T1 = B/16 segments:
-//   window = sum_t Bw_t + 16 (U - V),  U = sum_t (t+1) A_t,  V = sum_t A_t.
-// Thread k owns L2 = T1/256 consecutive segments (one if T1 < 256) and forms
-// a_k = sum A, u_k = sum_l (l+1) A_{k L2 + l}, b_k = sum Bw.  With the suffix
-// scan R_k = sum_{j >= k} a_j (so V = R_0):
-//   U - V = sum_k u_k + L2 sum_k R_k - (L2 + 1) R_0,
-// hence window = sum_k [b_k + 16 (u_k + X_k)] with X_k = L2 R_k for k > 0 and
-// X_0 = -R_0, a single tree sum.
-// Windows of one MSM all have t1 segments at w * t1; a batch passes win_seg
-// (per window: first segment, segment count) instead.
+// tab = null: every window has uni_n inputs at w * uni_n and uni_parts parts,
+// written at w * uni_parts + g.  Else per window (in_base, n_in, out_base,
+// wg_base), workgroup b serving the last window with wg_base <= b.
 template <class F>
-__global__ void __launch_bounds__(kReduceThreads)
-k_reduce_window(const uint32_t* __restrict__ segA, const uint32_t* __restrict__ segB, uint32_t t1,
-                const uint2* __restrict__ win_seg, uint32_t* __restrict__ winres) {
-    __shared__ Xyzz<F> sh[kReduceThreads];
-    const int k = threadIdx.x;
-    const uint32_t w = blockIdx.x;
-    size_t base = (size_t)w * t1;
-    if (win_seg) {
-        const uint2 ws = win_seg[w];
-        base = ws.x;
-        t1 = ws.y;
+__global__ void __launch_bounds__(kLevelThreads)
+k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ inB,
+               const RedLevel* __restrict__ tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts, uint32_t lm,
+               uint32_t* __restrict__ outA, uint32_t* __restrict__ outB) {
+    __shared__ Xyzz<F> sh[kLevelQuads];
+    uint32_t in_base, n_in, out_idx, g;
+    if (tab) {
+        uint32_t lo = 0, hi = ntab;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tab[mid].wg_base <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        const RedLevel d = tab[lo];
+        g = blockIdx.x - d.wg_base;
+        in_base = d.in_base;
+        n_in = d.n_in;
+        out_idx = d.out_base + g;
+    } else {
+        const uint32_t w = blockIdx.x / uni_parts;
+        g = blockIdx.x - w * uni_parts;
+        in_base = w * uni_n;
+        n_in = uni_n;
+        out_idx = blockIdx.x;
     }
-    const uint32_t l2 = t1 >= kReduceThreads ? t1 / kReduceThreads : 1u;
-    int log_l2 = 0;
-    while ((1u << log_l2) < l2) ++log_l2;
+    const uint32_t k = threadIdx.x >> 2;
+    const uint32_t first = g * kLevelQuads;
+    const uint32_t cnt = min(n_in - first, (uint32_t)kLevelQuads);
+    uint32_t span = 1;
+    while (span < cnt) span <<= 1;
+    const bool lead = quad_lane() == 0;
 
-    // local running sums over this thread's segments (one add site)
-    Xyzz<F> run = xyzz_inf<F>(), u = xyzz_inf<F>(), bs = xyzz_inf<F>();
-#pragma unroll 1
-    for (int st = 0; st < 3 * (int)l2; ++st) {
-        const uint32_t t = (uint32_t)k * l2 + (l2 - 1 - (uint32_t)(st / 3));
-        const int op = st % 3;
-        if (t >= t1) continue;
-        Xyzz<F> a, b;
-        if (op == 0) { a = run; b = xyzz_load<F>(segA, base + t); }
-        else if (op == 1) { a = u; b = run; }
-        else { a = bs; b = xyzz_load<F>(segB, base + t); }
-        const Xyzz<F> r = xyzz_add<F, MulCall>(a, b);
-        if (op == 0) run = r;
-        else if (op == 1) u = r;
-        else bs = r;
+    Xyzz<F> R = xyzz_inf<F>(), term = xyzz_inf<F>();
+    if (k < cnt) {
+        R = xyzz_load<F>(inA, (size_t)in_base + first + k);
+        term = xyzz_load<F>(inB, (size_t)in_base + first + k);
     }
-    // suffix scan R_k = sum_{j >= k} a_j
-    Xyzz<F> R = run;
-    sh[k] = R;
+    // suffix scan R_k = sum_{j >= k} A_j over the part
+    if (lead) sh[k] = R;
     __syncthreads();
 #pragma unroll 1
-    for (int s = 1; s < kReduceThreads; s <<= 1) {
-        const Xyzz<F> other = (k + s < kReduceThreads) ? sh[k + s] : xyzz_inf<F>();
+    for (uint32_t s = 1; s < span; s <<= 1) {
+        const Xyzz<F> other = (k + s < span) ? sh[k + s] : xyzz_inf<F>();
         __syncthreads();
-        R = xyzz_add<F, MulCall>(R, other);
-        sh[k] = R;
+        if (k < span) R = xyzz_add_quad<F>(R, other);
+        if (lead) sh[k] = R;
         __syncthreads();
     }
-    // term = b_k + 16 (u_k + X_k): one dbl site and one add site
-    Xyzz<F> cur = (k == 0) ? xyzz_neg<F>(R) : R;
-    const int steps = (k == 0 ? 0 : log_l2) + 1 + 4 + 1;
-    const int first_add = (k == 0 ? 0 : log_l2);
+    // term_k = B_k + m R_k (k > 0)
+    if (k > 0 && k < span) {
+        Xyzz<F> mR = R;
 #pragma unroll 1
-    for (int st = 0; st < steps; ++st) {
-        if (st == first_add || st == steps - 1) cur = xyzz_add<F, MulCall>(cur, st == first_add ? u : bs);
-        else cur = xyzz_dbl<F, MulCall>(cur);
+        for (uint32_t j = 0; j < lm; ++j) mR = xyzz_dbl_quad<F>(mR);
+        term = xyzz_add_quad<F>(term, mR);
     }
-    sh[k] = cur;
+    if (lead) sh[k] = term;
     __syncthreads();
 #pragma unroll 1
-    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
-        if (k < s) sh[k] = xyzz_add<F, MulCall>(sh[k], sh[k + s]);
+    for (uint32_t s = span >> 1; s > 0; s >>= 1) {
+        if (k < s) {
+            const Xyzz<F> v = xyzz_add_quad<F>(sh[k], sh[k + s]);
+            if (lead) sh[k] = v;
+        }
         __syncthreads();
     }
-    if (k == 0) xyzz_store<F>(winres, w, sh[0]);
+    if (threadIdx.x == 0) {
+        xyzz_store<F>(outA, out_idx, R);
+        xyzz_store<F>(outB, out_idx, sh[0]);
+    }
 }
 
 // --------------------------------------------------------- combine + output
@@ -318,9 +334,9 @@ __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ win
     for (int w = (int)jd.nwin - 2; w >= 0; --w) {
         if (!xyzz_is_inf<F>(r)) {
 #pragma unroll 1
-            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F>(r);
+            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F, MulInline>(r);
         }
-        r = xyzz_add_quad<F>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
+        r = xyzz_add_quad<F, MulInline>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
     }
     if (quad_lane() == 0) write_output<C>(r, out + (size_t)job * 3 * C::FB);
 }

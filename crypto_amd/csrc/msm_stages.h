@@ -19,9 +19,9 @@ namespace kz {
     void launch_reduce_seg_##sfx(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,     \
                                  const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA, \
                                  uint32_t* segB);                                                           \
-    void launch_reduce_window_##sfx(unsigned nwin, hipStream_t s, const uint32_t* segA,                     \
-                                    const uint32_t* segB, uint32_t t1, const uint2* win_seg,                \
-                                    uint32_t* winres);                                                      \
+    void launch_reduce_level_##sfx(unsigned grid, hipStream_t s, const uint32_t* inA, const uint32_t* inB,  \
+                                   const RedLevel* tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts,  \
+                                   uint32_t lm, uint32_t* outA, uint32_t* outB);                            \
     void launch_combine_##sfx(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,  \
                               uint32_t njobs, uint8_t* out);
 

@@ -1,13 +1,14 @@
-// BN254 G1: per-window bucket reduction kernel (msm_impl.h).
+// BN254 G1: bucket-reduction level kernel (msm_impl.h).
 #include "msm_impl.h"
 #include "msm_stages.h"
 
 namespace kz {
 
-void launch_reduce_window_bn(unsigned nwin, hipStream_t s, const uint32_t* segA, const uint32_t* segB,
-                             uint32_t t1, const uint2* win_seg, uint32_t* winres) {
-    hipLaunchKernelGGL(k_reduce_window<CurveBN254::Fp>, dim3(nwin), dim3(kReduceThreads), 0, s, segA, segB, t1,
-                       win_seg, winres);
+void launch_reduce_level_bn(unsigned grid, hipStream_t s, const uint32_t* inA, const uint32_t* inB,
+                            const RedLevel* tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts, uint32_t lm,
+                            uint32_t* outA, uint32_t* outB) {
+    hipLaunchKernelGGL(k_reduce_level<CurveBN254::Fp>, dim3(grid), dim3(kLevelThreads), 0, s, inA, inB, tab, ntab,
+                       uni_n, uni_parts, lm, outA, outB);
 }
 
 }  // namespace kz
