@@ -13,6 +13,7 @@ struct MsmWorkspace {
     DevBuf entries;
     DevBuf bsum, carry, run_bucket, segA, segB, winres;
     DevBuf lvA[2], lvB[2];  // reduction levels (ping-pong)
+    DevBuf sort_cnt, sort_pre;  // LDS sort: per-(window, part) histograms
     DevBuf out;
 };
 

@@ -14,6 +14,7 @@
 // order inside a bucket is whatever the atomics give; bucket sums do not
 // depend on it (the group is commutative), so results stay bit-exact.
 #include "msm_internal.h"
+#include "scan.h"
 
 namespace kz {
 
@@ -85,6 +86,154 @@ static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf
     else
         hipLaunchKernelGGL((k_digits<CB, true>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, counts,
                            entries);
+}
+
+// ------------------------------------------------------------- LDS sort
+// Single MSMs with c <= 16 sort without global atomics.  Global atomics from
+// all eight XCDs meet at the memory side (each XCD has its own L2), which made
+// the atomic count and scatter passes the slowest part of the sort.  Here a
+// workgroup owns one (window w, part p) pair -- part p = a contiguous range of
+// points -- and counts its digits in an LDS histogram of 2^(c-1) words
+// (128 KiB at c = 16; gfx950 gives a workgroup up to 160 KiB):
+//
+//   k_sort_count    hist[b] -> cnt[w][p][b]
+//   k_sort_colscan  pre[w][p][b] = sum_{p' < p} cnt[w][p'][b],
+//                   tot[w][b] = sum_p cnt[w][p][b]
+//   scan            offsets[w B + b] = exclusive prefix of tot
+//   k_sort_scatter  LDS cursor[b] = offsets[w B + b] + pre[w][p][b];
+//                   each digit takes a slot by an LDS atomic
+//
+// Workgroup ids interleave windows (id = p * wpad + w, wpad a multiple of 8),
+// so the workgroups of one window run on one XCD (ids go round-robin over
+// the eight XCDs) and its scattered entry writes meet in that XCD's L2.
+constexpr int kSortThreads = 1024;
+
+// Signed digit of window w (the carry chain runs over the lower windows).
+template <int CB>
+__device__ __forceinline__ int32_t digit_of_window(const uint32_t (&s)[8], uint32_t w) {
+    using D = Digits<CB>;
+    uint32_t carry = 0;
+    int32_t sel = 0;
+#pragma unroll
+    for (int ww = 0; ww < D::W; ++ww) {
+        int32_t d = (int32_t)(window_bits<CB>(s, ww) + carry);
+        carry = 0;
+        if (d > (int32_t)D::B) {
+            d -= (int32_t)(1u << CB);
+            carry = 1;
+        }
+        if ((uint32_t)ww == w) sel = d;
+    }
+    return sel;
+}
+
+template <int CB>
+__global__ void __launch_bounds__(kSortThreads)
+k_sort_count(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n, uint32_t per_part,
+             uint32_t wpad, uint32_t nparts, uint32_t* __restrict__ cnt) {
+    using D = Digits<CB>;
+    __shared__ uint32_t hist[D::B];
+    const uint32_t w = blockIdx.x % wpad, p = blockIdx.x / wpad;
+    if (w >= (uint32_t)D::W) return;  // padding workgroup (whole group leaves)
+    for (uint32_t b = threadIdx.x; b < D::B; b += kSortThreads) hist[b] = 0;
+    __syncthreads();
+    const uint32_t lo = p * per_part, hi = min(n, lo + per_part);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
+        if (inf && inf[i]) continue;
+        uint32_t s[8];
+        load_scalar(scalars, i, s);
+        const int32_t d = digit_of_window<CB>(s, w);
+        if (d != 0) atomicAdd(&hist[(d > 0 ? d : -d) - 1], 1u);
+    }
+    __syncthreads();
+    uint32_t* out = cnt + ((size_t)w * nparts + p) * D::B;
+    for (uint32_t b = threadIdx.x; b < D::B; b += kSortThreads) out[b] = hist[b];
+}
+
+// One thread per (window, bucket): prefix over the parts.
+__global__ void __launch_bounds__(256)
+k_sort_colscan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ pre, uint32_t* __restrict__ tot,
+               uint32_t nwin, uint32_t nparts, uint32_t nbw) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nwin * nbw) return;
+    const uint32_t w = t / nbw, b = t - w * nbw;
+    uint32_t run = 0;
+    for (uint32_t p = 0; p < nparts; ++p) {
+        const size_t idx = ((size_t)w * nparts + p) * nbw + b;
+        const uint32_t c = cnt[idx];
+        pre[idx] = run;
+        run += c;
+    }
+    tot[t] = run;
+}
+
+template <int CB>
+__global__ void __launch_bounds__(kSortThreads)
+k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n,
+               uint32_t per_part, uint32_t wpad, uint32_t nparts, const uint32_t* __restrict__ offsets,
+               const uint32_t* __restrict__ pre, uint32_t* __restrict__ entries) {
+    using D = Digits<CB>;
+    __shared__ uint32_t cur[D::B];
+    const uint32_t w = blockIdx.x % wpad, p = blockIdx.x / wpad;
+    if (w >= (uint32_t)D::W) return;
+    const uint32_t* pp = pre + ((size_t)w * nparts + p) * D::B;
+    for (uint32_t b = threadIdx.x; b < D::B; b += kSortThreads) cur[b] = offsets[w * D::B + b] + pp[b];
+    __syncthreads();
+    const uint32_t lo = p * per_part, hi = min(n, lo + per_part);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
+        if (inf && inf[i]) continue;
+        uint32_t s[8];
+        load_scalar(scalars, i, s);
+        const int32_t d = digit_of_window<CB>(s, w);
+        if (d != 0) {
+            const uint32_t pos = atomicAdd(&cur[(d > 0 ? d : -d) - 1], 1u);
+            entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
+        }
+    }
+}
+
+bool lds_sort_supported(int c, size_t n) { return c >= 5 && c <= 16 && n >= 4096; }
+
+void lds_sort_shape(int c, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part) {
+    const uint32_t W = (uint32_t)msm_num_windows(c);
+    wpad = (W + 7) / 8 * 8;
+    // about two workgroups per CU over the whole launch, parts of >= 4096 points
+    uint32_t parts = 1;
+    while (parts * W < 512 && ((size_t)parts * 2) * 4096 <= n) parts *= 2;
+    nparts = parts;
+    per_part = (uint32_t)((n + parts - 1) / parts);
+}
+
+template <int CB>
+static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
+                            uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
+                            uint32_t* entries, hipStream_t s) {
+    using D = Digits<CB>;
+    uint32_t wpad, nparts, per_part;
+    lds_sort_shape(CB, n, wpad, nparts, per_part);
+    const unsigned grid = wpad * nparts;
+    hipLaunchKernelGGL(k_sort_count<CB>, dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
+                       nparts, cnt);
+    const uint32_t nb = (uint32_t)D::W * D::B;
+    hipLaunchKernelGGL(k_sort_colscan, dim3(ceil_div(nb, 256)), dim3(256), 0, s, cnt, pre, tot, (uint32_t)D::W,
+                       nparts, D::B);
+    device_exclusive_scan(tot, offsets, nullptr, nb, block_sums, total, s);
+    hipLaunchKernelGGL(k_sort_scatter<CB>, dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
+                       nparts, offsets, pre, entries);
+}
+
+void lds_sort_dispatch(int c, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
+                       uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
+                       uint32_t* entries, hipStream_t s) {
+    switch (c) {
+#define KZ_C(CB)                                                                                            \
+    case CB:                                                                                                \
+        lds_sort_launch<CB>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s); \
+        break;
+        KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15) KZ_C(16)
+#undef KZ_C
+        default: throw std::invalid_argument("LDS sort: window bits out of range [5, 16]");
+    }
 }
 
 // Batch form: the window size is a per-job runtime value, so the windows are

@@ -175,22 +175,28 @@ void reserve(const Stages& st, const Shape& sh, MsmWorkspace& ws) {
 
 uint32_t parts_of(uint32_t n_in) { return (n_in + kReduceFanIn - 1) / kReduceFanIn; }
 
-// Sort: clear the histogram, count, scan, scatter.  `digits(scatter, counts,
-// entries)` launches the count or scatter pass.
+// Global-atomic sort: clear the histogram, count, scan, scatter.
+// `digits(scatter, counts, entries)` launches the count or scatter pass.
 template <class Digits>
-void run_stages(const Stages& st, const Shape& sh, Digits&& digits, const MsmBatchPlan* plan,
+void atomic_sort(const Shape& sh, Digits&& digits, MsmWorkspace& ws, hipStream_t s) {
+    uint32_t* counts = ws.counts.as<uint32_t>();
+    device_fill_u32(counts, 0, sh.nb, s);
+    digits(false, counts, nullptr);
+    device_exclusive_scan(counts, ws.offsets.as<uint32_t>(), ws.cursor.as<uint32_t>(), sh.nb,
+                          ws.block_sums.as<uint32_t>(), ws.total.as<uint32_t>(), s);
+    digits(true, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>());
+}
+
+// `sort()` fills ws.offsets and ws.entries.
+template <class Sort>
+void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPlan* plan,
                 const MsmJob* d_jobs, const MsmJob& one, uint32_t njobs, const uint32_t* d_pts, MsmWorkspace& ws,
                 hipStream_t s, uint8_t* d_out) {
-    uint32_t* counts = ws.counts.as<uint32_t>();
     uint32_t* offsets = ws.offsets.as<uint32_t>();
     const uint32_t max_chunks = sh.max_chunks();
     {
         StageTimer t(s, kStageSort);
-        device_fill_u32(counts, 0, sh.nb, s);
-        digits(false, counts, nullptr);
-        device_exclusive_scan(counts, offsets, ws.cursor.as<uint32_t>(), sh.nb, ws.block_sums.as<uint32_t>(),
-                              ws.total.as<uint32_t>(), s);
-        digits(true, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>());
+        sort();
     }
     {
         StageTimer t(s, kStageAccumulate);
@@ -256,10 +262,24 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     one.n = (uint32_t)n;
     one.c = (uint32_t)c;
     one.nwin = (uint32_t)W;
-    auto digits = [&](bool scatter, uint32_t* cc, uint32_t* entries) {
-        digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)n, cc, entries, s);
+    const bool lds = lds_sort_supported(c, n);
+    if (lds) {
+        const size_t words = lds_sort_words(c, n);
+        ws.sort_cnt.ensure(words * sizeof(uint32_t));
+        ws.sort_pre.ensure(words * sizeof(uint32_t));
+    }
+    auto sort = [&] {
+        if (lds) {
+            lds_sort_dispatch(c, d_scalars, d_inf, (uint32_t)n, ws.sort_cnt.as<uint32_t>(), ws.sort_pre.as<uint32_t>(),
+                              ws.counts.as<uint32_t>(), ws.offsets.as<uint32_t>(), ws.block_sums.as<uint32_t>(),
+                              ws.total.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
+        } else {
+            atomic_sort(sh, [&](bool scatter, uint32_t* cc, uint32_t* entries) {
+                digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)n, cc, entries, s);
+            }, ws, s);
+        }
     };
-    run_stages(st, sh, digits, nullptr, nullptr, one, 1, d_pts, ws, s, d_out);
+    run_stages(st, sh, sort, nullptr, nullptr, one, 1, d_pts, ws, s, d_out);
 }
 
 // ------------------------------------------------------------------ batches
@@ -362,10 +382,12 @@ void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_
     const Shape sh = batch_shape(plan);
     reserve(st, sh, ws);
     const MsmJob* d_jobs = plan.d_jobs.as<MsmJob>();
-    auto digits = [&](bool scatter, uint32_t* cc, uint32_t* entries) {
-        digits_multi_dispatch(scatter, d_scalars, d_inf, d_jobs, plan.njobs, plan.dig_blocks, cc, entries, s);
+    auto sort = [&] {
+        atomic_sort(sh, [&](bool scatter, uint32_t* cc, uint32_t* entries) {
+            digits_multi_dispatch(scatter, d_scalars, d_inf, d_jobs, plan.njobs, plan.dig_blocks, cc, entries, s);
+        }, ws, s);
     };
-    run_stages(st, sh, digits, &plan, d_jobs, MsmJob{}, plan.njobs, d_pts, ws, s, d_out);
+    run_stages(st, sh, sort, &plan, d_jobs, MsmJob{}, plan.njobs, d_pts, ws, s, d_out);
 }
 
 // ------------------------------------------------------------ conversions

@@ -81,11 +81,12 @@ def test_msm_large_closed_form(ca, curve, n):
 @pytest.mark.parametrize("curve", [1, 2])
 def test_msm_every_window_size(ca, curve):
     """All instantiated window sizes (incl. c = 8, 16 with the carry window)
-    must give the same bytes, also for scalars >= 2^255."""
+    must give the same bytes, also for scalars >= 2^255.  n = 5000 takes the
+    LDS-histogram sort for c <= 16 and the global-atomic sort for c = 17, 18."""
     import torch
     from crypto_amd import device as D
 
-    n = 3000
+    n = 5000
     k0, d = 77, 0x9999999999999999
     points = O.gen_points_progression(curve, k0, d, n)
     rnd = random.Random(curve)
@@ -103,6 +104,36 @@ def test_msm_every_window_size(ca, curve):
         assert out[:pb] == want, c
         assert out[pb:] == O.compress(curve, want), c
     del torch
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("c", [0, 7, 16])
+def test_msm_sort_paths_agree_with_identity_points(ca, curve, c):
+    """Identity points (all-zero wire) and zero scalars inside an MSM large
+    enough for the LDS sort; the same MSM with c = 17 (global-atomic sort)
+    and the closed form over the non-identity points must agree."""
+    from crypto_amd import device as D
+
+    n = 20000
+    k0, d = 4321, 0x77777777
+    pb = O.POINT_BYTES[curve]
+    points = bytearray(O.gen_points_progression(curve, k0, d, n))
+    zero_at = set(range(0, n, 97))
+    for i in zero_at:
+        points[i * pb:(i + 1) * pb] = bytes(pb)
+    rnd = random.Random(31 + c)
+    sc = [0 if i % 89 == 0 else rnd.getrandbits(255) for i in range(n)]
+    scalars = b"".join(s.to_bytes(32, "little") for s in sc)
+    r = M.CURVES[curve]["r"]
+    t = sum(s * (k0 + i * d) for i, s in enumerate(sc) if i not in zero_at) % r
+    want = O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+    pp = D.PreparedPoints(curve, bytes(points))
+    d_sc = D.to_device(scalars)
+    print(f"sort paths curve={curve} c={c}", flush=True)
+    got = D.msm_prepared(pp, d_sc, window_bits=c).cpu().numpy().tobytes()
+    alt = D.msm_prepared(pp, d_sc, window_bits=17).cpu().numpy().tobytes()
+    assert got[:pb] == want
+    assert alt == got
 
 
 @pytest.mark.parametrize("curve", [1, 2])
