@@ -42,6 +42,7 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the BN254 and one-shot extra figures")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-versus-plain job set")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="open-loop serving run length (0 = skip)")
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 256 MiB device copy "
                          "(the HBM-counter calibration kernel), no JSON line")
@@ -298,6 +299,79 @@ def batched_set(lib, torch, D, reps: int, seed: int = 4242):
     }
 
 
+def serving_run(lib, torch, seconds: float, seed: int = 77):
+    """Commitment serving: 70% BLS12-381 requests with n = 4096, 30% BN254 with
+    n = 2^K, K uniform on {14..20}; evaluations uniform canonical Fr.
+    Capacity is measured closed-loop (a burst of requests, all submitted at
+    once) for the server and for plain one-at-a-time kzg_commit on the same
+    request sample; then an open-loop Poisson run at 75% of the server's
+    capacity gives commits/s and latency percentiles (submit -> completion,
+    timed by the server)."""
+    import numpy as np
+
+    import crypto_amd
+    from crypto_amd import inputs, serve
+
+    tau = 0x5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED
+    srs = {2: crypto_amd.KzgSrs.generate(2, tau, 4096), 1: crypto_amd.KzgSrs.generate(1, tau, 1 << 20)}
+    pool = {2: inputs.scalars(2, seed, 4096).view(np.uint8).reshape(-1).copy(),
+            1: inputs.scalars(1, seed + 1, 1 << 20).view(np.uint8).reshape(-1).copy()}
+
+    def make(rng):
+        if rng.random() < 0.7:
+            return 2, pool[2]
+        n = 1 << int(rng.integers(14, 21))
+        return 1, pool[1][: 32 * n]
+
+    server = serve.KzgServer(srs)
+    # warm up: every size once (twiddle tables, plans, workspaces)
+    warm = [server.submit(2, pool[2])] + [server.submit(1, pool[1][: 32 << k]) for k in range(14, 21)]
+    for tk in warm:
+        server.wait(tk)
+    rng = np.random.default_rng(seed)
+    sample = [make(rng) for _ in range(600)]
+    # closed loop: server
+    t0 = time.perf_counter()
+    tks = [server.submit(c, ev) for c, ev in sample]
+    res = [server.wait(tk)[0] for tk in tks]
+    cap_server = len(sample) / (time.perf_counter() - t0)
+    # plain: one kzg_commit after another (first 150 of the same sample)
+    plain = sample[:150]
+    blobs = [(c, ev.tobytes()) for c, ev in plain]
+    t0 = time.perf_counter()
+    alone = [srs[c].commit(b) for c, b in blobs]
+    cap_plain = len(plain) / (time.perf_counter() - t0)
+    identical = all(a == b for a, b in zip(alone, res[: len(plain)]))
+    # open loop at 75% of capacity
+    rate = 0.75 * cap_server
+    done, wall = serve.poisson_load(server, make, rate=rate, duration=seconds, seed=seed + 2)
+    st = server.stats()
+    server.close()
+    for s in srs.values():
+        s.close()
+    lat = np.array([d[2] for d in done]) * 1e3
+    lat_bls = np.array([d[2] for d in done if d[0] == 2]) * 1e3
+    lat_bn = np.array([d[2] for d in done if d[0] == 1]) * 1e3
+
+    def pct(a, p):
+        return round(float(np.percentile(a, p)), 3) if a.size else None
+
+    return {
+        "workload": "70% BLS12-381 n=4096, 30% BN254 n=2^K K~U{14..20}; iNTT -> MSM -> compressed commitment",
+        "capacity_commits_per_s": {"server": round(cap_server, 1), "plain_kzg_commit": round(cap_plain, 1)},
+        "capacity_speedup": round(cap_server / cap_plain, 3),
+        "poisson_rate_per_s": round(rate, 1),
+        "seconds": round(wall, 2),
+        "commits_per_s": round(len(done) / wall, 1),
+        "requests": len(done),
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99),
+                       "bls12_381_n4096": {"p50": pct(lat_bls, 50), "p99": pct(lat_bls, 99)},
+                       "bn254": {"p50": pct(lat_bn, 50), "p99": pct(lat_bn, 99)}},
+        "alone_vs_mix_bytes_identical": identical,
+        "server_stats": st,
+    }
+
+
 def load_traffic(curve, log2n):
     """HBM bytes per accumulation launch from the committed rocprof summary."""
     path = os.path.join(ROOT, "profiles", f"msm_accumulate_{CURVE_NAMES[curve]}_2p{log2n}_traffic.txt")
@@ -395,6 +469,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(curve, ext["scalars"], ext["points_wire"], args.cpu_log2n)
     if not args.no_batched:
         line["batched"] = batched_set(lib, torch, D, reps=3)
+    if args.serve_seconds > 0:
+        line["serving"] = serving_run(lib, torch, args.serve_seconds)
     if not args.no_extras:
         extras = {}
         other = 1 if curve == 2 else 2

@@ -8,6 +8,7 @@
 #include "field.h"
 #include "kinet/crypto/hip_runtime.h"
 #include "kinet/crypto/kzg.h"
+#include "kzg_internal.h"
 #include "msm.h"
 #include "ntt.h"
 
@@ -15,19 +16,6 @@ namespace kz {
 void fixed_base_mul(int curve, const uint32_t* d_scalars, size_t n, uint8_t* d_out_wire, hipStream_t s);
 }
 
-struct kzg_srs {
-    int curve = 0;
-    size_t n = 0;
-    kz::DevBuf pts;  // Montgomery affine records
-    kz::DevBuf inf;  // identity flags
-    kz::DevBuf wire; // wire points (kept for export)
-};
-
-struct kzg_ctx {
-    hipStream_t stream = nullptr;
-    kz::MsmWorkspace ws;
-    kz::DevBuf coeffs, tmp;
-};
 
 namespace kz {
 
@@ -45,7 +33,7 @@ __global__ void __launch_bounds__(256) k_powers(const uint32_t* __restrict__ tau
     fe_to_words<F>(fe_from_mont<F>(acc), out + (size_t)i * 8, 8);
 }
 
-static bool canonical_fr(int curve, const uint8_t* le32) {
+bool canonical_fr(int curve, const uint8_t* le32) {
     // r of BLS12-381 / BN254 as big-endian bytes
     static const uint8_t r_bls[32] = {0x73, 0xED, 0xA7, 0x53, 0x29, 0x9D, 0x7D, 0x48, 0x33, 0x39, 0xD8,
                                       0x08, 0x09, 0xA1, 0xD8, 0x05, 0x53, 0xBD, 0xA4, 0x02, 0xFF, 0xFE,
@@ -61,7 +49,7 @@ static bool canonical_fr(int curve, const uint8_t* le32) {
     return false;
 }
 
-static int log2_exact(size_t n) {
+int log2_exact(size_t n) {
     int l = 0;
     while (((size_t)1 << l) < n) ++l;
     return ((size_t)1 << l) == n ? l : -1;
@@ -216,21 +204,6 @@ static int commit_host(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8
 
 int kzg_commit(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out) {
     return commit_host(srs, evals, n, commit_out);
-}
-
-int kzg_commit_batch(const kzg_srs* const* srs, const uint8_t* const* evals, const size_t* n, size_t count,
-                     uint8_t* commits_out) {
-    if (count == 0) return KZ_CRYPTO_OK;
-    if (!srs || !evals || !n || !commits_out || !srs[0]) return KZ_CRYPTO_ERR_INPUT;
-    const int curve = srs[0]->curve;
-    const size_t cb = compressed_bytes(curve);
-    for (size_t i = 0; i < count; ++i)
-        if (!srs[i] || srs[i]->curve != curve) return KZ_CRYPTO_ERR_INPUT;
-    for (size_t i = 0; i < count; ++i) {
-        const int rc = commit_host(srs[i], evals[i], n[i], commits_out + cb * i);
-        if (rc != KZ_CRYPTO_OK) return rc;
-    }
-    return KZ_CRYPTO_OK;
 }
 
 int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* result) {

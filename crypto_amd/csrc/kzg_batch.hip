@@ -1,0 +1,145 @@
+// Batched commitments: many polynomials of one curve in one launch sequence
+// (one inverse-NTT launch per distinct size, one batched MSM over the SRS
+// prefix).  Used by kzg_commit_batch and by the serving loop (serve.hip).
+#include <algorithm>
+#include <cstring>
+
+#include "engine.h"
+#include "kinet/crypto/kzg.h"
+#include "kzg_internal.h"
+#include "ntt.h"
+#include "scan.h"
+
+namespace kz {
+
+CommitLane::CommitLane(bool own_stream) : owns_stream(own_stream) {
+    if (own_stream) KZ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    KZ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+}
+
+CommitLane::~CommitLane() {
+    if (done) (void)hipEventDestroy(done);
+    if (owns_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+size_t commit_batch_in_bytes(const std::vector<uint32_t>& n) {
+    size_t total = 0;
+    for (uint32_t v : n) total += v;
+    return 32 * total;
+}
+
+size_t commit_batch_out_bytes(int curve, size_t count) { return count * msm_out_bytes(curve) + 16; }
+
+void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector<uint32_t>& n) {
+    const int curve = srs->curve;
+    const size_t count = n.size();
+    size_t total = 0;
+    for (uint32_t v : n) total += v;
+    hipStream_t s = lane.stream;
+    uint32_t* ev = static_cast<uint32_t*>(lane.evals.ensure(32 * total));
+    uint32_t* co = static_cast<uint32_t*>(lane.coeffs.ensure(32 * total));
+    uint32_t* tmp = static_cast<uint32_t*>(lane.tmp.ensure(32 * total));
+    const size_t rec = msm_out_bytes(curve);
+    uint8_t* out = static_cast<uint8_t*>(lane.out.ensure(commit_batch_out_bytes(curve, count)));
+    int* err = reinterpret_cast<int*>(out + count * rec);
+    device_fill_u32(reinterpret_cast<uint32_t*>(err), 0, 1, s);
+    KZ_HIP(hipMemcpyAsync(ev, lane.pin_in.ptr, 32 * total, hipMemcpyHostToDevice, s));
+    const int field = curve == 2 ? 0 : 1;
+    size_t off = 0;
+    for (size_t i = 0; i < count;) {
+        size_t j = i;
+        while (j < count && n[j] == n[i]) ++j;
+        ntt_device_batch(field, log2_exact(n[i]), true, ev + off * 8, co + off * 8, tmp + off * 8, j - i, err, s);
+        off += (size_t)n[i] * (j - i);
+        i = j;
+    }
+    auto key = std::make_pair(curve, n);
+    auto it = lane.plans.find(key);
+    if (it == lane.plans.end()) {
+        if (lane.plans.size() >= 256) lane.plans.clear();
+        std::vector<MsmJob> specs(count);
+        uint64_t sc = 0;
+        for (size_t i = 0; i < count; ++i) {
+            specs[i] = MsmJob{};
+            specs[i].n = n[i];
+            specs[i].sc_base = sc;
+            sc += n[i];
+        }
+        auto plan = std::make_unique<MsmBatchPlan>();
+        msm_batch_plan(curve, specs.data(), count, 0, *plan);
+        it = lane.plans.emplace(key, std::move(plan)).first;
+    }
+    msm_batch_run(*it->second, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), co, lane.ws, s, out);
+    KZ_HIP(hipMemcpyAsync(lane.pin_out.ensure(commit_batch_out_bytes(curve, count)), out, count * rec + sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipEventRecord(lane.done, s));
+}
+
+static CommitLane& engine_lane() {
+    Engine& e = engine();
+    static CommitLane* lane = [&e] {
+        CommitLane* l = new CommitLane(false);
+        l->stream = e.stream;
+        return l;
+    }();
+    return *lane;
+}
+
+}  // namespace kz
+
+using namespace kz;
+
+extern "C" {
+
+// Jobs that share an SRS handle run as one batch (sizes sorted so that equal
+// sizes share inverse-NTT launches); different handles run one after another.
+int kzg_commit_batch(const kzg_srs* const* srs, const uint8_t* const* evals, const size_t* n, size_t count,
+                     uint8_t* commits_out) {
+    if (count == 0) return KZ_CRYPTO_OK;
+    if (!srs || !evals || !n || !commits_out || !srs[0]) return KZ_CRYPTO_ERR_INPUT;
+    const int curve = srs[0]->curve;
+    const size_t cb = compressed_bytes(curve);
+    for (size_t i = 0; i < count; ++i) {
+        if (!srs[i] || srs[i]->curve != curve || !evals[i]) return KZ_CRYPTO_ERR_INPUT;
+        const int lg = log2_exact(n[i]);
+        if (n[i] == 0 || lg < 0 || lg > (curve == 2 ? 32 : 28) || n[i] > srs[i]->n || n[i] > (1u << 26))
+            return KZ_CRYPTO_ERR_INPUT;
+    }
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        CommitLane& lane = engine_lane();
+        std::map<const kzg_srs*, std::vector<size_t>> groups;
+        for (size_t i = 0; i < count; ++i) groups[srs[i]].push_back(i);
+        std::vector<uint8_t> result(cb * count);
+        for (auto& grp : groups) {
+            std::vector<size_t>& idx = grp.second;
+            std::stable_sort(idx.begin(), idx.end(), [n](size_t a, size_t b) { return n[a] < n[b]; });
+            std::vector<uint32_t> sizes;
+            for (size_t i : idx) sizes.push_back((uint32_t)n[i]);
+            uint8_t* pin = static_cast<uint8_t*>(lane.pin_in.ensure(commit_batch_in_bytes(sizes)));
+            lane.pin_out.ensure(commit_batch_out_bytes(curve, idx.size()));
+            size_t off = 0;
+            for (size_t i : idx) {
+                std::memcpy(pin + off, evals[i], 32 * n[i]);
+                off += 32 * n[i];
+            }
+            commit_batch_launch(lane, grp.first, sizes);
+            KZ_HIP(hipEventSynchronize(lane.done));
+            const uint8_t* po = lane.pin_out.as<uint8_t>();
+            const size_t rec = msm_out_bytes(curve);
+            int err;
+            std::memcpy(&err, po + idx.size() * rec, sizeof(int));
+            if (err) return KZ_CRYPTO_ERR_INPUT;
+            for (size_t k = 0; k < idx.size(); ++k)
+                std::memcpy(result.data() + cb * idx[k], po + k * rec + point_wire_bytes(curve), cb);
+        }
+        std::memcpy(commits_out, result.data(), cb * count);
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Internal state behind include/kinet/crypto/kzg.h and kzg_serve.h: the SRS
+// and context handles, and the batched commit pipeline shared by
+// kzg_commit_batch and the serving loop.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "msm.h"
+
+struct kzg_srs {
+    int curve = 0;
+    size_t n = 0;
+    kz::DevBuf pts;   // Montgomery affine records
+    kz::DevBuf inf;   // identity flags
+    kz::DevBuf wire;  // wire points (kept for export)
+};
+
+struct kzg_ctx {
+    hipStream_t stream = nullptr;
+    kz::MsmWorkspace ws;
+    kz::DevBuf coeffs, tmp;
+};
+
+namespace kz {
+
+// value < r for a 32-byte LE element of the curve's scalar field
+bool canonical_fr(int curve, const uint8_t* le32);
+// log2(n) if n is a power of two, else -1
+int log2_exact(size_t n);
+
+// Host memory from hipHostMalloc (page-locked), grow-only.
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    void* ensure(size_t bytes) {
+        if (bytes > cap) {
+            if (ptr) KZ_HIP(hipHostFree(ptr));
+            ptr = nullptr;
+            cap = 0;
+            KZ_HIP(hipHostMalloc(&ptr, bytes < 4096 ? 4096 : bytes, hipHostMallocDefault));
+            cap = bytes < 4096 ? 4096 : bytes;
+        }
+        return ptr;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// One stream with everything a batch of commitments needs: device buffers
+// for evaluations / coefficients / scratch, the MSM workspace, page-locked
+// staging both ways, cached MSM batch plans (keyed by the job sizes) and a
+// completion event.  A lane runs one batch at a time.
+struct CommitLane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    MsmWorkspace ws;
+    DevBuf evals, coeffs, tmp, out, err;
+    PinnedBuf pin_in, pin_out;
+    std::map<std::pair<int, std::vector<uint32_t>>, std::unique_ptr<MsmBatchPlan>> plans;
+
+    explicit CommitLane(bool own_stream);
+    ~CommitLane();
+    CommitLane(const CommitLane&) = delete;
+    CommitLane& operator=(const CommitLane&) = delete;
+    bool owns_stream = false;
+};
+
+// Launches commitments to `count` polynomials of one curve against `srs` on
+// lane.stream: evaluations (32-byte LE, natural order; sizes n[i], powers of
+// two, sorted ascending) already staged back to back in lane.pin_in.  H2D,
+// one inverse-NTT launch sequence per distinct size, one batched MSM over the
+// SRS prefix, D2H of the records (wire + compressed) and the error flag into
+// lane.pin_out: record i at pin_out + i * msm_out_bytes, the flag (int) after
+// the last record.  Records the lane's done event.  Nothing synchronises.
+void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector<uint32_t>& n);
+// Bytes of lane.pin_in / lane.pin_out a batch of these sizes needs.
+size_t commit_batch_in_bytes(const std::vector<uint32_t>& n);
+size_t commit_batch_out_bytes(int curve, size_t count);
+
+}  // namespace kz

@@ -100,6 +100,14 @@ def library():
         lib.kinet_hip_msm_batch_run.argtypes = [vp, vp, vp, vp, vp, i32, vp]
         lib.kinet_hip_msm_set_window_table.argtypes = [u32, i32, ctypes.POINTER(i32), i32]
         lib.kinet_hip_msm_window_bits.argtypes = [u32, sz, i32]
+        lib.kzg_server_create.argtypes = [vp, vp, vp, pp]
+        lib.kzg_server_destroy.argtypes = [vp]
+        lib.kzg_server_destroy.restype = None
+        lib.kzg_server_submit.argtypes = [vp, u32, vp, sz]
+        lib.kzg_server_submit.restype = ctypes.c_int64
+        lib.kzg_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]
+        lib.kzg_server_get_stats.argtypes = [vp, vp]
         _lib = lib
     return _lib
 
