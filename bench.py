@@ -405,11 +405,13 @@ def main():
 
     if args.profile_run:
         timed_msm(lib, torch, D, curve, n, 20261203, 2, 0, args.window_bits, profile=False)
-        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        # calibration: a plain 1 GiB device copy of initialised data (1 GiB read + 1 GiB written)
+        src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
+        torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        print("profile run done: 2 MSMs + 256 MiB copy", flush=True)
+        print("profile run done: 2 MSMs + 1 GiB copy", flush=True)
         return
 
     ms, stage, result, check, ext = timed_msm(lib, torch, D, curve, n, 20261203, args.steps, args.warmup,
