@@ -44,7 +44,7 @@ def parse_args():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-versus-plain job set")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="open-loop serving run length (0 = skip)")
     ap.add_argument("--profile-run", action="store_true",
-                    help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 256 MiB device copy "
+                    help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 1 GiB device copy "
                          "(the HBM-counter calibration kernel), no JSON line")
     return ap.parse_args()
 
@@ -372,6 +372,82 @@ def serving_run(lib, torch, seconds: float, seed: int = 77):
     }
 
 
+def ntt_and_kzg(lib, torch, p_mad: float, reps: int = 10):
+    """Fr NTT throughput (2^20, 2^22; both fields) against the VALU and HBM
+    rooflines, and KZG commitments at n = 2^22 on both curves (device-resident
+    evaluations and SRS; iNTT -> MSM -> compressed point, per commit)."""
+    import crypto_amd
+
+    stream = torch.cuda.Stream()
+    out = {"ntt": {}, "kzg_2p22": {}}
+    with torch.cuda.stream(stream):
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(99)
+        for field, fname in ((0, "bls12_381_fr"), (1, "bn254_fr")):
+            for logn in (20, 22):
+                n = 1 << logn
+                d_in = torch.randint(0, 256, (32 * n,), dtype=torch.uint8, device="cuda", generator=gen)
+                d_in.view(-1, 32)[:, 31] &= 0x1F
+                d_out = torch.empty_like(d_in)
+                d_tmp = torch.empty_like(d_in)
+                d_err = torch.zeros(1, dtype=torch.int32, device="cuda")
+                for inverse in (0, 1):
+                    def call():
+                        rc = lib.kinet_hip_ntt_fr(field, inverse, d_in.data_ptr(), d_out.data_ptr(), d_tmp.data_ptr(),
+                                                  n, d_err.data_ptr(), stream.cuda_stream)
+                        assert rc == 0, rc
+                    call()
+                    stream.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(reps):
+                        call()
+                    e1.record(stream)
+                    stream.synchronize()
+                    ms = e0.elapsed_time(e1) / reps
+                    mads = (n // 2) * logn * 136 + (n if inverse else 0) * 136
+                    passes = 1 if logn <= 10 else (2 if logn <= 20 else 3)
+                    out["ntt"][f"{fname}_2p{logn}_{'inverse' if inverse else 'forward'}"] = {
+                        "ms": round(ms, 4),
+                        "melems_per_s": round(n / ms / 1e3, 1),
+                        "hbm_gbs_compulsory": round(64 * n / (ms * 1e-3) / 1e9, 1),
+                        "hbm_gbs_all_passes": round(64 * n * passes / (ms * 1e-3) / 1e9, 1),
+                        "valu_frac": round(mads / (ms * 1e-3) / p_mad, 4),
+                    }
+                assert int(d_err.item()) == 0
+                del d_in, d_out, d_tmp
+        tau = 0x0123456789ABCDEF0123456789ABCDEF0123456789ABCDEF0123456789ABCDEF % (1 << 252)
+        for curve in (2, 1):
+            n = 1 << 22
+            srs = crypto_amd.KzgSrs.generate(curve, tau, n)
+            ctx = ctypes.c_void_p()
+            assert lib.kzg_ctx_create(ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ctx)) == 0
+            d_ev = torch.randint(0, 256, (32 * n,), dtype=torch.uint8, device="cuda", generator=gen)
+            d_ev.view(-1, 32)[:, 31] &= 0x1F
+            d_rec = torch.empty(160, dtype=torch.uint8, device="cuda")
+            d_err = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+            def commit():
+                rc = lib.kzg_commit_async(ctx, srs.handle, d_ev.data_ptr(), n, 0, d_rec.data_ptr(), d_err.data_ptr())
+                assert rc == 0, rc
+            commit()
+            stream.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                commit()
+            e1.record(stream)
+            stream.synchronize()
+            ms = e0.elapsed_time(e1) / 3
+            out["kzg_2p22"][CURVE_NAMES[curve]] = {"ms_per_commit": round(ms, 3),
+                                                   "commits_per_s": round(1e3 / ms, 2),
+                                                   "error_flag": int(d_err.item())}
+            lib.kzg_ctx_destroy(ctx)
+            srs.close()
+            del d_ev
+    return out
+
+
 def load_traffic(curve, log2n):
     """HBM bytes per accumulation launch from the committed rocprof summary."""
     path = os.path.join(ROOT, "profiles", f"msm_accumulate_{CURVE_NAMES[curve]}_2p{log2n}_traffic.txt")
@@ -479,6 +555,7 @@ def main():
         oms, _, ores, ocheck, _ = timed_msm(lib, torch, D, other, n, 777, max(3, args.steps // 2), 2, 0, False)
         extras[f"{CURVE_NAMES[other]}_mpoints_s"] = round(n / oms / 1e3, 3)
         extras[f"{CURVE_NAMES[other]}_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
+        extras.update(ntt_and_kzg(lib, torch, p_mad))
         line["extras"] = extras
     print(json.dumps(line), flush=True)
 

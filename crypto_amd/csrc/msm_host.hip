@@ -162,7 +162,7 @@ void reserve(const Stages& st, const Shape& sh, MsmWorkspace& ws) {
     ws.entries.ensure((sh.max_entries + 1) * sizeof(uint32_t));
     ws.bsum.ensure((size_t)sh.nb * pw);
     ws.carry.ensure((size_t)(max_chunks + 1) * pw);
-    ws.run_bucket.ensure((size_t)(max_chunks + 1) * sizeof(uint32_t));
+    ws.run_bucket.ensure((size_t)(max_chunks + 1 + 2) * sizeof(uint32_t));  // fold list + 2 control words
     ws.segA.ensure((size_t)sh.nseg * pw);
     ws.segB.ensure((size_t)sh.nseg * pw);
     ws.winres.ensure((size_t)sh.nwin * pw);
@@ -206,7 +206,7 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
     {
         StageTimer t(s, kStageReduce);
         st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, ws.run_bucket.as<uint32_t>(),
-                ws.carry.as<uint32_t>());
+                ws.run_bucket.as<uint32_t>() + max_chunks + 1, ws.carry.as<uint32_t>());
         st.reduce_seg(ceil_div((size_t)sh.nseg * 4, 256), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(),
                       ws.carry.as<uint32_t>(), sh.nseg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
         // levels: segment pairs carry weight m = 16 (lm = 4); each level

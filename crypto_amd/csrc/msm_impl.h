@@ -82,7 +82,7 @@ __device__ __forceinline__ uint32_t bucket_of(const uint32_t* __restrict__ offse
 }
 
 template <class F>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 3)
 k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets, uint32_t nb,
              uint32_t chunk, const uint32_t* __restrict__ pts, uint32_t* __restrict__ bsum,
              uint32_t* __restrict__ carry) {
@@ -116,45 +116,67 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
 
 // ------------------------------------------------------------ carry fold
 // The carries of a bucket that spans chunks j0..j1 (j0 = the chunk holding its
-// first entry) sit in carry[j0+1..j1], a contiguous run.  Runs are long when a
-// bucket is huge -- e.g. the top window covers only the few scalar bits above
-// the last full window, so its handful of buckets hold most points -- and a
-// serial fold would put the whole run on one lane.  Instead the runs are
-// folded by a pairwise tree: level L adds carry[j + 2^L] into carry[j] for run
-// positions r = 0 mod 2^(L+1); after ceil(log2(longest run)) levels the run's
-// head carry[j0+1] holds its total.
+// first entry) sit in carry[j0+1..j1], a contiguous run whose total must end
+// up in its head carry[j0+1].  With random scalars almost every run has one
+// or two carries; runs are long only when a bucket is huge -- e.g. the top
+// window covers only the few scalar bits above the last full window, so its
+// handful of buckets hold most points -- and a serial fold would then put the
+// whole run on one lane.  So:
+//   k_carry_runs   a run of at most kShortRun carries is folded serially by
+//                  the thread of its head chunk; each chunk of a longer run
+//                  is appended to a compact list (and the longest length is
+//                  kept);
+//   k_fold_level   pairwise tree over the listed chunks: level L adds
+//                  carry[j + 2^L] into carry[j] for run positions
+//                  r = 0 mod 2^(L+1); levels past log2(longest run) return at
+//                  once.  Grid-stride over the list, so a launch is small
+//                  whatever the number of chunks.
+// ctl[0] = list length, ctl[1] = longest listed run (both zeroed before).
+// (Templated only so that each per-curve translation unit owns its copy.)
+constexpr uint32_t kShortRun = 8;
+constexpr uint32_t kFoldThreads = 64 * 1024;
 
-// run_bucket[j] = bucket whose run chunk j belongs to, or ~0 if chunk j starts
-// a bucket (or lies past the end).  (Templated only so that each per-curve
-// translation unit owns its copy.)
 template <class F>
 __global__ void __launch_bounds__(256)
 k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
-             uint32_t* __restrict__ run_bucket) {
+             uint32_t* __restrict__ list, uint32_t* __restrict__ ctl, uint32_t* __restrict__ carry) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nchunks) return;
     const uint32_t start = j * chunk;
-    uint32_t rb = 0xFFFFFFFFu;
-    if (start < offsets[nb]) {
-        const uint32_t g = bucket_of(offsets, nb, start);
-        if (offsets[g] < start) rb = g;
+    if (start >= offsets[nb]) return;
+    const uint32_t g = bucket_of(offsets, nb, start);
+    if (offsets[g] >= start) return;  // chunk j starts its bucket: no carry
+    const uint32_t head = offsets[g] / chunk + 1;
+    const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
+    if (len <= kShortRun) {
+        if (j != head || len == 1) return;
+        Xyzz<F> acc = xyzz_load<F>(carry, head);
+#pragma unroll 1
+        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add<F, MulCall>(acc, xyzz_load<F>(carry, head + k));
+        xyzz_store<F>(carry, head, acc);
+        return;
     }
-    run_bucket[j] = rb;
+    list[atomicAdd(&ctl[0], 1u)] = j;
+    if (j == head) atomicMax(&ctl[1], len);
 }
 
 template <class F>
 __global__ void __launch_bounds__(64)
-k_fold_level(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ run_bucket,
-             uint32_t nchunks, uint32_t level, uint32_t* __restrict__ carry) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nchunks) return;
-    const uint32_t g = run_bucket[j];
-    if (g == 0xFFFFFFFFu) return;
-    const uint32_t head = offsets[g] / chunk + 1;
-    const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
-    const uint32_t r = j - head, s = 1u << level;
-    if ((r & (2 * s - 1)) != 0 || r + s >= len) return;
-    xyzz_store<F>(carry, j, xyzz_add<F, MulCall>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s)));
+k_fold_level(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, const uint32_t* __restrict__ list,
+             const uint32_t* __restrict__ ctl, uint32_t level, uint32_t* __restrict__ carry) {
+    const uint32_t s = 1u << level;
+    if (s >= ctl[1]) return;
+    const uint32_t count = ctl[0];
+#pragma unroll 1
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t j = list[i];
+        const uint32_t g = bucket_of(offsets, nb, j * chunk);
+        const uint32_t head = offsets[g] / chunk + 1;
+        const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
+        const uint32_t r = j - head;
+        if ((r & (2 * s - 1)) != 0 || r + s >= len) continue;
+        xyzz_store<F>(carry, j, xyzz_add<F, MulCall>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s)));
+    }
 }
 
 // ---------------------------------------------------------- bucket reduce

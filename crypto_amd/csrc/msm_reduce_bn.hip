@@ -1,16 +1,19 @@
 // BN254 G1: first-level bucket reduction kernel (msm_impl.h).
 #include "msm_impl.h"
 #include "msm_stages.h"
+#include "scan.h"
 
 namespace kz {
 
 void launch_fold_bn(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
-                      uint32_t levels, uint32_t* run_bucket, uint32_t* carry) {
-    hipLaunchKernelGGL(k_carry_runs<CurveBN254::Fp>, dim3(ceil_div(nchunks, 256)), dim3(256), 0, s, offsets, nb, chunk, nchunks,
-                       run_bucket);
+                    uint32_t levels, uint32_t* list, uint32_t* ctl, uint32_t* carry) {
+    device_fill_u32(ctl, 0, 2, s);
+    hipLaunchKernelGGL(k_carry_runs<CurveBN254::Fp>, dim3(ceil_div(nchunks, 256)), dim3(256), 0, s, offsets, nb, chunk,
+                       nchunks, list, ctl, carry);
+    const unsigned grid = ceil_div(nchunks < kFoldThreads ? nchunks : kFoldThreads, 64);
     for (uint32_t level = 0; level < levels; ++level)
-        hipLaunchKernelGGL(k_fold_level<CurveBN254::Fp>, dim3(ceil_div(nchunks, 64)), dim3(64), 0, s, offsets, chunk,
-                           run_bucket, nchunks, level, carry);
+        hipLaunchKernelGGL(k_fold_level<CurveBN254::Fp>, dim3(grid), dim3(64), 0, s, offsets, nb, chunk, list, ctl, level,
+                           carry);
 }
 
 void launch_reduce_seg_bn(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,

@@ -15,7 +15,7 @@ namespace kz {
                                  const uint32_t* offsets, uint32_t nb, uint32_t chunk, const uint32_t* pts, \
                                  uint32_t* bsum, uint32_t* carry);                                          \
     void launch_fold_##sfx(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk,             \
-                           uint32_t nchunks, uint32_t levels, uint32_t* run_bucket, uint32_t* carry);       \
+                           uint32_t nchunks, uint32_t levels, uint32_t* list, uint32_t* ctl, uint32_t* carry); \
     void launch_reduce_seg_##sfx(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,     \
                                  const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA, \
                                  uint32_t* segB);                                                           \
