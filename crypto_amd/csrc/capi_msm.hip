@@ -126,8 +126,8 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
     }
 }
 
-int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
-                           size_t n, int window_bits, int chunk, void* d_out, void* stream) {
+static int msm_prepared_impl(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                             size_t n, int window_bits, int chunk, void* d_out, void* stream, bool g1) {
     if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
     if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
@@ -145,6 +145,7 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
         }
         MsmParams prm;
         prm.c = window_bits;
+        prm.g1 = g1;
         if (chunk > 0) prm.chunk = chunk;
         msm_device((int)curve, static_cast<const uint32_t*>(d_pts), static_cast<const uint8_t*>(d_inf),
                    static_cast<const uint32_t*>(d_scalars), n, prm, e.ws, s, static_cast<uint8_t*>(d_out));
@@ -153,6 +154,16 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
         set_last_error(ex.what());
         return GPUKIT_ERR_BACKEND;
     }
+}
+
+int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                           size_t n, int window_bits, int chunk, void* d_out, void* stream) {
+    return msm_prepared_impl(curve, d_scalars, d_pts, d_inf, n, window_bits, chunk, d_out, stream, false);
+}
+
+int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                              size_t n, int window_bits, int chunk, void* d_out, void* stream) {
+    return msm_prepared_impl(curve, d_scalars, d_pts, d_inf, n, window_bits, chunk, d_out, stream, true);
 }
 
 void kinet_hip_profile_enable(int on) { msm_set_profiling(on); }

@@ -44,11 +44,44 @@ struct Xyzz {
 // Multiplication policy of the point operations.  MulInline expands every
 // field product in place (the hot accumulation loop); MulCall calls one
 // out-of-line copy, keeping latency-bound kernels (bucket reduction, window
-// combine) inside the instruction cache.  Operands and result travel in
-// VGPRs, so the call needs no scratch.
+// combine) inside the instruction cache.
+//
+// The out-of-line copy takes its operands as 2 x 14 plain 32-bit arguments
+// (limbs past N are zero), which the calling convention keeps in VGPRs, and
+// returns the product in VGPRs: no stack traffic.  (Passed as two Fe structs,
+// the second one would go through the stack as a byval copy in scratch.)
 template <class F>
-__host__ __device__ __noinline__ Fe<F> fe_mul_call(Fe<F> a, Fe<F> b) {
+__host__ __device__ __noinline__ Fe<F> fe_mul_flat(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t a4,
+                                                  uint32_t a5, uint32_t a6, uint32_t a7, uint32_t a8, uint32_t a9,
+                                                  uint32_t a10, uint32_t a11, uint32_t a12, uint32_t a13,
+                                                  uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3, uint32_t b4,
+                                                  uint32_t b5, uint32_t b6, uint32_t b7, uint32_t b8, uint32_t b9,
+                                                  uint32_t b10, uint32_t b11, uint32_t b12, uint32_t b13) {
+    const uint32_t av[14] = {a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13};
+    const uint32_t bv[14] = {b0, b1, b2, b3, b4, b5, b6, b7, b8, b9, b10, b11, b12, b13};
+    Fe<F> a, b;
+    for (int i = 0; i < F::N; ++i) {
+        a.v[i] = av[i];
+        b.v[i] = bv[i];
+    }
     return fe_mul<F>(a, b);
+}
+
+template <int I, class F>
+KZ_DEV uint32_t limb_or_zero(const Fe<F>& a) {
+    if constexpr (I < F::N) return a.v[I];
+    else return 0u;
+}
+
+template <class F>
+KZ_DEV Fe<F> fe_mul_call(const Fe<F>& a, const Fe<F>& b) {
+    return fe_mul_flat<F>(limb_or_zero<0>(a), limb_or_zero<1>(a), limb_or_zero<2>(a), limb_or_zero<3>(a),
+                          limb_or_zero<4>(a), limb_or_zero<5>(a), limb_or_zero<6>(a), limb_or_zero<7>(a),
+                          limb_or_zero<8>(a), limb_or_zero<9>(a), limb_or_zero<10>(a), limb_or_zero<11>(a),
+                          limb_or_zero<12>(a), limb_or_zero<13>(a), limb_or_zero<0>(b), limb_or_zero<1>(b),
+                          limb_or_zero<2>(b), limb_or_zero<3>(b), limb_or_zero<4>(b), limb_or_zero<5>(b),
+                          limb_or_zero<6>(b), limb_or_zero<7>(b), limb_or_zero<8>(b), limb_or_zero<9>(b),
+                          limb_or_zero<10>(b), limb_or_zero<11>(b), limb_or_zero<12>(b), limb_or_zero<13>(b));
 }
 
 struct MulInline {

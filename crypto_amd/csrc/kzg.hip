@@ -11,6 +11,8 @@
 #include "kzg_internal.h"
 #include "msm.h"
 #include "ntt.h"
+#include "glv.h"
+#include "scan.h"
 
 namespace kz {
 void fixed_base_mul(int curve, const uint32_t* d_scalars, size_t n, uint8_t* d_out_wire, hipStream_t s);
@@ -75,6 +77,7 @@ static void commit_device(kzg_ctx* ctx, const kzg_srs* srs, const uint32_t* d_ev
     const int field = srs->curve == 2 ? 0 : 1;
     ntt_device(field, logn, true, d_evals, coeffs, tmp, d_err, ctx->stream);
     MsmParams prm;
+    prm.g1 = srs->g1;
     prm.c = window_bits;
     msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), coeffs, n, prm, ctx->ws, ctx->stream,
                d_out);
@@ -106,7 +109,14 @@ int kzg_srs_create(uint32_t curve, const uint8_t* points, size_t n, kzg_srs** ou
         srs->curve = (int)curve;
         srs->n = n;
         srs_upload(srs.get(), points, true, e.stream);
+        // membership in G1 licenses the GLV split of the MSMs against this SRS
+        int* d_bad = static_cast<int*>(e.out.ensure(sizeof(int)));
+        device_fill_u32(reinterpret_cast<uint32_t*>(d_bad), 0, 1, e.stream);
+        check_g1(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), n, d_bad, e.stream);
+        int bad = 1;
+        KZ_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, e.stream));
         KZ_HIP(hipStreamSynchronize(e.stream));
+        srs->g1 = bad == 0;
         *out = srs.release();
         return KZ_CRYPTO_OK;
     } catch (const std::exception& ex) {
@@ -144,6 +154,7 @@ int kzg_srs_generate(uint32_t curve, const uint8_t tau[32], size_t n, kzg_srs** 
             std::lock_guard<std::mutex> g(e.mu);
             srs_upload(srs.get(), nullptr, false, s);
         }
+        srs->g1 = true;  // tau^i G lies in G1 by construction
         KZ_HIP(hipStreamSynchronize(s));
         *out = srs.release();
         return KZ_CRYPTO_OK;
@@ -221,6 +232,7 @@ int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* resul
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
         KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));
         MsmParams prm;
+        prm.g1 = srs->g1;
         msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
         KZ_HIP(hipStreamSynchronize(s));
@@ -243,6 +255,7 @@ int kzg_commit_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, uint8
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
         KZ_HIP(hipMemcpyAsync(d_sc, coeffs, 32 * n, hipMemcpyHostToDevice, s));
         MsmParams prm;
+        prm.g1 = srs->g1;
         msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(commit_out, d_out + point_wire_bytes(srs->curve), compressed_bytes(srs->curve),
                               hipMemcpyDeviceToHost, s));

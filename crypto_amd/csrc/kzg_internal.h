@@ -16,6 +16,7 @@ struct kzg_srs {
     kz::DevBuf pts;   // Montgomery affine records
     kz::DevBuf inf;   // identity flags
     kz::DevBuf wire;  // wire points (kept for export)
+    bool g1 = false;  // every point lies in G1 (generated, or checked at creation)
 };
 
 struct kzg_ctx {

@@ -14,12 +14,14 @@ struct MsmWorkspace {
     DevBuf bsum, carry, run_bucket, segA, segB, winres;
     DevBuf lvA[2], lvB[2];  // reduction levels (ping-pong)
     DevBuf sort_cnt, sort_pre;  // LDS sort: per-(window, part) histograms
+    DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 128-bit scalars
     DevBuf out;
 };
 
 struct MsmParams {
-    int c = 0;       // window bits; 0 = choose from n
-    int chunk = 32;  // sorted entries per accumulation thread
+    int c = 0;         // window bits; 0 = choose from n
+    int chunk = 32;    // sorted entries per accumulation thread
+    bool g1 = false;   // caller guarantees every point lies in G1: BLS12-381 may use the GLV split
 };
 
 // Device descriptor of one MSM inside a batch.  A batch runs every job's
@@ -48,11 +50,12 @@ struct RedLevel {
 };
 
 int msm_default_window_bits(size_t n);
-// Number of windows for c-bit signed digits over 256-bit scalars, including
-// the extra window that takes the final carry when c divides 256.
-inline int msm_num_windows(int c) {
-    int w = (256 + c - 1) / c;
-    if (w * c == 256) ++w;
+// Number of windows for c-bit signed digits over `bits`-bit scalars (256;
+// 128 for the halves of a GLV split), including the extra window that takes
+// the final carry when c divides the bit count.
+inline int msm_num_windows(int c, int bits = 256) {
+    int w = (bits + c - 1) / c;
+    if (w * c == bits) ++w;
     return w;
 }
 

@@ -18,10 +18,13 @@
 
 namespace kz {
 
-template <int CB>
+// Windows of c-bit signed digits over BITS-bit scalars (256, or 128 for the
+// halves of a GLV split), with the extra window for the final carry when c
+// divides BITS.
+template <int CB, int BITS = 256>
 struct Digits {
-    static constexpr int W0 = (256 + CB - 1) / CB;
-    static constexpr int W = W0 + ((W0 * CB == 256) ? 1 : 0);
+    static constexpr int W0 = (BITS + CB - 1) / CB;
+    static constexpr int W = W0 + ((W0 * CB == BITS) ? 1 : 0);
     static constexpr uint32_t B = 1u << (CB - 1);
 };
 
@@ -43,6 +46,18 @@ __device__ __forceinline__ void load_scalar(const uint32_t* __restrict__ scalars
     uint4 a = q[0], b = q[1];
     s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+}
+
+// Scalars of at most BITS bits: 128-bit ones skip the (zero) upper half.
+template <int BITS>
+__device__ __forceinline__ void load_scalar_bits(const uint32_t* __restrict__ scalars, uint32_t i, uint32_t (&s)[8]) {
+    if (BITS > 128) {
+        load_scalar(scalars, i, s);
+    } else {
+        const uint4 a = reinterpret_cast<const uint4*>(scalars + (size_t)i * 8)[0];
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+        s[4] = s[5] = s[6] = s[7] = 0;
+    }
 }
 
 template <int CB, bool SCATTER>
@@ -109,9 +124,9 @@ static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf
 constexpr int kSortThreads = 1024;
 
 // Signed digit of window w (the carry chain runs over the lower windows).
-template <int CB>
+template <int CB, int BITS>
 __device__ __forceinline__ int32_t digit_of_window(const uint32_t (&s)[8], uint32_t w) {
-    using D = Digits<CB>;
+    using D = Digits<CB, BITS>;
     uint32_t carry = 0;
     int32_t sel = 0;
 #pragma unroll
@@ -127,11 +142,11 @@ __device__ __forceinline__ int32_t digit_of_window(const uint32_t (&s)[8], uint3
     return sel;
 }
 
-template <int CB>
+template <int CB, int BITS>
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_count(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n, uint32_t per_part,
              uint32_t wpad, uint32_t nparts, uint32_t* __restrict__ cnt) {
-    using D = Digits<CB>;
+    using D = Digits<CB, BITS>;
     __shared__ uint32_t hist[D::B];
     const uint32_t w = blockIdx.x % wpad, p = blockIdx.x / wpad;
     if (w >= (uint32_t)D::W) return;  // padding workgroup (whole group leaves)
@@ -141,8 +156,8 @@ k_sort_count(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ i
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
         if (inf && inf[i]) continue;
         uint32_t s[8];
-        load_scalar(scalars, i, s);
-        const int32_t d = digit_of_window<CB>(s, w);
+        load_scalar_bits<BITS>(scalars, i, s);
+        const int32_t d = digit_of_window<CB, BITS>(s, w);
         if (d != 0) atomicAdd(&hist[(d > 0 ? d : -d) - 1], 1u);
     }
     __syncthreads();
@@ -167,12 +182,12 @@ k_sort_colscan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ pre, uin
     tot[t] = run;
 }
 
-template <int CB>
+template <int CB, int BITS>
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n,
                uint32_t per_part, uint32_t wpad, uint32_t nparts, const uint32_t* __restrict__ offsets,
                const uint32_t* __restrict__ pre, uint32_t* __restrict__ entries) {
-    using D = Digits<CB>;
+    using D = Digits<CB, BITS>;
     __shared__ uint32_t cur[D::B];
     const uint32_t w = blockIdx.x % wpad, p = blockIdx.x / wpad;
     if (w >= (uint32_t)D::W) return;
@@ -183,8 +198,8 @@ k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
         if (inf && inf[i]) continue;
         uint32_t s[8];
-        load_scalar(scalars, i, s);
-        const int32_t d = digit_of_window<CB>(s, w);
+        load_scalar_bits<BITS>(scalars, i, s);
+        const int32_t d = digit_of_window<CB, BITS>(s, w);
         if (d != 0) {
             const uint32_t pos = atomicAdd(&cur[(d > 0 ? d : -d) - 1], 1u);
             entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
@@ -194,8 +209,8 @@ k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
 
 bool lds_sort_supported(int c, size_t n) { return c >= 5 && c <= 16 && n >= 4096; }
 
-void lds_sort_shape(int c, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part) {
-    const uint32_t W = (uint32_t)msm_num_windows(c);
+void lds_sort_shape(int c, int bits, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part) {
+    const uint32_t W = (uint32_t)msm_num_windows(c, bits);
     wpad = (W + 7) / 8 * 8;
     // about two workgroups per CU over the whole launch, parts of >= 4096 points
     uint32_t parts = 1;
@@ -204,31 +219,35 @@ void lds_sort_shape(int c, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t&
     per_part = (uint32_t)((n + parts - 1) / parts);
 }
 
-template <int CB>
+template <int CB, int BITS>
 static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
                             uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
                             uint32_t* entries, hipStream_t s) {
-    using D = Digits<CB>;
+    using D = Digits<CB, BITS>;
     uint32_t wpad, nparts, per_part;
-    lds_sort_shape(CB, n, wpad, nparts, per_part);
+    lds_sort_shape(CB, BITS, n, wpad, nparts, per_part);
     const unsigned grid = wpad * nparts;
-    hipLaunchKernelGGL(k_sort_count<CB>, dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
+    hipLaunchKernelGGL((k_sort_count<CB, BITS>), dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
                        nparts, cnt);
     const uint32_t nb = (uint32_t)D::W * D::B;
     hipLaunchKernelGGL(k_sort_colscan, dim3(ceil_div(nb, 256)), dim3(256), 0, s, cnt, pre, tot, (uint32_t)D::W,
                        nparts, D::B);
     device_exclusive_scan(tot, offsets, nullptr, nb, block_sums, total, s);
-    hipLaunchKernelGGL(k_sort_scatter<CB>, dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
+    hipLaunchKernelGGL((k_sort_scatter<CB, BITS>), dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
                        nparts, offsets, pre, entries);
 }
 
-void lds_sort_dispatch(int c, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
+void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
                        uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
                        uint32_t* entries, hipStream_t s) {
+    if (bits != 256 && bits != 128) throw std::invalid_argument("LDS sort: scalar bits must be 256 or 128");
     switch (c) {
-#define KZ_C(CB)                                                                                            \
-    case CB:                                                                                                \
-        lds_sort_launch<CB>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s); \
+#define KZ_C(CB)                                                                                                     \
+    case CB:                                                                                                         \
+        if (bits == 256)                                                                                             \
+            lds_sort_launch<CB, 256>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s);  \
+        else                                                                                                         \
+            lds_sort_launch<CB, 128>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s);  \
         break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15) KZ_C(16)
 #undef KZ_C

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <vector>
 
+#include "glv.h"
 #include "msm_internal.h"
 #include "msm_stages.h"
 #include "scan.h"
@@ -143,7 +144,7 @@ struct Shape {
 };
 
 // A bucket holds at most n entries (one per point), so its carry run is at
-// most ceil(n / chunk) long: that many tree levels always suffice.
+// most ceil(n / chunk) + 1 long: log2 of that many tree levels suffice.
 uint32_t fold_levels(size_t max_n, uint32_t chunk) {
     uint32_t levels = 0;
     while (((size_t)1 << levels) < (max_n + chunk - 1) / chunk + 1) ++levels;
@@ -244,9 +245,29 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(curve);
-    const int c = prm.c ? prm.c : msm_window_bits(curve, n, false);
+    // GLV (BLS12-381, points in G1): 2n points with 128-bit scalars
+    int bits = 256;
+    size_t npts = n;
+    int c = prm.c;
+    if (prm.g1 && curve == 2 && n <= (1u << 29)) {
+        const int cg = c ? c : msm_window_bits(curve, 2 * n, false);
+        if (lds_sort_supported(cg, 2 * n)) {
+            bits = 128;
+            npts = 2 * n;
+            c = cg;
+            const size_t pw = (size_t)affine_words(curve) * 4;
+            uint32_t* pts2 = static_cast<uint32_t*>(ws.glv_pts.ensure(2 * n * pw));
+            uint8_t* inf2 = static_cast<uint8_t*>(ws.glv_inf.ensure(2 * n));
+            uint32_t* sc2 = static_cast<uint32_t*>(ws.glv_sc.ensure(2 * n * 32));
+            glv_expand(curve, d_pts, d_inf, d_scalars, n, pts2, inf2, sc2, s);
+            d_pts = pts2;
+            d_inf = inf2;
+            d_scalars = sc2;
+        }
+    }
+    if (!c) c = msm_window_bits(curve, n, false);
     if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
-    const int W = msm_num_windows(c);
+    const int W = msm_num_windows(c, bits);
     const uint32_t B = 1u << (c - 1);
     Shape sh;
     sh.nb = (uint32_t)W * B;
@@ -254,28 +275,28 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     sh.t1 = B / kSegBuckets;
     sh.nseg = (uint32_t)W * sh.t1;
     sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
-    sh.levels = fold_levels(n, sh.chunk);
-    sh.max_entries = n * (size_t)W;
+    sh.levels = fold_levels(npts, sh.chunk);
+    sh.max_entries = npts * (size_t)W;
     sh.level_out = (size_t)W * parts_of(sh.t1);
     reserve(st, sh, ws);
     MsmJob one{};
-    one.n = (uint32_t)n;
+    one.n = (uint32_t)npts;
     one.c = (uint32_t)c;
     one.nwin = (uint32_t)W;
-    const bool lds = lds_sort_supported(c, n);
+    const bool lds = lds_sort_supported(c, npts);
     if (lds) {
-        const size_t words = lds_sort_words(c, n);
+        const size_t words = lds_sort_words(c, bits, npts);
         ws.sort_cnt.ensure(words * sizeof(uint32_t));
         ws.sort_pre.ensure(words * sizeof(uint32_t));
     }
     auto sort = [&] {
         if (lds) {
-            lds_sort_dispatch(c, d_scalars, d_inf, (uint32_t)n, ws.sort_cnt.as<uint32_t>(), ws.sort_pre.as<uint32_t>(),
-                              ws.counts.as<uint32_t>(), ws.offsets.as<uint32_t>(), ws.block_sums.as<uint32_t>(),
-                              ws.total.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
+            lds_sort_dispatch(c, bits, d_scalars, d_inf, (uint32_t)npts, ws.sort_cnt.as<uint32_t>(),
+                              ws.sort_pre.as<uint32_t>(), ws.counts.as<uint32_t>(), ws.offsets.as<uint32_t>(),
+                              ws.block_sums.as<uint32_t>(), ws.total.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
         } else {
             atomic_sort(sh, [&](bool scatter, uint32_t* cc, uint32_t* entries) {
-                digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)n, cc, entries, s);
+                digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)npts, cc, entries, s);
             }, ws, s);
         }
     };

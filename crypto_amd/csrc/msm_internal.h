@@ -15,13 +15,13 @@ void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8
 // (W 2^(c-1) + 1 words) and entries.  cnt and pre need lds_sort_words(c, n)
 // words each, tot W 2^(c-1) words.
 bool lds_sort_supported(int c, size_t n);
-void lds_sort_shape(int c, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part);
-inline size_t lds_sort_words(int c, size_t n) {
+void lds_sort_shape(int c, int bits, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part);
+inline size_t lds_sort_words(int c, int bits, size_t n) {
     uint32_t wpad, nparts, per_part;
-    lds_sort_shape(c, n, wpad, nparts, per_part);
-    return (size_t)msm_num_windows(c) * nparts * ((size_t)1 << (c - 1));
+    lds_sort_shape(c, bits, n, wpad, nparts, per_part);
+    return (size_t)msm_num_windows(c, bits) * nparts * ((size_t)1 << (c - 1));
 }
-void lds_sort_dispatch(int c, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
+void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
                        uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
                        uint32_t* entries, hipStream_t s);
 

@@ -44,20 +44,34 @@ class PreparedPoints:
 
 
 def msm_prepared(pp: PreparedPoints, d_scalars: torch.Tensor, n: int | None = None, window_bits: int = 0,
-                 out: torch.Tensor | None = None, stream=None, sync: bool = True, chunk: int = 0) -> torch.Tensor:
-    """MSM over prepared points; returns the device output record (wire || compressed)."""
+                 out: torch.Tensor | None = None, stream=None, sync: bool = True, chunk: int = 0,
+                 g1: bool = False) -> torch.Tensor:
+    """MSM over prepared points; returns the device output record (wire || compressed).
+    g1=True: the caller guarantees the points lie in G1 (kinet_hip_msm_prepared_g1)."""
     lib = require_device()
     n = pp.n if n is None else n
     pb = point_bytes(pp.curve)
     if out is None:
         out = torch.empty(pb + pb // 2, dtype=torch.uint8, device=d_scalars.device)
-    rc = lib.kinet_hip_msm_prepared(pp.curve, d_scalars.data_ptr(), pp.pts.data_ptr(), pp.inf.data_ptr(), n,
-                                    window_bits, chunk, out.data_ptr(), stream_handle(stream))
+    fn = lib.kinet_hip_msm_prepared_g1 if g1 else lib.kinet_hip_msm_prepared
+    rc = fn(pp.curve, d_scalars.data_ptr(), pp.pts.data_ptr(), pp.inf.data_ptr(), n, window_bits, chunk,
+            out.data_ptr(), stream_handle(stream))
     if rc != GPUKIT_OK:
-        raise KinetError("kinet_hip_msm_prepared", rc, GPUKIT_STATUS)
+        raise KinetError(fn.__name__, rc, GPUKIT_STATUS)
     if sync:
         (stream or torch.cuda.current_stream()).synchronize()
     return out
+
+
+def check_g1(pp: PreparedPoints) -> bool:
+    """True if every non-identity prepared point is on the curve and in G1."""
+    lib = require_device()
+    bad = torch.zeros(1, dtype=torch.int32, device=pp.pts.device)
+    rc = lib.kinet_hip_points_check_g1(pp.curve, pp.pts.data_ptr(), pp.inf.data_ptr(), pp.n, bad.data_ptr(),
+                                       stream_handle())
+    if rc != GPUKIT_OK:
+        raise KinetError("kinet_hip_points_check_g1", rc, GPUKIT_STATUS)
+    return int(bad.item()) == 0
 
 
 class MsmBatch:

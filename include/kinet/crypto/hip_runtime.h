@@ -36,6 +36,22 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                            size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
+/* The same MSM for points the caller guarantees to lie in G1 (for example
+ * checked once by kinet_hip_points_check_g1, or generated as multiples of the
+ * generator).  BLS12-381 then uses the GLV endomorphism phi(x, y) = (beta x, y)
+ * = lambda (x, y): each 256-bit scalar is split as s = s1 + lambda s2 mod r
+ * with 128-bit halves, giving a 2n-point MSM over half the windows.  Results
+ * equal kinet_hip_msm_prepared's for points of G1; for other points they are
+ * unspecified.  BN254 (cofactor 1) runs the plain path. */
+int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                              size_t n, int window_bits, int chunk, void* d_out, void* stream);
+
+/* Sets *d_bad (device int, zeroed by the caller) nonzero if any non-identity
+ * prepared point is off the curve or, for BLS12-381, outside G1 (tested as
+ * phi(P) = lambda P).  Asynchronous on `stream`. */
+int kinet_hip_points_check_g1(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int* d_bad,
+                              void* stream);
+
 /* ------------------------------------------------------------- batches
  * Many independent MSMs as one pipeline: every stage is launched once for
  * all jobs (bucket ids, windows and outputs of the jobs laid end to end), so

@@ -180,3 +180,84 @@ def test_fixed_base_generator(ca, curve):
     got = out.cpu().numpy().tobytes()
     for i, k in enumerate(ks):
         assert got[pb * i: pb * (i + 1)] == O.g1_mul_gen(curve, k.to_bytes(32, "little")), (i, k)
+
+
+def _non_g1_point(curve):
+    """A point of E(Fp) outside G1 (BLS12-381 has a large cofactor)."""
+    p, b = M.CURVES[curve]["p"], M.CURVES[curve]["b"]
+    for x in range(1, 1000):
+        y = M.sqrt(curve, (x * x * x + b) % p)
+        if y is not None:
+            P = (x, y)
+            if M.mul(curve, P, M.CURVES[curve]["r"]) is not None:
+                return P
+    raise AssertionError("no point found")
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_check_g1(ca, curve):
+    """kinet_hip_points_check_g1: multiples of G pass (identity points are
+    skipped); a point off the curve fails; for BLS12-381 a point of E(Fp)
+    outside G1 fails."""
+    from crypto_amd import device as D
+
+    pb = O.POINT_BYTES[curve]
+    good = O.gen_points_progression(curve, 11, 13, 300) + bytes(pb)
+    assert D.check_g1(D.PreparedPoints(curve, good))
+    off = bytearray(good)
+    off[pb - 1] ^= 1  # flip the low bit of the first point's y
+    assert not D.check_g1(D.PreparedPoints(curve, bytes(off)))
+    if curve == 2:
+        P = _non_g1_point(curve)
+        assert M.on_curve(curve, P)
+        bad = good[: pb * 10] + M.to_wire(curve, P) + good[pb * 10:]
+        assert not D.check_g1(D.PreparedPoints(curve, bad))
+
+
+@pytest.mark.parametrize("n", [2048, 5000, 1 << 16])
+@pytest.mark.parametrize("c", [0, 9, 12, 15, 16])
+def test_msm_g1_glv_matches_plain(ca, n, c):
+    """BLS12-381 through the GLV split (kinet_hip_msm_prepared_g1) gives the
+    plain path's bytes and the closed form, for scalars of every range
+    (0, r - 1, r, >= 2^255, 2^256 - 1, lambda) and identity points."""
+    from crypto_amd import device as D
+
+    if n == 1 << 16 and c not in (0, 16):
+        pytest.skip("large size covered at the default and widest window")
+    curve = 2
+    k0, d = 0x5A5A5A5A, 0x1F2E3D4C5B6A7
+    pb = O.POINT_BYTES[curve]
+    points = bytearray(O.gen_points_progression(curve, k0, d, n))
+    zero_at = set(range(3, n, 211))
+    for i in zero_at:
+        points[i * pb:(i + 1) * pb] = bytes(pb)
+    r = M.CURVES[curve]["r"]
+    lam = 0xAC45A4010001A40200000000FFFFFFFF
+    rnd = random.Random(n + c)
+    sc = [rnd.getrandbits(256) for _ in range(n)]
+    sc[:8] = [0, 1, r - 1, r, r + 1, (1 << 256) - 1, lam, lam * lam % r]
+    sc[8:20] = [(1 << 255) + rnd.getrandbits(200) for _ in range(12)]
+    scalars = b"".join(s.to_bytes(32, "little") for s in sc)
+    t = sum(s * (k0 + i * d) for i, s in enumerate(sc) if i not in zero_at) % r
+    want = O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+    pp = D.PreparedPoints(curve, bytes(points))
+    d_sc = D.to_device(scalars)
+    print(f"glv n={n} c={c}", flush=True)
+    got = D.msm_prepared(pp, d_sc, window_bits=c, g1=True).cpu().numpy().tobytes()
+    plain = D.msm_prepared(pp, d_sc, window_bits=(c if c else 0)).cpu().numpy().tobytes()
+    assert got[:pb] == want
+    assert got == plain
+
+
+def test_msm_g1_golden_bls(ca):
+    """The reference's golden BLS12-381 outputs through the GLV path."""
+    from crypto_amd import device as D
+
+    for case in MSM_CASES:
+        if case["curve"] != 2 or case["n"] < 2048 or "scalars" in case:
+            continue
+        scalars, points = _inputs(case)
+        pp = D.PreparedPoints(2, points)
+        assert D.check_g1(pp)
+        out = D.msm_prepared(pp, D.to_device(scalars), g1=True).cpu().numpy().tobytes()
+        assert out[:96].hex() == case["expected"], case["n"]
