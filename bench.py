@@ -50,13 +50,37 @@ def parse_args():
 
 
 # --------------------------------------------------------------------- work
-def window_stats(scalars, c: int, chunk: int):
+GLV_LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF  # BLS12-381 G1 endomorphism eigenvalue
+
+
+def glv_halves(scalars):
+    """(n, 4) LE limbs -> (2n, 4) limbs [s1; s2] with s = s1 + lambda s2 mod r,
+    the split the library's GLV path makes (for the work count below)."""
+    import numpy as np
+
+    from crypto_amd import inputs
+
+    r = inputs.ORDER[2]
+    lo, hi = [], []
+    for v in inputs.to_int(scalars):
+        q, rem = divmod(v % r, GLV_LAMBDA)
+        lo.append(rem)
+        hi.append(q)
+    m = (1 << 64) - 1
+    out = np.zeros((2 * len(lo), 4), dtype=np.uint64)
+    for i, v in enumerate(lo + hi):
+        out[i, 0] = v & m
+        out[i, 1] = v >> 64
+    return out
+
+
+def window_stats(scalars, c: int, chunk: int, bits: int = 256):
     """Exact entry / bucket / chunk counts of the signed-digit schedule (numpy),
     for the algorithmic work of the accumulation launch."""
     import numpy as np
 
-    W = (256 + c - 1) // c
-    if W * c == 256:
+    W = (bits + c - 1) // c
+    if W * c == bits:
         W += 1
     B = 1 << (c - 1)
     n = scalars.shape[0]
@@ -92,8 +116,11 @@ def gpu_fixed_base(lib, torch, curve, d_scalars, n, stream):
     return out
 
 
-def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile):
-    """Returns (ms_per_step, stage_ms dict, last output bytes, check bytes, extras)."""
+def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile, g1=True):
+    """Returns (ms_per_step, stage_ms dict, last output bytes, check bytes, extras).
+    g1: the points are checked once (outside the timed region) to lie in G1,
+    and the MSM runs through kinet_hip_msm_prepared_g1 (GLV split on
+    BLS12-381); else through the plain kinet_hip_msm_prepared."""
     from crypto_amd import inputs
 
     stream = torch.cuda.Stream()
@@ -107,10 +134,13 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         stream.synchronize()
         gen_s = time.time() - t0
         pp = D.PreparedPoints(curve, d_pts_wire)
+        in_g1 = D.check_g1(pp) if g1 else False
+        if g1 and not in_g1:
+            raise RuntimeError("generated points failed the G1 check")
         pb = 96 if curve == 2 else 64
         out = torch.empty(pb + pb // 2, dtype=torch.uint8, device="cuda")
         for _ in range(warmup):
-            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False)
+            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False, g1=in_g1)
         stream.synchronize()
         torch.cuda.synchronize()
         if profile:
@@ -119,7 +149,7 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(steps):
-            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False)
+            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False, g1=in_g1)
         e1.record(stream)
         stream.synchronize()
         torch.cuda.synchronize()
@@ -140,7 +170,8 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         tk = np.array([[(t >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]], dtype=np.uint64)
         d_t = torch.from_numpy(tk.view("u1").reshape(-1).copy()).to("cuda")
         check = gpu_fixed_base(lib, torch, curve, d_t, 1, stream).cpu().numpy().tobytes()
-    return ms, stage, result, check, {"points_gen_s": gen_s, "scalars": ss, "points_wire": d_pts_wire}
+    return ms, stage, result, check, {"points_gen_s": gen_s, "scalars": ss, "points_wire": d_pts_wire,
+                                      "path": "g1_glv" if (in_g1 and curve == 2) else "plain"}
 
 
 def cpu_baseline(curve, scalars, points_wire, log2n):
@@ -496,8 +527,10 @@ def main():
     verified = result[:pb] == check
     value = n / ms / 1e3  # Mpoints/s
 
-    c = args.window_bits or (min(16, max(5, args.log2n - 4)))
-    ws = window_stats(ext["scalars"], c, chunk)
+    glv = ext["path"] == "g1_glv"
+    n_eff = 2 * n if glv else n
+    c = args.window_bits or lib.kinet_hip_msm_window_bits(curve, n_eff, 0)
+    ws = window_stats(glv_halves(ext["scalars"]) if glv else ext["scalars"], c, chunk, 128 if glv else 256)
     nl = 12 if curve == 2 else 8          # 32-bit limbs of Fp: 2n^2 + n multiply-adds per product
     mads_per_mul = 2 * nl * nl + nl
     acc_mads = ws["madds"] * 10 * mads_per_mul       # XYZZ mixed addition = 8M + 2S
@@ -522,8 +555,10 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic: seeded uniform scalars in [0,r); points k_i*G, k_i seeded uniform, generated on the GPU",
-        "config": {"workload": f"{CURVE_NAMES[curve]} G1 MSM, N=2^{args.log2n}, plain Pippenger, one MSM per step",
+        "config": {"workload": f"{CURVE_NAMES[curve]} G1 MSM, N=2^{args.log2n}, Pippenger, one MSM per step",
                    "curve": CURVE_NAMES[curve], "n": n, "window_bits": c, "chunk": chunk,
+                   "path": ("G1 points (checked once), GLV split: 2N points x 128-bit scalars"
+                            if glv else "plain: N points x 256-bit scalars"),
                    "inputs": "resident in HBM; points prepared once"},
         "verified": verified,
         "roofline": {
@@ -551,6 +586,10 @@ def main():
         line["serving"] = serving_run(lib, torch, args.serve_seconds)
     if not args.no_extras:
         extras = {}
+        pms, _, pres, pcheck, _ = timed_msm(lib, torch, D, curve, n, 20261203, max(3, args.steps // 2), 2,
+                                            args.window_bits, False, g1=False)
+        extras[f"{CURVE_NAMES[curve]}_plain_path_mpoints_s"] = round(n / pms / 1e3, 3)
+        extras[f"{CURVE_NAMES[curve]}_plain_path_verified"] = pres[:pb] == pcheck
         other = 1 if curve == 2 else 2
         oms, _, ores, ocheck, _ = timed_msm(lib, torch, D, other, n, 777, max(3, args.steps // 2), 2, 0, False)
         extras[f"{CURVE_NAMES[other]}_mpoints_s"] = round(n / oms / 1e3, 3)

@@ -168,6 +168,16 @@ int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void*
 
 void kinet_hip_profile_enable(int on) { msm_set_profiling(on); }
 
+int kinet_hip_msm_set_reduce_segment(int seg) {
+    try {
+        msm_set_reduce_segment(seg);
+        return GPUKIT_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BAD_SIZE;
+    }
+}
+
 int kinet_hip_profile_collect(double* stage_ms, int* stage_calls, int nstages) {
     if (!stage_ms || !stage_calls) return GPUKIT_ERR_NULL_ARG;
     try {

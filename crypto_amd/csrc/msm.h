@@ -92,6 +92,7 @@ struct MsmBatchPlan {
     uint32_t nseg = 0;        // first-level reduction segments
     uint32_t dig_blocks = 0;  // blocks of the digit kernels
     uint32_t chunk = 32;
+    uint32_t seg = 16;        // buckets per first-level reduction segment
     uint32_t levels = 0;      // carry-fold tree levels
     size_t max_entries = 0;
     std::vector<MsmJob> jobs;  // host copy of the descriptors
@@ -122,6 +123,9 @@ void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_
 // Opt-in per-stage timing of msm_device (HIP events on the MSM's stream).
 enum MsmStage { kStageSort = 0, kStageAccumulate = 1, kStageReduce = 2, kStageCombine = 3, kStageCount = 4 };
 void msm_set_profiling(int on);
+// Buckets per first-level reduction segment (power of two <= 16; 0 = the
+// built-in rule) for MSMs and batch plans created afterwards.
+void msm_set_reduce_segment(int seg);
 int msm_collect_profile(double* sums_ms, int* counts, int nstages);
 
 // Compress an affine wire point on the device (one point).

@@ -56,6 +56,9 @@ void msm_set_window_table(int curve, bool batched, const int* c_by_log, int coun
 namespace {
 
 std::atomic<int> g_profiling{0};
+// First-level reduction segment (buckets per running-sum chain): the chain is
+// 3 seg point additions long, so short segments cut latency; 0 = built-in rule.
+std::atomic<int> g_seg_override{0};
 std::mutex g_prof_mu;
 struct StageEvents {
     int stage;
@@ -88,6 +91,12 @@ struct StageTimer {
 
 void msm_set_profiling(int on) { g_profiling.store(on); }
 
+void msm_set_reduce_segment(int seg) {
+    if (seg != 0 && (seg < 1 || seg > 16 || (seg & (seg - 1)) != 0))
+        throw std::invalid_argument("segment must be 0 or a power of two in [1, 16]");
+    g_seg_override.store(seg);
+}
+
 int msm_collect_profile(double* sums_ms, int* counts, int nstages) {
     std::lock_guard<std::mutex> g(g_prof_mu);
     for (int i = 0; i < nstages; ++i) {
@@ -112,7 +121,26 @@ int msm_collect_profile(double* sums_ms, int* counts, int nstages) {
 // ---------------------------------------------------------------- pipeline
 namespace {
 
-constexpr uint32_t kSegBuckets = 16;  // must match kSeg in msm_impl.h
+// Built-in rule: 16-bucket segments while that still leaves >= 2^14 segments
+// (4 lanes each, so >= 1024 waves over 256 CUs); below that, halve down to 4,
+// trading a short chain for more waves.  Measured on BLS12-381 / BN254 MSMs
+// from 2^12 to 2^20 points: seg 4 cuts the reduction by 15-45 % at n <= 2^18,
+// seg 16 is best at 2^20 (c = 16).  min_buckets_per_window >= 16 (c >= 5).
+uint32_t reduce_segment(uint64_t total_buckets, uint32_t min_buckets_per_window) {
+    uint32_t seg = (uint32_t)g_seg_override.load();
+    if (!seg) {
+        seg = 16;
+        while (seg > 4 && total_buckets / seg < (1u << 14)) seg >>= 1;
+    }
+    while (seg > min_buckets_per_window) seg >>= 1;
+    return seg;
+}
+
+uint32_t log2u(uint32_t v) {
+    uint32_t l = 0;
+    while ((1u << l) < v) ++l;
+    return l;
+}
 
 struct Stages {
     decltype(&launch_accumulate_bls) accumulate;
@@ -135,6 +163,7 @@ struct Shape {
     uint32_t nb;        // global buckets
     uint32_t nwin;      // global windows
     uint32_t t1;        // segments per window (single MSM), 0 = per-window tables
+    uint32_t seg;       // buckets per first-level segment
     uint32_t nseg;      // first-level segments
     uint32_t chunk;     // sorted entries per accumulation thread
     uint32_t levels;    // carry-fold tree levels
@@ -209,12 +238,12 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
         st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, ws.run_bucket.as<uint32_t>(),
                 ws.run_bucket.as<uint32_t>() + max_chunks + 1, ws.carry.as<uint32_t>());
         st.reduce_seg(ceil_div((size_t)sh.nseg * 4, 256), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(),
-                      ws.carry.as<uint32_t>(), sh.nseg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
-        // levels: segment pairs carry weight m = 16 (lm = 4); each level
-        // multiplies it by kReduceFanIn = 2^7
+                      ws.carry.as<uint32_t>(), sh.nseg, sh.seg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
+        // levels: segment pairs carry weight m = seg; each level multiplies it
+        // by kReduceFanIn = 2^7
         const uint32_t* inA = ws.segA.as<uint32_t>();
         const uint32_t* inB = ws.segB.as<uint32_t>();
-        uint32_t lm = 4, n_in = sh.t1;
+        uint32_t lm = log2u(sh.seg), n_in = sh.t1;
         const size_t nlev = plan ? plan->level_grid.size() : 0;
         for (size_t lv = 0;; ++lv) {
             const uint32_t parts = plan ? 0 : parts_of(n_in);
@@ -272,7 +301,8 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     Shape sh;
     sh.nb = (uint32_t)W * B;
     sh.nwin = (uint32_t)W;
-    sh.t1 = B / kSegBuckets;
+    sh.seg = reduce_segment(sh.nb, B);
+    sh.t1 = B / sh.seg;
     sh.nseg = (uint32_t)W * sh.t1;
     sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
     sh.levels = fold_levels(npts, sh.chunk);
@@ -311,6 +341,7 @@ Shape batch_shape(const MsmBatchPlan& p) {
     sh.nb = p.nb;
     sh.nwin = p.nwin;
     sh.t1 = 0;
+    sh.seg = p.seg;
     sh.nseg = p.nseg;
     sh.chunk = p.chunk;
     sh.levels = p.levels;
@@ -328,14 +359,22 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     plan.njobs = (uint32_t)njobs;
     plan.chunk = (uint32_t)(chunk > 0 ? chunk : 32);
     plan.jobs.assign(specs, specs + njobs);
+    uint64_t total_buckets = 0;
+    uint32_t min_b = 1u << 17;
+    for (auto& j : plan.jobs) {
+        const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, true);
+        if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
+        j.c = (uint32_t)c;
+        total_buckets += (uint64_t)msm_num_windows(c) << (c - 1);
+        if ((1u << (c - 1)) < min_b) min_b = 1u << (c - 1);
+    }
+    plan.seg = reduce_segment(total_buckets, min_b);
     std::vector<uint32_t> lv_base, lv_n;  // per window: first segment, segment count
     uint64_t nb = 0, nwin = 0, nseg = 0, blocks = 0;
     size_t entries = 0, max_n = 0;
     for (auto& j : plan.jobs) {
-        const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, true);
-        if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
-        const uint32_t W = (uint32_t)msm_num_windows(c), B = 1u << (c - 1), t1 = B / kSegBuckets;
-        j.c = (uint32_t)c;
+        const int c = (int)j.c;
+        const uint32_t W = (uint32_t)msm_num_windows(c), B = 1u << (c - 1), t1 = B / plan.seg;
         j.nwin = W;
         j.win_base = (uint32_t)nwin;
         j.gb_base = (uint32_t)nb;

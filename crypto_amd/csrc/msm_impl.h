@@ -182,10 +182,11 @@ k_fold_level(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
 // ---------------------------------------------------------- bucket reduce
 // Window sum W = sum_b (b+1) S_b over the window's bucket sums, in levels.
 //
-//   k_reduce_seg    per segment t of kSeg = 16 buckets (one quad each): adds
-//                   each bucket's folded carry run to its sum, then by running
-//                   sums from the top  A_t = sum_l S_{16t+l},
-//                   B_t = sum_l (l+1) S_{16t+l},  so  W = sum_t [B_t + 16 t A_t].
+//   k_reduce_seg    per segment t of `seg` buckets (a power of two dividing
+//                   the bucket count; one quad each): adds each bucket's folded
+//                   carry run to its sum, then by running sums from the top
+//                   A_t = sum_l S_{seg t+l},  B_t = sum_l (l+1) S_{seg t+l},
+//                   so  W = sum_t [B_t + seg t A_t].
 //   k_reduce_level  generic step.  A window's inputs (A_i, B_i), i < n, with
 //                   W = sum_i [B_i + m i A_i]  (m = 2^lm)  are cut into parts of
 //                   kLevelQuads = 128 consecutive inputs; part g yields
@@ -199,21 +200,20 @@ k_fold_level(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
 // A' = R_0.  The running sums cost 3 point additions per bucket; the levels,
 // which see 16x and more fewer inputs, trade extra additions for depth
 // log2(128) per scan/tree instead of a serial pass over a window.
-constexpr int kSeg = 16;
 constexpr int kLevelQuads = (int)kReduceFanIn;
 constexpr int kLevelThreads = 4 * kLevelQuads;
 
 template <class F>
 __global__ void __launch_bounds__(256)
 k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ bsum,
-             const uint32_t* __restrict__ carry, uint32_t nseg, uint32_t* __restrict__ segA,
+             const uint32_t* __restrict__ carry, uint32_t nseg, uint32_t seg, uint32_t* __restrict__ segA,
              uint32_t* __restrict__ segB) {
     const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (t >= nseg) return;  // whole quads leave together
     Xyzz<F> run = xyzz_inf<F>(), tot = xyzz_inf<F>();
 #pragma unroll 1
-    for (int l = kSeg - 1; l >= 0; --l) {
-        const uint32_t gb = t * kSeg + (uint32_t)l;
+    for (int l = (int)seg - 1; l >= 0; --l) {
+        const uint32_t gb = t * seg + (uint32_t)l;
         const uint32_t off = offsets[gb], end = offsets[gb + 1];
         Xyzz<F> S = xyzz_inf<F>();
         if (end > off) {

@@ -17,9 +17,9 @@ void launch_fold_bls(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32
 }
 
 void launch_reduce_seg_bls(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,
-                            const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA,
+                            const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t seg, uint32_t* segA,
                             uint32_t* segB) {
-    hipLaunchKernelGGL(k_reduce_seg<CurveBLS12381::Fp>, dim3(grid), dim3(256), 0, s, offsets, chunk, bsum, carry, nseg, segA,
+    hipLaunchKernelGGL(k_reduce_seg<CurveBLS12381::Fp>, dim3(grid), dim3(256), 0, s, offsets, chunk, bsum, carry, nseg, seg, segA,
                        segB);
 }
 

@@ -17,9 +17,9 @@ void launch_fold_bn(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_
 }
 
 void launch_reduce_seg_bn(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,
-                            const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA,
+                            const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t seg, uint32_t* segA,
                             uint32_t* segB) {
-    hipLaunchKernelGGL(k_reduce_seg<CurveBN254::Fp>, dim3(grid), dim3(256), 0, s, offsets, chunk, bsum, carry, nseg, segA,
+    hipLaunchKernelGGL(k_reduce_seg<CurveBN254::Fp>, dim3(grid), dim3(256), 0, s, offsets, chunk, bsum, carry, nseg, seg, segA,
                        segB);
 }
 

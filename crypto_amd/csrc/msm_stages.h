@@ -17,8 +17,8 @@ namespace kz {
     void launch_fold_##sfx(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk,             \
                            uint32_t nchunks, uint32_t levels, uint32_t* list, uint32_t* ctl, uint32_t* carry); \
     void launch_reduce_seg_##sfx(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,     \
-                                 const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t* segA, \
-                                 uint32_t* segB);                                                           \
+                                 const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t seg,  \
+                                 uint32_t* segA, uint32_t* segB);                                           \
     void launch_reduce_level_##sfx(unsigned grid, hipStream_t s, const uint32_t* inA, const uint32_t* inB,  \
                                    const RedLevel* tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts,  \
                                    uint32_t lm, uint32_t* outA, uint32_t* outB);                            \

@@ -92,6 +92,8 @@ def library():
         lib.kzg_commit_async.argtypes = [vp, vp, vp, sz, i32, vp, vp]
         lib.kinet_hip_profile_enable.argtypes = [i32]
         lib.kinet_hip_profile_enable.restype = None
+        lib.kinet_hip_msm_set_reduce_segment.argtypes = [i32]
+        lib.kinet_hip_msm_set_reduce_segment.restype = i32
         lib.kinet_hip_profile_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32), i32]
         lib.kinet_hip_ntt_fr_batch.argtypes = [u32, i32, vp, vp, vp, sz, sz, vp, vp]
         lib.kinet_hip_msm_batch_create.argtypes = [u32, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_uint64),

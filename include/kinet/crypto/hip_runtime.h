@@ -101,6 +101,13 @@ int kinet_hip_g1_fixed_base_mul(uint32_t curve, const void* d_scalars, size_t n,
 void kinet_hip_profile_enable(int on);
 int kinet_hip_profile_collect(double* stage_ms, int* stage_calls, int nstages);
 
+/* Tuning knob of the bucket reduction: buckets per first-level running-sum
+ * segment (a power of two in [1, 16]; 0 restores the built-in rule) for MSMs
+ * and batch plans created afterwards.  Results do not depend on it.  Returns
+ * GPUKIT_ERR_BAD_SIZE for other values.  No counterpart in the reference
+ * (its reduction is a CPU loop). */
+int kinet_hip_msm_set_reduce_segment(int seg);
+
 /* Measured VALU throughput of the current device, in instructions per second
  * summed over all lanes: op 0 v_mad_u64_u32 (the roofline denominator of the
  * MSM and NTT kernels), 1 v_mul_lo_u32 + add, 2 v_mul_hi_u32 + xor,
