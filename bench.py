@@ -37,7 +37,10 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--curve", choices=["bls12_381", "bn254"], default="bls12_381")
     ap.add_argument("--log2n", type=int, default=20)
-    ap.add_argument("--window-bits", type=int, default=0, help="0 = library default")
+    ap.add_argument("--window-bits", type=int, default=0, help="0 = the schedule table")
+    ap.add_argument("--tune-file", default=None,
+                    help="schedule table to install (default: crypto_amd/tune/mi355x.tune when present; "
+                         "'none' = built-in rules)")
     ap.add_argument("--cpu-log2n", type=int, default=18, help="CPU-baseline sample size (log2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the BN254 and one-shot extra figures")
@@ -502,6 +505,11 @@ def main():
     from crypto_amd import device as D
 
     lib = crypto_amd.lib.require_device()
+    import crypto_amd.tune as T
+
+    tune_file = args.tune_file or (T.DEFAULT_TABLE if os.path.exists(T.DEFAULT_TABLE) else "none")
+    if tune_file != "none":
+        T.load(tune_file)
     curve = 2 if args.curve == "bls12_381" else 1
     n = 1 << args.log2n
     chunk = 32
@@ -529,7 +537,8 @@ def main():
 
     glv = ext["path"] == "g1_glv"
     n_eff = 2 * n if glv else n
-    c = args.window_bits or lib.kinet_hip_msm_window_bits(curve, n_eff, 0)
+    c = args.window_bits or lib.kinet_hip_msm_window_bits(curve, n, 2 if glv else 0)
+    cell = T.msm_cell(curve, 2 if glv else 0, args.log2n)
     ws = window_stats(glv_halves(ext["scalars"]) if glv else ext["scalars"], c, chunk, 128 if glv else 256)
     nl = 12 if curve == 2 else 8          # 32-bit limbs of Fp: 2n^2 + n multiply-adds per product
     mads_per_mul = 2 * nl * nl + nl
@@ -557,6 +566,8 @@ def main():
         "data": "synthetic: seeded uniform scalars in [0,r); points k_i*G, k_i seeded uniform, generated on the GPU",
         "config": {"workload": f"{CURVE_NAMES[curve]} G1 MSM, N=2^{args.log2n}, Pippenger, one MSM per step",
                    "curve": CURVE_NAMES[curve], "n": n, "window_bits": c, "chunk": chunk,
+                   "schedule": (f"tuned table {os.path.relpath(tune_file, ROOT)}: c={cell[0]} segment={cell[1]}"
+                                if tune_file != "none" and cell[0] else "built-in rule"),
                    "path": ("G1 points (checked once), GLV split: 2N points x 128-bit scalars"
                             if glv else "plain: N points x 256-bit scalars"),
                    "inputs": "resident in HBM; points prepared once"},

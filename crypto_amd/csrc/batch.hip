@@ -122,10 +122,10 @@ int kinet_hip_msm_batch_run(kinet_msm_batch* b, const void* d_scalars, const voi
     }
 }
 
-int kinet_hip_msm_set_window_table(uint32_t curve, int batched, const int* c_by_log, int count) {
+int kinet_hip_msm_set_window_table(uint32_t curve, int mode, const int* c_by_log, int count) {
     if (count > 0 && !c_by_log) return GPUKIT_ERR_NULL_ARG;
     try {
-        msm_set_window_table((int)curve, batched != 0, c_by_log, count);
+        msm_set_window_table((int)curve, mode, c_by_log, count);
         return GPUKIT_OK;
     } catch (const std::exception& ex) {
         set_last_error(ex.what());
@@ -133,9 +133,9 @@ int kinet_hip_msm_set_window_table(uint32_t curve, int batched, const int* c_by_
     }
 }
 
-int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int batched) {
-    if (curve != 1 && curve != 2) return -1;
-    return msm_window_bits((int)curve, n, batched != 0);
+int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode) {
+    if (curve != 1 && curve != 2 || mode < 0 || mode >= kMsmModes) return -1;
+    return msm_window_bits((int)curve, n, mode);
 }
 
 }  // extern "C"

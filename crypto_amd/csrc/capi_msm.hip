@@ -32,6 +32,7 @@ Engine& engine() {
         Engine* eng = new Engine();
         KZ_HIP(hipGetDevice(&eng->device));
         KZ_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+        tune_load_env();  // tuning tables named by KINET_HIP_TUNE_FILE
         return eng;
     }();
     return *e;

@@ -25,4 +25,8 @@ struct Engine {
 // device is usable.
 Engine& engine();
 
+// Installs the tuning file named by KINET_HIP_TUNE_FILE, if set (throws on a
+// malformed file).  Called once by engine().
+void tune_load_env();
+
 }  // namespace kz

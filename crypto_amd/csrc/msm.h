@@ -22,6 +22,7 @@ struct MsmParams {
     int c = 0;         // window bits; 0 = choose from n
     int chunk = 32;    // sorted entries per accumulation thread
     bool g1 = false;   // caller guarantees every point lies in G1: BLS12-381 may use the GLV split
+    int seg = 0;       // first-level reduction segment; 0 = tuned cell (when c = 0) or rule
 };
 
 // Device descriptor of one MSM inside a batch.  A batch runs every job's
@@ -103,15 +104,24 @@ struct MsmBatchPlan {
     DevBuf d_jobs, d_levels;
 };
 
-// Window bits used for an n-point MSM: the installed tuning table when one is
-// set for (curve, batched), else msm_default_window_bits(n).
-int msm_window_bits(int curve, size_t n, bool batched);
-// Installs a tuning table: c_by_log[k] = window bits for 2^k <= n < 2^(k+1)
-// (0 = default rule), k < count <= 32.  count = 0 clears it.
-void msm_set_window_table(int curve, bool batched, const int* c_by_log, int count);
+// Tuning modes: one MSM, a job of a batch, one MSM over G1 points through
+// the GLV split (BLS12-381; 2n points of 128-bit scalars).
+enum MsmMode { kMsmPlain = 0, kMsmBatched = 1, kMsmGlv = 2, kMsmModes = 3 };
+// Window bits used for an n-point MSM in `mode`: the tuning table entry of
+// (curve, mode, floor(log2 n)) when set, else msm_default_window_bits(n)
+// (of 2n for GLV).
+int msm_window_bits(int curve, size_t n, int mode);
+// Tuned first-level reduction segment of that size class, 0 = built-in rule.
+int msm_tuned_segment(int curve, size_t n, int mode);
+// Installs window bits: c_by_log[k] for 2^k <= n < 2^(k+1) (0 = default
+// rule), k < count <= 32; count = 0 clears them.  Segments are kept.
+void msm_set_window_table(int curve, int mode, const int* c_by_log, int count);
+// One tuning table cell (lg = floor(log2 n)); c or seg 0 = built-in rule.
+void msm_set_tuned(int curve, int mode, int lg, int c, int seg);
+void msm_get_tuned(int curve, int mode, int lg, int& c, int& seg);
 
 // Builds a plan (synchronous upload of the descriptor tables).  Each spec
-// gives n, pt_base, sc_base and c (0 = msm_window_bits(curve, n, true)).
+// gives n, pt_base, sc_base and c (0 = msm_window_bits(curve, n, kMsmBatched)).
 void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, MsmBatchPlan& plan);
 // Grows ws for the plan; after this, msm_batch_run allocates nothing and
 // issues only kernel launches (so it can be captured in a HIP graph).

@@ -22,32 +22,73 @@ int msm_default_window_bits(size_t n) {
     return c;
 }
 
-// ------------------------------------------------------------ window tables
+// ------------------------------------------------------------ tuning tables
+// Per (curve, mode, floor(log2 n)): window bits and first-level reduction
+// segment, 0 = the built-in rule.  Filled by the autotuner or a cache file.
 namespace {
 
 std::mutex g_table_mu;
-int g_table[3][2][32];  // [curve][batched][floor(log2 n)], 0 = default rule
+struct TuneCell {
+    int c = 0, seg = 0;
+};
+TuneCell g_table[3][kMsmModes][32];
+
+int floor_log2(size_t n) {
+    int lg = 0;
+    while (((size_t)1 << (lg + 1)) <= n) ++lg;
+    return lg;
+}
+
+void check_mode(int curve, int mode) {
+    if (curve < 1 || curve > 2 || mode < 0 || mode >= kMsmModes) throw std::invalid_argument("bad curve or mode");
+}
 
 }  // namespace
 
-int msm_window_bits(int curve, size_t n, bool batched) {
-    int lg = 0;
-    while (((size_t)1 << (lg + 1)) <= n) ++lg;
-    if (curve >= 1 && curve <= 2 && lg < 32) {
+int msm_window_bits(int curve, size_t n, int mode) {
+    const int lg = floor_log2(n);
+    if (curve >= 1 && curve <= 2 && mode >= 0 && mode < kMsmModes && lg < 32) {
         std::lock_guard<std::mutex> g(g_table_mu);
-        const int c = g_table[curve][batched ? 1 : 0][lg];
+        const int c = g_table[curve][mode][lg].c;
         if (c) return c;
     }
-    return msm_default_window_bits(n);
+    // GLV runs 2n points
+    return msm_default_window_bits(mode == kMsmGlv ? 2 * n : n);
 }
 
-void msm_set_window_table(int curve, bool batched, const int* c_by_log, int count) {
-    if (curve < 1 || curve > 2 || count < 0 || count > 32) throw std::invalid_argument("bad window table");
+int msm_tuned_segment(int curve, size_t n, int mode) {
+    const int lg = floor_log2(n);
+    if (curve < 1 || curve > 2 || mode < 0 || mode >= kMsmModes || lg >= 32) return 0;
+    std::lock_guard<std::mutex> g(g_table_mu);
+    return g_table[curve][mode][lg].seg;
+}
+
+void msm_set_window_table(int curve, int mode, const int* c_by_log, int count) {
+    check_mode(curve, mode);
+    if (count < 0 || count > 32) throw std::invalid_argument("bad window table");
     for (int k = 0; k < count; ++k)
         if (c_by_log[k] != 0 && (c_by_log[k] < 5 || c_by_log[k] > 18))
             throw std::invalid_argument("window bits out of range [5, 18]");
     std::lock_guard<std::mutex> g(g_table_mu);
-    for (int k = 0; k < 32; ++k) g_table[curve][batched ? 1 : 0][k] = k < count ? c_by_log[k] : 0;
+    for (int k = 0; k < 32; ++k) g_table[curve][mode][k].c = k < count ? c_by_log[k] : 0;
+}
+
+void msm_set_tuned(int curve, int mode, int lg, int c, int seg) {
+    check_mode(curve, mode);
+    if (lg < 0 || lg >= 32) throw std::invalid_argument("size class out of range");
+    if (c != 0 && (c < 5 || c > 18)) throw std::invalid_argument("window bits out of range [5, 18]");
+    if (seg != 0 && (seg > 16 || (seg & (seg - 1)) != 0))
+        throw std::invalid_argument("segment must be 0 or a power of two in [1, 16]");
+    std::lock_guard<std::mutex> g(g_table_mu);
+    g_table[curve][mode][lg] = TuneCell{c, seg};
+}
+
+void msm_get_tuned(int curve, int mode, int lg, int& c, int& seg) {
+    check_mode(curve, mode);
+    if (lg < 0 || lg >= 32) throw std::invalid_argument("size class out of range");
+    std::lock_guard<std::mutex> g(g_table_mu);
+    c = g_table[curve][mode][lg].c;
+    seg = g_table[curve][mode][lg].seg;
 }
 
 // ---------------------------------------------------------------- profiler
@@ -126,8 +167,11 @@ namespace {
 // trading a short chain for more waves.  Measured on BLS12-381 / BN254 MSMs
 // from 2^12 to 2^20 points: seg 4 cuts the reduction by 15-45 % at n <= 2^18,
 // seg 16 is best at 2^20 (c = 16).  min_buckets_per_window >= 16 (c >= 5).
-uint32_t reduce_segment(uint64_t total_buckets, uint32_t min_buckets_per_window) {
+// A tuned segment (nonzero `tuned`) wins over the rule; the global override
+// wins over both.
+uint32_t reduce_segment(uint64_t total_buckets, uint32_t min_buckets_per_window, int tuned = 0) {
     uint32_t seg = (uint32_t)g_seg_override.load();
+    if (!seg) seg = (uint32_t)tuned;
     if (!seg) {
         seg = 16;
         while (seg > 4 && total_buckets / seg < (1u << 14)) seg >>= 1;
@@ -279,7 +323,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     size_t npts = n;
     int c = prm.c;
     if (prm.g1 && curve == 2 && n <= (1u << 29)) {
-        const int cg = c ? c : msm_window_bits(curve, 2 * n, false);
+        const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
             bits = 128;
             npts = 2 * n;
@@ -294,14 +338,16 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
             d_scalars = sc2;
         }
     }
-    if (!c) c = msm_window_bits(curve, n, false);
+    if (!c) c = msm_window_bits(curve, n, kMsmPlain);
     if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
+    // the tuned segment belongs to the tuned window width
+    const int tuned_seg = prm.seg ? prm.seg : prm.c ? 0 : msm_tuned_segment(curve, n, bits == 128 ? kMsmGlv : kMsmPlain);
     const int W = msm_num_windows(c, bits);
     const uint32_t B = 1u << (c - 1);
     Shape sh;
     sh.nb = (uint32_t)W * B;
     sh.nwin = (uint32_t)W;
-    sh.seg = reduce_segment(sh.nb, B);
+    sh.seg = reduce_segment(sh.nb, B, tuned_seg);
     sh.t1 = B / sh.seg;
     sh.nseg = (uint32_t)W * sh.t1;
     sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
@@ -362,7 +408,7 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     uint64_t total_buckets = 0;
     uint32_t min_b = 1u << 17;
     for (auto& j : plan.jobs) {
-        const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, true);
+        const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, kMsmBatched);
         if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
         j.c = (uint32_t)c;
         total_buckets += (uint64_t)msm_num_windows(c) << (c - 1);

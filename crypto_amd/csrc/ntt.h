@@ -27,4 +27,12 @@ void ntt_device_batch(int field, int logn, bool inverse, const uint32_t* d_in, u
 // synchronises), so later transforms only launch kernels.
 void ntt_prepare(int field, int logn, bool inverse, hipStream_t s);
 
+// Pass split of a size class: log2 of the sub-transform lengths of passes
+// 1..3 (each <= 10, summing to logn, unused passes 0 at the end).  The tuned
+// cell (all zero = built-in split) and the split in effect.
+bool ntt_split_valid(int logn, int p1, int p2, int p3);
+void ntt_set_schedule(int field, int logn, int p1, int p2, int p3);
+void ntt_get_schedule(int field, int logn, int& p1, int& p2, int& p3);
+void ntt_schedule_of(int field, int logn, int& p1, int& p2, int& p3);
+
 }  // namespace kz

@@ -226,11 +226,27 @@ const NttTables& tables(int field, int logn, bool inverse, hipStream_t s) {
     return ref;
 }
 
-void plan(int logn, bool inverse, NttPass* passes, int& np) {
+// Tuned pass splits per (field, log2 n): log2 of the sub-transform lengths of
+// passes 1..3 (unused passes 0); all zero = the built-in split.
+std::mutex g_sched_mu;
+int g_sched[2][kNttMaxLog + 1][3];
+
+void builtin_split(int logn, int (&p)[3]) {
+    p[0] = p[1] = p[2] = 0;
+    if (logn <= 10) p[0] = logn;
+    else if (logn <= 20) { p[0] = (logn + 1) / 2; p[1] = logn - p[0]; }
+    else { p[0] = (logn + 2) / 3; p[1] = (logn - p[0] + 1) / 2; p[2] = logn - p[0] - p[1]; }
+}
+
+void plan(int field, int logn, bool inverse, NttPass* passes, int& np) {
+    int sp[3];
+    ntt_schedule_of(field, logn, sp[0], sp[1], sp[2]);
+    // la, lb, lc: lengths of the column pass, the inner column pass and the
+    // row pass (la = lb = 0 for one pass, lb = 0 for two)
     int la, lb, lc;
-    if (logn <= 10) { la = 0; lb = 0; lc = logn; np = 1; }
-    else if (logn <= 20) { la = (logn + 1) / 2; lb = 0; lc = logn - la; np = 2; }
-    else { la = (logn + 2) / 3; lb = (logn - la + 1) / 2; lc = logn - la - lb; np = 3; }
+    if (sp[2]) { la = sp[0]; lb = sp[1]; lc = sp[2]; np = 3; }
+    else if (sp[1]) { la = sp[0]; lb = 0; lc = sp[1]; np = 2; }
+    else { la = 0; lb = 0; lc = sp[0]; np = 1; }
     const uint32_t A = 1u << la, B = 1u << lb, C = 1u << lc, M = B * C;
     // sub-DFTs per workgroup: fill the 1024-element tile, but never more than
     // the n / L sub-DFTs the pass has
@@ -267,6 +283,43 @@ void plan(int logn, bool inverse, NttPass* passes, int& np) {
 
 }  // namespace
 
+bool ntt_split_valid(int logn, int p1, int p2, int p3) {
+    if (logn < 0 || logn > kNttMaxLog) return false;
+    if (p1 < 0 || p2 < 0 || p3 < 0 || p1 > 10 || p2 > 10 || p3 > 10) return false;
+    if (p1 + p2 + p3 != logn) return false;
+    if (p3 && (!p1 || !p2)) return false;  // zeros only trailing
+    if (p2 && !p1) return false;
+    return true;
+}
+
+void ntt_set_schedule(int field, int logn, int p1, int p2, int p3) {
+    if (field < 0 || field > 1 || logn < 0 || logn > kNttMaxLog) throw std::invalid_argument("NTT size class");
+    if (!(p1 == 0 && p2 == 0 && p3 == 0) && !ntt_split_valid(logn, p1, p2, p3))
+        throw std::invalid_argument("NTT pass split must sum to log2 n with passes of at most 2^10");
+    std::lock_guard<std::mutex> g(g_sched_mu);
+    g_sched[field][logn][0] = p1;
+    g_sched[field][logn][1] = p2;
+    g_sched[field][logn][2] = p3;
+}
+
+void ntt_get_schedule(int field, int logn, int& p1, int& p2, int& p3) {
+    if (field < 0 || field > 1 || logn < 0 || logn > kNttMaxLog) throw std::invalid_argument("NTT size class");
+    std::lock_guard<std::mutex> g(g_sched_mu);
+    p1 = g_sched[field][logn][0];
+    p2 = g_sched[field][logn][1];
+    p3 = g_sched[field][logn][2];
+}
+
+void ntt_schedule_of(int field, int logn, int& p1, int& p2, int& p3) {
+    ntt_get_schedule(field, logn, p1, p2, p3);
+    if (p1 | p2 | p3) return;
+    int b[3];
+    builtin_split(logn, b);
+    p1 = b[0];
+    p2 = b[1];
+    p3 = b[2];
+}
+
 void ntt_prepare(int field, int logn, bool inverse, hipStream_t s) {
     if (field < 0 || field > 1 || logn < 0 || logn > kNttMaxLog || logn > kTwoAdicity[field])
         throw std::invalid_argument("NTT size out of range");
@@ -282,7 +335,7 @@ void ntt_device_batch(int field, int logn, bool inverse, const uint32_t* d_in, u
     const NttTables& t = tables(field, logn, inverse, s);
     NttPass passes[3];
     int np = 0;
-    plan(logn, inverse, passes, np);
+    plan(field, logn, inverse, passes, np);
     const uint32_t n = 1u << logn;
     for (int i = 0; i < np; ++i) {
         const uint32_t* src = (i == 0) ? d_in : d_tmp;

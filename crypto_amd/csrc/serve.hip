@@ -53,7 +53,7 @@ uint64_t batch_footprint(int curve, const std::vector<uint32_t>& n) {
     const uint64_t rec = (uint64_t)xyzz_words(curve) * 4;
     uint64_t bytes = 0;
     for (uint32_t v : n) {
-        const int c = msm_window_bits(curve, v, true);
+        const int c = msm_window_bits(curve, v, kMsmBatched);
         const uint64_t W = (uint64_t)msm_num_windows(c), B = 1ull << (c - 1);
         bytes += 3ull * 32 * v;                     // evals, coeffs, scratch
         bytes += (uint64_t)v * W * (4 + rec / 32);  // entries + carries

@@ -104,6 +104,15 @@ def library():
         lib.kinet_hip_msm_batch_run.argtypes = [vp, vp, vp, vp, vp, i32, vp]
         lib.kinet_hip_msm_set_window_table.argtypes = [u32, i32, ctypes.POINTER(i32), i32]
         lib.kinet_hip_msm_window_bits.argtypes = [u32, sz, i32]
+        ip = ctypes.POINTER(i32)
+        lib.kinet_hip_tune_msm.argtypes = [u32, i32, i32, i32]
+        lib.kinet_hip_tune_ntt.argtypes = [u32, i32, i32]
+        lib.kinet_hip_tune_save.argtypes = [ctypes.c_char_p]
+        lib.kinet_hip_tune_load.argtypes = [ctypes.c_char_p]
+        lib.kinet_hip_tune_clear.argtypes = []
+        lib.kinet_hip_tune_clear.restype = None
+        lib.kinet_hip_tune_get_msm.argtypes = [u32, i32, i32, ip, ip]
+        lib.kinet_hip_tune_get_ntt.argtypes = [u32, i32, ip, ip, ip]
         lib.kzg_server_create.argtypes = [vp, vp, vp, pp]
         lib.kzg_server_destroy.argtypes = [vp]
         lib.kzg_server_destroy.restype = None

@@ -79,12 +79,49 @@ int kinet_hip_msm_batch_window_bits(const kinet_msm_batch* batch, size_t job);
 int kinet_hip_msm_batch_run(kinet_msm_batch* batch, const void* d_scalars, const void* d_pts, const void* d_inf,
                             void* d_out, int use_graph, void* stream);
 
-/* Window-size tuning table per curve and mode (batched = 0: single MSMs,
- * 1: batches): c_by_log[k] = window bits for 2^k <= n < 2^(k+1), 0 = default
- * rule, count <= 32 (0 clears).  The library default is c = floor(log2 n) - 4
- * clamped to [5, 16]. */
-int kinet_hip_msm_set_window_table(uint32_t curve, int batched, const int* c_by_log, int count);
-int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int batched);
+/* ------------------------------------------------------------- tuning
+ * Schedules are chosen per size class k = floor(log2 n) from in-process
+ * tables; an empty cell means the built-in rule.  MSM cells are per curve
+ * and mode (0: single MSM, 1: a job of a batch, 2: a single MSM over G1
+ * points through the GLV split, kinet_hip_msm_prepared_g1) and hold the
+ * window bits c and the first-level reduction segment; NTT cells are per
+ * field and hold the pass split (log2 of the sub-transform lengths of passes
+ * 1..3).  Schedules never change results, only speed.
+ *
+ * Window bits only: c_by_log[k] for 2^k <= n < 2^(k+1), 0 = rule, count <= 32
+ * (0 clears).  The rule is c = floor(log2 n) - 4 clamped to [5, 16] (of 2n
+ * for mode 2). */
+int kinet_hip_msm_set_window_table(uint32_t curve, int mode, const int* c_by_log, int count);
+/* Window bits an n-point MSM of `mode` uses now (-1 for a bad curve / mode). */
+int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode);
+
+/* Autotuner: times the candidate schedules of every size class 2^log_min ..
+ * 2^log_max on the current device with synthetic inputs (multiples of the
+ * generator, seeded scalars / field elements), checks that every candidate's
+ * output equals the built-in schedule's byte for byte (GPUKIT_ERR_DETERMINISM
+ * otherwise), and installs the fastest.  Candidates: MSM window bits around
+ * log2 n - 4, then reduction segments 16/8/4/2 for the best c (modes 0, 2;
+ * batches of equal-size jobs totalling >= 2^20 points for mode 1); NTT every
+ * split of log2 n into one to three passes of at most 2^10.  Synchronous;
+ * holds the library's device lock, so other calls wait.  Mode 2 needs
+ * log_min >= 11 (smaller GLV MSMs run the plain path).  Sizes are limited to
+ * log2 n <= 24. */
+int kinet_hip_tune_msm(uint32_t curve, int mode, int log_min, int log_max);
+int kinet_hip_tune_ntt(uint32_t field, int log_min, int log_max);
+
+/* The tuning tables as text, one cell per line ("msm <curve> <mode> <k> <c>
+ * <seg>", "ntt <field> <k> <p1> <p2> <p3>", '#' comments): save writes every
+ * set cell; load installs the cells of a file (validated; nothing is
+ * installed if any line is malformed).  The engine loads the file named by
+ * the environment variable KINET_HIP_TUNE_FILE, if set, on first use. */
+int kinet_hip_tune_save(const char* path);
+int kinet_hip_tune_load(const char* path);
+/* Drops every tuned cell (back to the built-in rules). */
+void kinet_hip_tune_clear(void);
+/* Current schedule of a size class: MSM c and segment (0 = rule), NTT pass
+ * split (p1, p2, p3; unused passes 0).  GPUKIT_ERR_BAD_SIZE for bad ids. */
+int kinet_hip_tune_get_msm(uint32_t curve, int mode, int log_n, int* c, int* seg);
+int kinet_hip_tune_get_ntt(uint32_t field, int log_n, int* p1, int* p2, int* p3);
 
 /* Fixed-base batch scalar multiplication: out[i] = k_i * G for n scalars
  * (32-byte LE, any 256-bit value) on the device, written in the point wire
