@@ -39,7 +39,7 @@ def parse_args():
     ap.add_argument("--log2n", type=int, default=20)
     ap.add_argument("--window-bits", type=int, default=0, help="0 = the schedule table")
     ap.add_argument("--tune-file", default=None,
-                    help="schedule table to install (default: crypto_amd/tune/mi355x.tune when present; "
+                    help="schedule table to install (default: crypto_amd/tables/mi355x.tune when present; "
                          "'none' = built-in rules)")
     ap.add_argument("--cpu-log2n", type=int, default=18, help="CPU-baseline sample size (log2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

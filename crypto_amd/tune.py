@@ -22,7 +22,7 @@ from .lib import KinetError, library, require_device
 _STATUS = {-1: "null argument", -2: "bad size", -3: "backend", -6: "schedules disagree"}
 
 MODE_SINGLE, MODE_BATCHED, MODE_GLV = 0, 1, 2
-DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune", "mi355x.tune")
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tables", "mi355x.tune")
 
 
 def _check(rc: int, what: str) -> None:
