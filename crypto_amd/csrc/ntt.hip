@@ -8,8 +8,8 @@
 //
 // Schedule: n = A * B * C (each <= 2^10), one to three passes of the same
 // kernel.  A pass runs independent sub-DFTs of length L <= 1024 held in LDS
-// (radix-2 DIT: bit-reversed load, log2 L butterfly stages, twiddles w_L^k
-// staged in LDS), optionally multiplies its outputs by the four-step twiddle
+// (DIT: bit-reversed load, log2 L radix-2 stages fused in pairs into radix-4
+// register passes, twiddles w_L^k staged in LDS), optionally multiplies its outputs by the four-step twiddle
 // w_n^(e*k), and writes them back through a strided address map:
 //   pass 1  columns of length A, stride B*C, twiddle w_n^(m k)      (in place)
 //   pass 2  columns of length B inside each row, twiddle w_(BC)^(c k) (in place)
@@ -95,17 +95,45 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
     if (bad) atomicOr(err, 1);
     __syncthreads();
 
-    for (uint32_t s = 0; s < d.logL; ++s) {
-        const uint32_t half = 1u << s;
+    // Butterflies: radix-2 DIT stages s = 0 .. logL-1 (stage s pairs elements
+    // h = 2^s apart with twiddle w_(2h)^i = shtw[i << (logL-1-s)]), fused two
+    // at a time: a group of 4h elements holds, for each i < h, the four
+    // elements e, e+h, e+2h, e+3h that stages s and s+1 combine among
+    // themselves, so they stay in registers across both stages and LDS is
+    // read and written once per two stages.  An odd logL starts with the
+    // twiddle-free stage 0 alone.  Groups never straddle sub-DFTs (4h | L).
+    uint32_t s = 0;
+    if (d.logL & 1) {
         for (uint32_t p = tid; p < tile / 2; p += kNttThreads) {
-            const uint32_t b = p >> (d.logL - 1), pp = p & (half_L - 1);
-            const uint32_t i = pp & (half - 1), g = pp >> s;
-            const uint32_t i0 = b * L + g * 2 * half + i, i1 = i0 + half;
-            const Fe<F> u = fe_load_words9<F>(sh + i0 * F::N);
-            Fe<F> v = fe_load_words9<F>(sh + i1 * F::N);
-            if (s != 0) v = fe_mul<F>(v, fe_load_words9<F>(shtw + (i << (d.logL - 1 - s)) * F::N));
-            fe_store_words9<F>(sh + i0 * F::N, fe_add<F>(u, v));
-            fe_store_words9<F>(sh + i1 * F::N, fe_sub<F>(u, v));
+            const Fe<F> u = fe_load_words9<F>(sh + (2 * p) * F::N);
+            const Fe<F> v = fe_load_words9<F>(sh + (2 * p + 1) * F::N);
+            fe_store_words9<F>(sh + (2 * p) * F::N, fe_add<F>(u, v));
+            fe_store_words9<F>(sh + (2 * p + 1) * F::N, fe_sub<F>(u, v));
+        }
+        __syncthreads();
+        s = 1;
+    }
+    for (; s < d.logL; s += 2) {
+        const uint32_t h = 1u << s, sh1 = d.logL - 1 - s, sh2 = d.logL - 2 - s;
+        for (uint32_t q = tid; q < tile / 4; q += kNttThreads) {
+            const uint32_t i = q & (h - 1), e = ((q >> s) << (s + 2)) + i;
+            Fe<F> x0 = fe_load_words9<F>(sh + e * F::N);
+            Fe<F> x1 = fe_load_words9<F>(sh + (e + h) * F::N);
+            Fe<F> x2 = fe_load_words9<F>(sh + (e + 2 * h) * F::N);
+            Fe<F> x3 = fe_load_words9<F>(sh + (e + 3 * h) * F::N);
+            if (s != 0) {  // stage s: w_(2h)^i (= 1 at s = 0)
+                const Fe<F> w = fe_load_words9<F>(shtw + (i << sh1) * F::N);
+                x1 = fe_mul<F>(x1, w);
+                x3 = fe_mul<F>(x3, w);
+            }
+            const Fe<F> a0 = fe_add<F>(x0, x1), a1 = fe_sub<F>(x0, x1);
+            // stage s+1: pairs (e, e+2h) with w_(4h)^i, (e+h, e+3h) with w_(4h)^(i+h)
+            const Fe<F> a2 = fe_mul<F>(fe_add<F>(x2, x3), fe_load_words9<F>(shtw + (i << sh2) * F::N));
+            const Fe<F> a3 = fe_mul<F>(fe_sub<F>(x2, x3), fe_load_words9<F>(shtw + ((i + h) << sh2) * F::N));
+            fe_store_words9<F>(sh + e * F::N, fe_add<F>(a0, a2));
+            fe_store_words9<F>(sh + (e + 2 * h) * F::N, fe_sub<F>(a0, a2));
+            fe_store_words9<F>(sh + (e + h) * F::N, fe_add<F>(a1, a3));
+            fe_store_words9<F>(sh + (e + 3 * h) * F::N, fe_sub<F>(a1, a3));
         }
         __syncthreads();
     }
