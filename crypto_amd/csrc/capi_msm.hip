@@ -167,6 +167,64 @@ int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void*
     return msm_prepared_impl(curve, d_scalars, d_pts, d_inf, n, window_bits, chunk, d_out, stream, true);
 }
 
+int kinet_hip_table_window_bits(size_t n) { return msm_table_window_bits(n); }
+
+size_t kinet_hip_table_bytes(uint32_t curve, size_t n, int window_bits) {
+    if (!valid_curve(curve) || window_bits < 5 || window_bits > 18) return 0;
+    return (size_t)msm_table_rows(window_bits) * n * affine_words((int)curve) * sizeof(uint32_t);
+}
+
+int kinet_hip_points_table_build(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int window_bits,
+                                 void* d_table, void* stream) {
+    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (n > 0 && (!d_pts || !d_table)) return GPUKIT_ERR_NULL_ARG;
+    if (window_bits < 5 || window_bits > 18) return GPUKIT_ERR_BAD_SIZE;
+    try {
+        (void)engine();
+        points_table_build((int)curve, static_cast<const uint32_t*>(d_pts), static_cast<const uint8_t*>(d_inf), n,
+                           window_bits, static_cast<uint32_t*>(d_table), static_cast<hipStream_t>(stream));
+        return GPUKIT_OK;
+    } catch (const std::invalid_argument& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BAD_SIZE;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
+int kinet_hip_msm_table(uint32_t curve, const void* d_scalars, const void* d_table, const void* d_inf,
+                        size_t table_n, size_t n, int window_bits, void* d_out, void* stream) {
+    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (!d_out || (n > 0 && (!d_scalars || !d_table))) return GPUKIT_ERR_NULL_ARG;
+    if (window_bits < 5 || window_bits > 18 || n > table_n || table_n >= ((size_t)1 << 31))
+        return GPUKIT_ERR_BAD_SIZE;
+    if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (n == 0) {
+            KZ_HIP(hipMemsetAsync(d_out, 0, msm_out_bytes((int)curve), s));
+            uint8_t* d_zero = static_cast<uint8_t*>(d_out);
+            compress_on_device((int)curve, d_zero, d_zero + point_wire_bytes((int)curve), s);
+            return GPUKIT_OK;
+        }
+        MsmParams prm;
+        prm.c = window_bits;
+        prm.table_stride = (uint32_t)table_n;
+        msm_device((int)curve, static_cast<const uint32_t*>(d_table), static_cast<const uint8_t*>(d_inf),
+                   static_cast<const uint32_t*>(d_scalars), n, prm, e.ws, s, static_cast<uint8_t*>(d_out));
+        return GPUKIT_OK;
+    } catch (const std::invalid_argument& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BAD_SIZE;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return GPUKIT_ERR_BACKEND;
+    }
+}
+
 void kinet_hip_profile_enable(int on) { msm_set_profiling(on); }
 
 int kinet_hip_msm_set_reduce_segment(int seg) {

@@ -23,6 +23,11 @@ struct MsmParams {
     int chunk = 32;    // sorted entries per accumulation thread
     bool g1 = false;   // caller guarantees every point lies in G1: BLS12-381 may use the GLV split
     int seg = 0;       // first-level reduction segment; 0 = tuned cell (when c = 0) or rule
+    // > 0: d_pts is a window table built with window bits c for table_stride
+    // points (row w = 2^(c w) P_i at w * table_stride + i, n <= table_stride);
+    // every window's digits then share one bucket set, and no window combine
+    // (no doubling chain) is left.  d_inf stays the flags of the n points.
+    uint32_t table_stride = 0;
 };
 
 // Device descriptor of one MSM inside a batch.  A batch runs every job's
@@ -42,6 +47,9 @@ struct MsmJob {
 
 // Inputs per workgroup (one part) of a bucket-reduction level.
 constexpr uint32_t kReduceFanIn = 128;
+// Carry runs of at most this many chunks are folded serially by one thread;
+// longer ones go through the pairwise tree.
+constexpr uint32_t kFoldShortRun = 32;
 
 // One window's slice of a bucket-reduction level: inputs [in_base, in_base +
 // n_in), outputs [out_base, out_base + ceil(n_in / kReduceFanIn)), served by
@@ -137,6 +145,17 @@ void msm_set_profiling(int on);
 // built-in rule) for MSMs and batch plans created afterwards.
 void msm_set_reduce_segment(int seg);
 int msm_collect_profile(double* sums_ms, int* counts, int nstages);
+
+// Window tables (MsmParams::table_stride): rows = windows of c-bit signed
+// digits over 256-bit scalars; the table holds rows * n affine records.
+inline int msm_table_rows(int c) { return msm_num_windows(c, 256); }
+// Window bits for a table of n points: the widest c <= 16 that still gives
+// about 32 digits per bucket (rows * n / 2^(c-1) >= 32), at least 5.
+int msm_table_window_bits(size_t n);
+// Builds the table of n prepared points (d_table: msm_table_rows(c) * n *
+// affine_words(curve) words).  Asynchronous on s.
+void points_table_build(int curve, const uint32_t* d_pts, const uint8_t* d_inf, size_t n, int c, uint32_t* d_table,
+                        hipStream_t s);
 
 // Compress an affine wire point on the device (one point).
 void compress_on_device(int curve, const uint8_t* d_wire, uint8_t* d_out, hipStream_t s);

@@ -13,6 +13,11 @@
 // second pass scatters (point index | sign << 31) to its bucket's slot.  The
 // order inside a bucket is whatever the atomics give; bucket sums do not
 // depend on it (the group is commutative), so results stay bit-exact.
+//
+// Window-table MSMs (table_stride = T > 0): the points are a table whose row w
+// holds 2^(c w) P_i at index w T + i, so digit d of window w is the point
+// w T + i in bucket |d| - 1 of a single window shared by all rows: keys are
+// |d| - 1 and entries (w T + i | sign << 31).
 #include "msm_internal.h"
 #include "scan.h"
 
@@ -62,7 +67,7 @@ __device__ __forceinline__ void load_scalar_bits(const uint32_t* __restrict__ sc
 
 template <int CB, bool SCATTER>
 __global__ void __launch_bounds__(256)
-k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n,
+k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n, uint32_t table_stride,
          uint32_t* __restrict__ counts_or_cursor, uint32_t* __restrict__ entries) {
     using D = Digits<CB>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,27 +85,27 @@ k_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, 
             carry = 1;
         }
         if (d != 0) {
-            const uint32_t gb = (uint32_t)w * D::B + (uint32_t)(d > 0 ? d : -d) - 1u;
+            const uint32_t gb = (table_stride ? 0u : (uint32_t)w * D::B) + (uint32_t)(d > 0 ? d : -d) - 1u;
             if (!SCATTER) {
                 atomicAdd(&counts_or_cursor[gb], 1u);
             } else {
                 const uint32_t pos = atomicAdd(&counts_or_cursor[gb], 1u);
-                entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
+                entries[pos] = ((uint32_t)w * table_stride + i) | (d < 0 ? 0x80000000u : 0u);
             }
         }
     }
 }
 
 template <int CB>
-static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* counts,
-                   uint32_t* entries, hipStream_t s) {
+static void launch(bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t table_stride,
+                   uint32_t* counts, uint32_t* entries, hipStream_t s) {
     const unsigned grid = ceil_div(n, 256);
     if (!scatter)
-        hipLaunchKernelGGL((k_digits<CB, false>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, counts,
-                           nullptr);
+        hipLaunchKernelGGL((k_digits<CB, false>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, table_stride,
+                           counts, nullptr);
     else
-        hipLaunchKernelGGL((k_digits<CB, true>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, counts,
-                           entries);
+        hipLaunchKernelGGL((k_digits<CB, true>), dim3(grid), dim3(256), 0, s, d_scalars, d_inf, n, table_stride,
+                           counts, entries);
 }
 
 // ------------------------------------------------------------- LDS sort
@@ -185,14 +190,17 @@ k_sort_colscan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ pre, uin
 template <int CB, int BITS>
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, uint32_t n,
-               uint32_t per_part, uint32_t wpad, uint32_t nparts, const uint32_t* __restrict__ offsets,
-               const uint32_t* __restrict__ pre, uint32_t* __restrict__ entries) {
+               uint32_t per_part, uint32_t wpad, uint32_t nparts, uint32_t table_stride,
+               const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pre,
+               uint32_t* __restrict__ entries) {
     using D = Digits<CB, BITS>;
     __shared__ uint32_t cur[D::B];
     const uint32_t w = blockIdx.x % wpad, p = blockIdx.x / wpad;
     if (w >= (uint32_t)D::W) return;
     const uint32_t* pp = pre + ((size_t)w * nparts + p) * D::B;
-    for (uint32_t b = threadIdx.x; b < D::B; b += kSortThreads) cur[b] = offsets[w * D::B + b] + pp[b];
+    const uint32_t* wo = offsets + (table_stride ? 0u : w * D::B);
+    const uint32_t base = w * table_stride;
+    for (uint32_t b = threadIdx.x; b < D::B; b += kSortThreads) cur[b] = wo[b] + pp[b];
     __syncthreads();
     const uint32_t lo = p * per_part, hi = min(n, lo + per_part);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
@@ -202,7 +210,7 @@ k_sort_scatter(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
         const int32_t d = digit_of_window<CB, BITS>(s, w);
         if (d != 0) {
             const uint32_t pos = atomicAdd(&cur[(d > 0 ? d : -d) - 1], 1u);
-            entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
+            entries[pos] = (base + i) | (d < 0 ? 0x80000000u : 0u);
         }
     }
 }
@@ -220,34 +228,39 @@ void lds_sort_shape(int c, int bits, size_t n, uint32_t& wpad, uint32_t& nparts,
 }
 
 template <int CB, int BITS>
-static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
-                            uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
-                            uint32_t* entries, hipStream_t s) {
+static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t table_stride,
+                            uint32_t* cnt, uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums,
+                            uint32_t* total, uint32_t* entries, hipStream_t s) {
     using D = Digits<CB, BITS>;
     uint32_t wpad, nparts, per_part;
     lds_sort_shape(CB, BITS, n, wpad, nparts, per_part);
     const unsigned grid = wpad * nparts;
     hipLaunchKernelGGL((k_sort_count<CB, BITS>), dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
                        nparts, cnt);
-    const uint32_t nb = (uint32_t)D::W * D::B;
-    hipLaunchKernelGGL(k_sort_colscan, dim3(ceil_div(nb, 256)), dim3(256), 0, s, cnt, pre, tot, (uint32_t)D::W,
-                       nparts, D::B);
+    // per-window buckets: prefix over the parts of each window; one shared
+    // bucket set (window table): the (window, part) rows are the parts
+    const uint32_t nwin = table_stride ? 1u : (uint32_t)D::W;
+    const uint32_t rows = table_stride ? (uint32_t)D::W * nparts : nparts;
+    const uint32_t nb = nwin * D::B;
+    hipLaunchKernelGGL(k_sort_colscan, dim3(ceil_div(nb, 256)), dim3(256), 0, s, cnt, pre, tot, nwin, rows, D::B);
     device_exclusive_scan(tot, offsets, nullptr, nb, block_sums, total, s);
     hipLaunchKernelGGL((k_sort_scatter<CB, BITS>), dim3(grid), dim3(kSortThreads), 0, s, d_scalars, d_inf, n, per_part, wpad,
-                       nparts, offsets, pre, entries);
+                       nparts, table_stride, offsets, pre, entries);
 }
 
-void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
-                       uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
-                       uint32_t* entries, hipStream_t s) {
+void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
+                       uint32_t table_stride, uint32_t* cnt, uint32_t* pre, uint32_t* tot, uint32_t* offsets,
+                       uint32_t* block_sums, uint32_t* total, uint32_t* entries, hipStream_t s) {
     if (bits != 256 && bits != 128) throw std::invalid_argument("LDS sort: scalar bits must be 256 or 128");
     switch (c) {
 #define KZ_C(CB)                                                                                                     \
     case CB:                                                                                                         \
         if (bits == 256)                                                                                             \
-            lds_sort_launch<CB, 256>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s);  \
+            lds_sort_launch<CB, 256>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
+                                     entries, s);                                                                    \
         else                                                                                                         \
-            lds_sort_launch<CB, 128>(d_scalars, d_inf, n, cnt, pre, tot, offsets, block_sums, total, entries, s);  \
+            lds_sort_launch<CB, 128>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
+                                     entries, s);                                                                    \
         break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15) KZ_C(16)
 #undef KZ_C
@@ -334,10 +347,10 @@ void digits_multi_dispatch(bool scatter, const uint32_t* d_scalars, const uint8_
 }
 
 void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
-                     uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s) {
+                     uint32_t table_stride, uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s) {
     switch (c) {
 #define KZ_C(CB) \
-    case CB: launch<CB>(scatter, d_scalars, d_inf, n, counts_or_cursor, entries, s); break;
+    case CB: launch<CB>(scatter, d_scalars, d_inf, n, table_stride, counts_or_cursor, entries, s); break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15)
         KZ_C(16) KZ_C(17) KZ_C(18)
 #undef KZ_C

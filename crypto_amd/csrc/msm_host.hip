@@ -180,6 +180,19 @@ uint32_t reduce_segment(uint64_t total_buckets, uint32_t min_buckets_per_window,
     return seg;
 }
 
+// Carry-fold scratch in ws.run_bucket: 3 control words (padded to 4), the
+// list of long-run chunks (4 words, 16-byte aligned, per entry), then the
+// long runs {head, len} (2 words each; a long run has more than
+// kFoldShortRun chunks, so there are fewer than max_chunks / kFoldShortRun).
+size_t fold_words(uint32_t max_chunks) {
+    return 4 + 4 * (size_t)max_chunks + 2 * ((size_t)max_chunks / kFoldShortRun + 1);
+}
+uint32_t* fold_ctl(MsmWorkspace& ws) { return ws.run_bucket.as<uint32_t>(); }
+uint32_t* fold_list(MsmWorkspace& ws) { return ws.run_bucket.as<uint32_t>() + 4; }
+uint32_t* fold_runs(MsmWorkspace& ws, uint32_t max_chunks) {
+    return ws.run_bucket.as<uint32_t>() + 4 + 4 * (size_t)max_chunks;
+}
+
 uint32_t log2u(uint32_t v) {
     uint32_t l = 0;
     while ((1u << l) < v) ++l;
@@ -192,13 +205,14 @@ struct Stages {
     decltype(&launch_reduce_seg_bls) reduce_seg;
     decltype(&launch_reduce_level_bls) reduce_level;
     decltype(&launch_combine_bls) combine;
+    decltype(&launch_table_rows_bls) table_rows;
     int xyzz_words;
 };
 
 const Stages kBls = {launch_accumulate_bls, launch_fold_bls,    launch_reduce_seg_bls,
-                     launch_reduce_level_bls, launch_combine_bls, 56};
-const Stages kBn = {launch_accumulate_bn, launch_fold_bn, launch_reduce_seg_bn, launch_reduce_level_bn,
-                    launch_combine_bn,    36};
+                     launch_reduce_level_bls, launch_combine_bls, launch_table_rows_bls, 56};
+const Stages kBn = {launch_accumulate_bn, launch_fold_bn,    launch_reduce_seg_bn,
+                    launch_reduce_level_bn, launch_combine_bn, launch_table_rows_bn, 36};
 
 const Stages& stages_of(int curve) { return curve == 2 ? kBls : kBn; }
 
@@ -236,7 +250,7 @@ void reserve(const Stages& st, const Shape& sh, MsmWorkspace& ws) {
     ws.entries.ensure((sh.max_entries + 1) * sizeof(uint32_t));
     ws.bsum.ensure((size_t)sh.nb * pw);
     ws.carry.ensure((size_t)(max_chunks + 1) * pw);
-    ws.run_bucket.ensure((size_t)(max_chunks + 1 + 2) * sizeof(uint32_t));  // fold list + 2 control words
+    ws.run_bucket.ensure(fold_words(max_chunks) * sizeof(uint32_t));
     ws.segA.ensure((size_t)sh.nseg * pw);
     ws.segB.ensure((size_t)sh.nseg * pw);
     ws.winres.ensure((size_t)sh.nwin * pw);
@@ -279,8 +293,8 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
     }
     {
         StageTimer t(s, kStageReduce);
-        st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, ws.run_bucket.as<uint32_t>(),
-                ws.run_bucket.as<uint32_t>() + max_chunks + 1, ws.carry.as<uint32_t>());
+        st.fold(s, offsets, sh.nb, sh.chunk, max_chunks, sh.levels, fold_list(ws), fold_runs(ws, max_chunks),
+                fold_ctl(ws), ws.carry.as<uint32_t>());
         st.reduce_seg(ceil_div((size_t)sh.nseg * 4, 256), s, offsets, sh.chunk, ws.bsum.as<uint32_t>(),
                       ws.carry.as<uint32_t>(), sh.nseg, sh.seg, ws.segA.as<uint32_t>(), ws.segB.as<uint32_t>());
         // levels: segment pairs carry weight m = seg; each level multiplies it
@@ -322,7 +336,14 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     int bits = 256;
     size_t npts = n;
     int c = prm.c;
-    if (prm.g1 && curve == 2 && n <= (1u << 29)) {
+    const bool table = prm.table_stride != 0;
+    if (table) {
+        if (c < 5 || c > 18) throw std::invalid_argument("a window-table MSM needs the table's window bits");
+        if (n > prm.table_stride) throw std::invalid_argument("more scalars than table points");
+        if ((uint64_t)msm_table_rows(c) * prm.table_stride >= 0x80000000ull)
+            throw std::invalid_argument("window table beyond 2^31 points");
+    }
+    if (prm.g1 && !table && curve == 2 && n <= (1u << 29)) {
         const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
             bits = 128;
@@ -344,21 +365,22 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     const int tuned_seg = prm.seg ? prm.seg : prm.c ? 0 : msm_tuned_segment(curve, n, bits == 128 ? kMsmGlv : kMsmPlain);
     const int W = msm_num_windows(c, bits);
     const uint32_t B = 1u << (c - 1);
+    const uint32_t bw = table ? 1u : (uint32_t)W;  // bucket windows (a table shares one)
     Shape sh;
-    sh.nb = (uint32_t)W * B;
-    sh.nwin = (uint32_t)W;
-    sh.seg = reduce_segment(sh.nb, B, tuned_seg);
+    sh.nb = bw * B;
+    sh.nwin = bw;
+    sh.seg = reduce_segment(sh.nb, B, table ? prm.seg : tuned_seg);
     sh.t1 = B / sh.seg;
-    sh.nseg = (uint32_t)W * sh.t1;
+    sh.nseg = bw * sh.t1;
     sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
-    sh.levels = fold_levels(npts, sh.chunk);
+    sh.levels = fold_levels(table ? npts * (size_t)W : npts, sh.chunk);
     sh.max_entries = npts * (size_t)W;
-    sh.level_out = (size_t)W * parts_of(sh.t1);
+    sh.level_out = (size_t)bw * parts_of(sh.t1);
     reserve(st, sh, ws);
     MsmJob one{};
     one.n = (uint32_t)npts;
     one.c = (uint32_t)c;
-    one.nwin = (uint32_t)W;
+    one.nwin = bw;
     const bool lds = lds_sort_supported(c, npts);
     if (lds) {
         const size_t words = lds_sort_words(c, bits, npts);
@@ -367,16 +389,31 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     }
     auto sort = [&] {
         if (lds) {
-            lds_sort_dispatch(c, bits, d_scalars, d_inf, (uint32_t)npts, ws.sort_cnt.as<uint32_t>(),
+            lds_sort_dispatch(c, bits, d_scalars, d_inf, (uint32_t)npts, prm.table_stride, ws.sort_cnt.as<uint32_t>(),
                               ws.sort_pre.as<uint32_t>(), ws.counts.as<uint32_t>(), ws.offsets.as<uint32_t>(),
                               ws.block_sums.as<uint32_t>(), ws.total.as<uint32_t>(), ws.entries.as<uint32_t>(), s);
         } else {
             atomic_sort(sh, [&](bool scatter, uint32_t* cc, uint32_t* entries) {
-                digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)npts, cc, entries, s);
+                digits_dispatch(c, scatter, d_scalars, d_inf, (uint32_t)npts, prm.table_stride, cc, entries, s);
             }, ws, s);
         }
     };
     run_stages(st, sh, sort, nullptr, nullptr, one, 1, d_pts, ws, s, d_out);
+}
+
+int msm_table_window_bits(size_t n) {
+    for (int c = 16; c > 5; --c)
+        if ((uint64_t)msm_table_rows(c) * n >= (32ull << (c - 1))) return c;
+    return 5;
+}
+
+void points_table_build(int curve, const uint32_t* d_pts, const uint8_t* d_inf, size_t n, int c, uint32_t* d_table,
+                        hipStream_t s) {
+    if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
+    if (n == 0) return;
+    if ((uint64_t)msm_table_rows(c) * n >= 0x80000000ull) throw std::invalid_argument("window table beyond 2^31 points");
+    stages_of(curve).table_rows(d_pts, d_inf, (uint32_t)n, (uint32_t)c, (uint32_t)msm_table_rows(c), d_table, s);
+    KZ_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ batches

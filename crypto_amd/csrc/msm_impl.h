@@ -115,65 +115,79 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------ carry fold
-// The carries of a bucket that spans chunks j0..j1 (j0 = the chunk holding its
-// first entry) sit in carry[j0+1..j1], a contiguous run whose total must end
-// up in its head carry[j0+1].  With random scalars almost every run has one
-// or two carries; runs are long only when a bucket is huge -- e.g. the top
-// window covers only the few scalar bits above the last full window, so its
-// handful of buckets hold most points -- and a serial fold would then put the
-// whole run on one lane.  So:
-//   k_carry_runs   a run of at most kShortRun carries is folded serially by
-//                  the thread of its head chunk; each chunk of a longer run
-//                  is appended to a compact list (and the longest length is
-//                  kept);
+// The carries of a bucket whose entries span chunks j0..j1 (j0 = the chunk
+// holding its first entry) sit in carry[j0+1 .. j1]; they must all end up in
+// carry[j0+1], the head, which the reduction adds to the bucket sum.  With
+// random scalars most buckets have no carry or one; runs are long when a
+// bucket is large -- the top window's few buckets, GLV's carry window, or the
+// single shared bucket set of a window table (about rows * n / 2^(c-1)
+// entries per bucket).  So:
+//   k_carry_runs   one thread per bucket reads its run [head, head + len) off
+//                  the offsets; runs of at most kShortRun carries are folded
+//                  serially by that thread; a longer run is recorded as
+//                  {head, len} (and the longest length is kept);
+//   k_list_runs    a workgroup per recorded run reserves len slots of a
+//                  compact list (one atomic) and its threads write {chunk j,
+//                  position r in the run, len} for the run's chunks (one run
+//                  can hold most of a window's chunks);
 //   k_fold_level   pairwise tree over the listed chunks: level L adds
 //                  carry[j + 2^L] into carry[j] for run positions
 //                  r = 0 mod 2^(L+1); levels past log2(longest run) return at
 //                  once.  Grid-stride over the list, so a launch is small
 //                  whatever the number of chunks.
-// ctl[0] = list length, ctl[1] = longest listed run (both zeroed before).
+// ctl[0] = list length, ctl[1] = longest listed run, ctl[2] = recorded runs
+// (all zeroed before).
 // (Templated only so that each per-curve translation unit owns its copy.)
-constexpr uint32_t kShortRun = 8;
+constexpr uint32_t kShortRun = kFoldShortRun;
 constexpr uint32_t kFoldThreads = 64 * 1024;
 
 template <class F>
 __global__ void __launch_bounds__(256)
-k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
-             uint32_t* __restrict__ list, uint32_t* __restrict__ ctl, uint32_t* __restrict__ carry) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nchunks) return;
-    const uint32_t start = j * chunk;
-    if (start >= offsets[nb]) return;
-    const uint32_t g = bucket_of(offsets, nb, start);
-    if (offsets[g] >= start) return;  // chunk j starts its bucket: no carry
-    const uint32_t head = offsets[g] / chunk + 1;
-    const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
+k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint2* __restrict__ runs,
+             uint32_t* __restrict__ ctl, uint32_t* __restrict__ carry) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nb) return;
+    const uint32_t lo = offsets[g], hi = offsets[g + 1];
+    if (hi <= lo) return;  // empty bucket
+    const uint32_t head = lo / chunk + 1, len = (hi - 1) / chunk + 1 - head;
+    if (len <= 1) return;  // no carry, or one already in place
     if (len <= kShortRun) {
-        if (j != head || len == 1) return;
         Xyzz<F> acc = xyzz_load<F>(carry, head);
 #pragma unroll 1
         for (uint32_t k = 1; k < len; ++k) acc = xyzz_add<F, MulCall>(acc, xyzz_load<F>(carry, head + k));
         xyzz_store<F>(carry, head, acc);
         return;
     }
-    list[atomicAdd(&ctl[0], 1u)] = j;
-    if (j == head) atomicMax(&ctl[1], len);
+    runs[atomicAdd(&ctl[2], 1u)] = make_uint2(head, len);
+    atomicMax(&ctl[1], len);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256)
+k_list_runs(const uint2* __restrict__ runs, uint4* __restrict__ list, uint32_t* __restrict__ ctl) {
+    __shared__ uint32_t base;
+    const uint32_t nruns = ctl[2];
+#pragma unroll 1
+    for (uint32_t r = blockIdx.x; r < nruns; r += gridDim.x) {
+        const uint2 run = runs[r];  // {head, len}
+        __syncthreads();            // the previous run's readers of `base` are done
+        if (threadIdx.x == 0) base = atomicAdd(&ctl[0], run.y);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < run.y; k += blockDim.x) list[base + k] = make_uint4(run.x + k, k, run.y, 0u);
+    }
 }
 
 template <class F>
 __global__ void __launch_bounds__(64)
-k_fold_level(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, const uint32_t* __restrict__ list,
-             const uint32_t* __restrict__ ctl, uint32_t level, uint32_t* __restrict__ carry) {
+k_fold_level(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, uint32_t level,
+             uint32_t* __restrict__ carry) {
     const uint32_t s = 1u << level;
     if (s >= ctl[1]) return;
     const uint32_t count = ctl[0];
 #pragma unroll 1
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const uint32_t j = list[i];
-        const uint32_t g = bucket_of(offsets, nb, j * chunk);
-        const uint32_t head = offsets[g] / chunk + 1;
-        const uint32_t len = (offsets[g + 1] - 1) / chunk + 1 - head;
-        const uint32_t r = j - head;
+        const uint4 e = list[i];
+        const uint32_t j = e.x, r = e.y, len = e.z;
         if ((r & (2 * s - 1)) != 0 || r + s >= len) continue;
         xyzz_store<F>(carry, j, xyzz_add<F, MulCall>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s)));
     }
@@ -361,6 +375,37 @@ __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ win
         r = xyzz_add_quad<F, MulInline>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
     }
     if (quad_lane() == 0) write_output<C>(r, out + (size_t)job * 3 * C::FB);
+}
+
+// ------------------------------------------------------------ window tables
+// Rows of a window table: row w holds 2^(c w) P_i at w n + i (row 0 = the
+// points), as affine Montgomery records; one thread per point walks the rows
+// by c doublings each and normalises every row point on its own.  The curves'
+// groups have odd order, so doubling a non-identity point never gives the
+// identity.  Identity points (inf[i]) are copied as they are into every row:
+// MSMs skip them through the same flags.
+template <class C>
+__global__ void __launch_bounds__(64)
+k_table_rows(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, uint32_t n, uint32_t c,
+             uint32_t nwin, uint32_t* __restrict__ table) {
+    using F = typename C::Fp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Affine<F> a = affine_load<F>(pts, i);
+    affine_store<F>(table, i, a);
+    const bool id = inf && inf[i];
+    Xyzz<F> q = xyzz_from_affine<F>(a);
+#pragma unroll 1
+    for (uint32_t w = 1; w < nwin; ++w) {
+        Affine<F> r = a;
+        if (!id) {
+#pragma unroll 1
+            for (uint32_t k = 0; k < c; ++k) q = xyzz_dbl<F>(q);
+            bool is_inf;
+            xyzz_to_affine<F>(q, r, is_inf);
+        }
+        affine_store<F>(table, (size_t)w * n + i, r);
+    }
 }
 
 template <class C>

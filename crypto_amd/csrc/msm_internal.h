@@ -8,8 +8,10 @@
 namespace kz {
 
 // Signed-digit count (scatter = false) or scatter (scatter = true) pass.
+// table_stride > 0: the points are a window table of that row stride, and
+// every window's digits share one bucket set (keys |d| - 1, entries w T + i).
 void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
-                     uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s);
+                     uint32_t table_stride, uint32_t* counts_or_cursor, uint32_t* entries, hipStream_t s);
 
 // LDS-histogram sort of one MSM (c in [5, 16], n >= 4096): fills offsets
 // (W 2^(c-1) + 1 words) and entries.  cnt and pre need lds_sort_words(c, n)
@@ -21,9 +23,9 @@ inline size_t lds_sort_words(int c, int bits, size_t n) {
     lds_sort_shape(c, bits, n, wpad, nparts, per_part);
     return (size_t)msm_num_windows(c, bits) * nparts * ((size_t)1 << (c - 1));
 }
-void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n, uint32_t* cnt,
-                       uint32_t* pre, uint32_t* tot, uint32_t* offsets, uint32_t* block_sums, uint32_t* total,
-                       uint32_t* entries, hipStream_t s);
+void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
+                       uint32_t table_stride, uint32_t* cnt, uint32_t* pre, uint32_t* tot, uint32_t* offsets,
+                       uint32_t* block_sums, uint32_t* total, uint32_t* entries, hipStream_t s);
 
 // The same passes over every job of a batch (d_jobs: njobs descriptors on the
 // device, nblocks = total 256-thread blocks).

@@ -6,14 +6,16 @@
 namespace kz {
 
 void launch_fold_bls(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk, uint32_t nchunks,
-                     uint32_t levels, uint32_t* list, uint32_t* ctl, uint32_t* carry) {
-    device_fill_u32(ctl, 0, 2, s);
-    hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div(nchunks, 256)), dim3(256), 0, s, offsets, nb, chunk,
-                       nchunks, list, ctl, carry);
+                     uint32_t levels, uint32_t* list, uint32_t* runs, uint32_t* ctl, uint32_t* carry) {
+    device_fill_u32(ctl, 0, 3, s);
+    uint4* l4 = reinterpret_cast<uint4*>(list);
+    uint2* r2 = reinterpret_cast<uint2*>(runs);
+    hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div(nb, 256)), dim3(256), 0, s, offsets, nb, chunk, r2, ctl,
+                       carry);
+    hipLaunchKernelGGL(k_list_runs<CurveBLS12381::Fp>, dim3(256), dim3(256), 0, s, r2, l4, ctl);
     const unsigned grid = ceil_div(nchunks < kFoldThreads ? nchunks : kFoldThreads, 64);
     for (uint32_t level = 0; level < levels; ++level)
-        hipLaunchKernelGGL(k_fold_level<CurveBLS12381::Fp>, dim3(grid), dim3(64), 0, s, offsets, nb, chunk, list, ctl, level,
-                           carry);
+        hipLaunchKernelGGL(k_fold_level<CurveBLS12381::Fp>, dim3(grid), dim3(64), 0, s, l4, ctl, level, carry);
 }
 
 void launch_reduce_seg_bls(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,

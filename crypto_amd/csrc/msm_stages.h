@@ -15,7 +15,8 @@ namespace kz {
                                  const uint32_t* offsets, uint32_t nb, uint32_t chunk, const uint32_t* pts, \
                                  uint32_t* bsum, uint32_t* carry);                                          \
     void launch_fold_##sfx(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32_t chunk,             \
-                           uint32_t nchunks, uint32_t levels, uint32_t* list, uint32_t* ctl, uint32_t* carry); \
+                           uint32_t nchunks, uint32_t levels, uint32_t* list, uint32_t* runs, uint32_t* ctl,   \
+                           uint32_t* carry);                                                                \
     void launch_reduce_seg_##sfx(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,     \
                                  const uint32_t* bsum, const uint32_t* carry, uint32_t nseg, uint32_t seg,  \
                                  uint32_t* segA, uint32_t* segB);                                           \
@@ -23,7 +24,9 @@ namespace kz {
                                    const RedLevel* tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts,  \
                                    uint32_t lm, uint32_t* outA, uint32_t* outB);                            \
     void launch_combine_##sfx(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,  \
-                              uint32_t njobs, uint8_t* out);
+                              uint32_t njobs, uint8_t* out);                                                \
+    void launch_table_rows_##sfx(const uint32_t* pts, const uint8_t* inf, uint32_t n, uint32_t c,           \
+                                 uint32_t nwin, uint32_t* table, hipStream_t s);
 
 KZ_DECLARE_STAGES(bls)
 KZ_DECLARE_STAGES(bn)

@@ -10,6 +10,11 @@ void launch_combine_bls(hipStream_t s, const uint32_t* winres, const MsmJob* job
                        jobs, one, njobs, out);
 }
 
+void launch_table_rows_bls(const uint32_t* pts, const uint8_t* inf, uint32_t n, uint32_t c, uint32_t nwin,
+                          uint32_t* table, hipStream_t s) {
+    hipLaunchKernelGGL(k_table_rows<CurveBLS12381>, dim3(ceil_div(n, 64)), dim3(64), 0, s, pts, inf, n, c, nwin, table);
+}
+
 void launch_compress_bls(const uint8_t* wire, uint8_t* out, hipStream_t s) {
     hipLaunchKernelGGL(k_compress_point<CurveBLS12381>, dim3(1), dim3(64), 0, s, wire, out);
 }

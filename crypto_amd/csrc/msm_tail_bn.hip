@@ -10,6 +10,11 @@ void launch_combine_bn(hipStream_t s, const uint32_t* winres, const MsmJob* jobs
                        one, njobs, out);
 }
 
+void launch_table_rows_bn(const uint32_t* pts, const uint8_t* inf, uint32_t n, uint32_t c, uint32_t nwin,
+                         uint32_t* table, hipStream_t s) {
+    hipLaunchKernelGGL(k_table_rows<CurveBN254>, dim3(ceil_div(n, 64)), dim3(64), 0, s, pts, inf, n, c, nwin, table);
+}
+
 void launch_compress_bn(const uint8_t* wire, uint8_t* out, hipStream_t s) {
     hipLaunchKernelGGL(k_compress_point<CurveBN254>, dim3(1), dim3(64), 0, s, wire, out);
 }

@@ -67,8 +67,9 @@ bool device_equal(const void* a, const void* b, size_t bytes, int* d_flag, hipSt
     return h == 0;
 }
 
-// Mean milliseconds of fn() on stream s: one untimed warm-up, then three
-// trials of enough calls for about 3 ms each; the fastest trial counts.
+// Mean milliseconds of fn() on stream s: one untimed warm-up, then five
+// trials of at least two calls and about 20 ms each; the fastest trial counts
+// (a single call of a ~7 ms MSM varies by several per cent from run to run).
 template <class Fn>
 float time_ms(Fn&& fn, hipStream_t s) {
     hipEvent_t a, b;
@@ -82,9 +83,10 @@ float time_ms(Fn&& fn, hipStream_t s) {
         KZ_HIP(hipEventSynchronize(b));
         float first = 0;
         KZ_HIP(hipEventElapsedTime(&first, a, b));
-        int reps = first > 0 ? (int)(3.0f / first) + 1 : 20;
-        if (reps > 20) reps = 20;
-        for (int t = 0; t < 3; ++t) {
+        int reps = first > 0 ? (int)(20.0f / first) + 1 : 50;
+        if (reps < 2) reps = 2;
+        if (reps > 50) reps = 50;
+        for (int t = 0; t < 5; ++t) {
             KZ_HIP(hipEventRecord(a, s));
             for (int r = 0; r < reps; ++r) fn();
             KZ_HIP(hipEventRecord(b, s));

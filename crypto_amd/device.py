@@ -63,6 +63,42 @@ def msm_prepared(pp: PreparedPoints, d_scalars: torch.Tensor, n: int | None = No
     return out
 
 
+class PointsTable:
+    """Window table of prepared points (kinet_hip_points_table_build): row w
+    holds 2^(c w) P_i, so an MSM over it reduces one window and combines none."""
+
+    def __init__(self, pp: PreparedPoints, window_bits: int = 0, stream=None):
+        lib = require_device()
+        self.pp = pp
+        self.curve, self.n = pp.curve, pp.n
+        self.c = window_bits or lib.kinet_hip_table_window_bits(pp.n)
+        nbytes = lib.kinet_hip_table_bytes(pp.curve, pp.n, self.c)
+        if nbytes == 0 and pp.n:
+            raise KinetError("kinet_hip_table_bytes", -2, GPUKIT_STATUS)
+        self.table = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=pp.pts.device)
+        rc = lib.kinet_hip_points_table_build(pp.curve, pp.pts.data_ptr(), pp.inf.data_ptr(), pp.n, self.c,
+                                              self.table.data_ptr(), stream_handle(stream))
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_points_table_build", rc, GPUKIT_STATUS)
+        (stream or torch.cuda.current_stream()).synchronize()
+
+    def msm(self, d_scalars: torch.Tensor, n: int | None = None, out: torch.Tensor | None = None, stream=None,
+            sync: bool = True) -> torch.Tensor:
+        """MSM of the first n table points (default all) with the scalars; device output record."""
+        lib = require_device()
+        n = self.n if n is None else n
+        pb = point_bytes(self.curve)
+        if out is None:
+            out = torch.empty(pb + pb // 2, dtype=torch.uint8, device=d_scalars.device)
+        rc = lib.kinet_hip_msm_table(self.curve, d_scalars.data_ptr(), self.table.data_ptr(),
+                                     self.pp.inf.data_ptr(), self.n, n, self.c, out.data_ptr(), stream_handle(stream))
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_msm_table", rc, GPUKIT_STATUS)
+        if sync:
+            (stream or torch.cuda.current_stream()).synchronize()
+        return out
+
+
 def check_g1(pp: PreparedPoints) -> bool:
     """True if every non-identity prepared point is on the curve and in G1."""
     lib = require_device()

@@ -104,6 +104,11 @@ def library():
         lib.kinet_hip_msm_batch_run.argtypes = [vp, vp, vp, vp, vp, i32, vp]
         lib.kinet_hip_msm_set_window_table.argtypes = [u32, i32, ctypes.POINTER(i32), i32]
         lib.kinet_hip_msm_window_bits.argtypes = [u32, sz, i32]
+        lib.kinet_hip_table_window_bits.argtypes = [sz]
+        lib.kinet_hip_table_bytes.argtypes = [u32, sz, i32]
+        lib.kinet_hip_table_bytes.restype = sz
+        lib.kinet_hip_points_table_build.argtypes = [u32, vp, vp, sz, i32, vp, vp]
+        lib.kinet_hip_msm_table.argtypes = [u32, vp, vp, vp, sz, sz, i32, vp, vp]
         ip = ctypes.POINTER(i32)
         lib.kinet_hip_tune_msm.argtypes = [u32, i32, i32, i32]
         lib.kinet_hip_tune_ntt.argtypes = [u32, i32, i32]

@@ -46,6 +46,31 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
 int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                               size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
+/* ------------------------------------------------------------- window tables
+ * Fixed point sets (a KZG SRS) can trade HBM for the doubling chain: a window
+ * table of n prepared points for window bits c holds rows w = 0 .. R-1
+ * (R = windows of c-bit signed digits over 256-bit scalars: ceil(256 / c),
+ * plus one when c divides 256) with row w = 2^(c w) P_i at record w n + i.
+ * An MSM over the table sends the digit of every window to one shared set of
+ * 2^(c-1) buckets, so it reduces one window instead of R and needs no window
+ * combine; the accumulation work is unchanged.  Any curve points qualify (no
+ * subgroup condition).  Results equal kinet_hip_msm_prepared's byte for byte.
+ *
+ * Window bits the library picks for a table of n points (the widest c <= 16
+ * leaving about 32 digits per bucket). */
+int kinet_hip_table_window_bits(size_t n);
+/* Device bytes of a table of n points (0 for a bad curve or c). */
+size_t kinet_hip_table_bytes(uint32_t curve, size_t n, int window_bits);
+/* Builds the table of n prepared points (d_inf may be NULL) into d_table
+ * (kinet_hip_table_bytes).  Asynchronous on `stream`. */
+int kinet_hip_points_table_build(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int window_bits,
+                                 void* d_table, void* stream);
+/* MSM of the first n <= table_n points of a table built for table_n points
+ * with these window bits; d_inf = the points' identity flags (or NULL).
+ * Output as kinet_hip_msm_prepared. */
+int kinet_hip_msm_table(uint32_t curve, const void* d_scalars, const void* d_table, const void* d_inf,
+                        size_t table_n, size_t n, int window_bits, void* d_out, void* stream);
+
 /* Sets *d_bad (device int, zeroed by the caller) nonzero if any non-identity
  * prepared point is off the curve or, for BLS12-381, outside G1 (tested as
  * phi(P) = lambda P).  Asynchronous on `stream`. */
