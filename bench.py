@@ -119,11 +119,13 @@ def gpu_fixed_base(lib, torch, curve, d_scalars, n, stream):
     return out
 
 
-def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile, g1=True):
+def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile, g1=True, table=False):
     """Returns (ms_per_step, stage_ms dict, last output bytes, check bytes, extras).
     g1: the points are checked once (outside the timed region) to lie in G1,
     and the MSM runs through kinet_hip_msm_prepared_g1 (GLV split on
-    BLS12-381); else through the plain kinet_hip_msm_prepared."""
+    BLS12-381); else through the plain kinet_hip_msm_prepared.
+    table: a window table of the points is built once (outside the timed
+    region) and the MSM runs through kinet_hip_msm_table."""
     from crypto_amd import inputs
 
     stream = torch.cuda.Stream()
@@ -142,8 +144,17 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
             raise RuntimeError("generated points failed the G1 check")
         pb = 96 if curve == 2 else 64
         out = torch.empty(pb + pb // 2, dtype=torch.uint8, device="cuda")
+        t0 = time.time()
+        tab = D.PointsTable(pp, window_bits=window_bits, stream=stream) if table else None
+        table_s = time.time() - t0
+
+        def step():
+            if tab is not None:
+                tab.msm(d_s, out=out, stream=stream, sync=False)
+            else:
+                D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False, g1=in_g1)
         for _ in range(warmup):
-            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False, g1=in_g1)
+            step()
         stream.synchronize()
         torch.cuda.synchronize()
         if profile:
@@ -152,7 +163,7 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(steps):
-            D.msm_prepared(pp, d_s, window_bits=window_bits, out=out, stream=stream, sync=False, g1=in_g1)
+            step()
         e1.record(stream)
         stream.synchronize()
         torch.cuda.synchronize()
@@ -173,8 +184,12 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         tk = np.array([[(t >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]], dtype=np.uint64)
         d_t = torch.from_numpy(tk.view("u1").reshape(-1).copy()).to("cuda")
         check = gpu_fixed_base(lib, torch, curve, d_t, 1, stream).cpu().numpy().tobytes()
-    return ms, stage, result, check, {"points_gen_s": gen_s, "scalars": ss, "points_wire": d_pts_wire,
-                                      "path": "g1_glv" if (in_g1 and curve == 2) else "plain"}
+        extra = {"points_gen_s": gen_s, "scalars": ss, "points_wire": d_pts_wire,
+                 "path": "table" if table else ("g1_glv" if (in_g1 and curve == 2) else "plain")}
+        if tab is not None:
+            extra.update(table_s=table_s, table_gb=tab.table.numel() / 1e9, table_c=tab.c)
+            del tab
+    return ms, stage, result, check, extra
 
 
 def cpu_baseline(curve, scalars, points_wire, log2n):
@@ -464,17 +479,30 @@ def ntt_and_kzg(lib, torch, p_mad: float, reps: int = 10):
             def commit():
                 rc = lib.kzg_commit_async(ctx, srs.handle, d_ev.data_ptr(), n, 0, d_rec.data_ptr(), d_err.data_ptr())
                 assert rc == 0, rc
-            commit()
-            stream.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(3):
+            def timed():
                 commit()
-            e1.record(stream)
-            stream.synchronize()
-            ms = e0.elapsed_time(e1) / 3
+                stream.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(3):
+                    commit()
+                e1.record(stream)
+                stream.synchronize()
+                return e0.elapsed_time(e1) / 3, d_rec.cpu().numpy().tobytes()
+
+            ms, rec = timed()
+            t0 = time.time()
+            tbytes = srs.precompute()  # window table of the SRS
+            build_s = time.time() - t0
+            tms, trec = timed()
             out["kzg_2p22"][CURVE_NAMES[curve]] = {"ms_per_commit": round(ms, 3),
                                                    "commits_per_s": round(1e3 / ms, 2),
+                                                   "with_srs_window_table": {
+                                                       "ms_per_commit": round(tms, 3),
+                                                       "commits_per_s": round(1e3 / tms, 2),
+                                                       "table_gb": round(tbytes / 1e9, 2),
+                                                       "build_s": round(build_s, 2),
+                                                       "same_commitment": trec == rec},
                                                    "error_flag": int(d_err.item())}
             lib.kzg_ctx_destroy(ctx)
             srs.close()
@@ -605,6 +633,17 @@ def main():
         oms, _, ores, ocheck, _ = timed_msm(lib, torch, D, other, n, 777, max(3, args.steps // 2), 2, 0, False)
         extras[f"{CURVE_NAMES[other]}_mpoints_s"] = round(n / oms / 1e3, 3)
         extras[f"{CURVE_NAMES[other]}_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
+        # fixed point sets (an SRS): window table built once, then one window per MSM
+        for cv, seed in ((curve, 20261203), (other, 777)):
+            tms, tst, tres, tcheck, text = timed_msm(lib, torch, D, cv, n, seed, max(3, args.steps // 2), 2, 0,
+                                                     True, table=True)
+            extras[f"{CURVE_NAMES[cv]}_window_table_path"] = {
+                "mpoints_s": round(n / tms / 1e3, 3), "ms": round(tms, 4),
+                "verified": tres[:(96 if cv == 2 else 64)] == tcheck,
+                "stage_ms": {k: round(v, 4) for k, v in tst.items()},
+                "table": f"c={text['table_c']}, {text['table_gb']:.2f} GB, built once in {text['table_s']:.2f} s "
+                         "(outside the timed region)"}
+            torch.cuda.empty_cache()
         extras.update(ntt_and_kzg(lib, torch, p_mad))
         line["extras"] = extras
     print(json.dumps(line), flush=True)

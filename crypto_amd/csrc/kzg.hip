@@ -68,6 +68,20 @@ static void srs_upload(kzg_srs* srs, const uint8_t* d_wire_src, bool src_on_host
                      s);
 }
 
+void srs_msm(const kzg_srs* srs, const uint32_t* d_scalars, size_t n, int window_bits, MsmWorkspace& ws,
+             hipStream_t s, uint8_t* d_out) {
+    MsmParams prm;
+    if (srs->table_c && (window_bits == 0 || window_bits == srs->table_c)) {
+        prm.c = srs->table_c;
+        prm.table_stride = (uint32_t)srs->n;
+        msm_device(srs->curve, srs->table.as<uint32_t>(), srs->inf.as<uint8_t>(), d_scalars, n, prm, ws, s, d_out);
+        return;
+    }
+    prm.g1 = srs->g1;
+    prm.c = window_bits;
+    msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_scalars, n, prm, ws, s, d_out);
+}
+
 // iNTT of the evaluations into ctx->coeffs, then the MSM against the SRS.
 static void commit_device(kzg_ctx* ctx, const kzg_srs* srs, const uint32_t* d_evals, size_t n, int window_bits,
                           uint8_t* d_out, int* d_err) {
@@ -76,11 +90,7 @@ static void commit_device(kzg_ctx* ctx, const kzg_srs* srs, const uint32_t* d_ev
     uint32_t* tmp = static_cast<uint32_t*>(ctx->tmp.ensure(32 * n));
     const int field = srs->curve == 2 ? 0 : 1;
     ntt_device(field, logn, true, d_evals, coeffs, tmp, d_err, ctx->stream);
-    MsmParams prm;
-    prm.g1 = srs->g1;
-    prm.c = window_bits;
-    msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), coeffs, n, prm, ctx->ws, ctx->stream,
-               d_out);
+    srs_msm(srs, coeffs, n, window_bits, ctx->ws, ctx->stream, d_out);
 }
 
 static kzg_ctx* default_ctx() {
@@ -170,6 +180,31 @@ void kzg_srs_free(kzg_srs* srs) { delete srs; }
 
 size_t kzg_srs_size(const kzg_srs* srs) { return srs ? srs->n : 0; }
 
+int kzg_srs_precompute(kzg_srs* srs, int window_bits) {
+    if (!srs || window_bits < 0 || (window_bits > 0 && window_bits < 5) || window_bits > 18)
+        return KZ_CRYPTO_ERR_INPUT;
+    try {
+        Engine& e = engine();
+        std::lock_guard<std::mutex> g(e.mu);
+        const int c = window_bits ? window_bits : msm_table_window_bits(srs->n);
+        if ((uint64_t)msm_table_rows(c) * srs->n >= 0x80000000ull) return KZ_CRYPTO_ERR_INPUT;
+        srs->table_c = 0;  // unusable until rebuilt
+        srs->table.ensure((size_t)msm_table_rows(c) * srs->n * affine_words(srs->curve) * sizeof(uint32_t));
+        points_table_build(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), srs->n, c,
+                           srs->table.as<uint32_t>(), e.stream);
+        KZ_HIP(hipStreamSynchronize(e.stream));
+        srs->table_c = c;
+        return KZ_CRYPTO_OK;
+    } catch (const std::exception& ex) {
+        set_last_error(ex.what());
+        return KZ_CRYPTO_ERR_BACKEND;
+    }
+}
+
+size_t kzg_srs_table_bytes(const kzg_srs* srs) {
+    return srs && srs->table_c ? (size_t)msm_table_rows(srs->table_c) * srs->n * affine_words(srs->curve) * 4 : 0;
+}
+
 uint32_t kzg_srs_curve(const kzg_srs* srs) { return srs ? (uint32_t)srs->curve : 0; }
 
 int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* points_out) {
@@ -231,9 +266,7 @@ int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* resul
         uint32_t* d_sc = static_cast<uint32_t*>(e.h2d_a.ensure(32 * n));
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
         KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));
-        MsmParams prm;
-        prm.g1 = srs->g1;
-        msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
+        srs_msm(srs, d_sc, n, 0, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
         KZ_HIP(hipStreamSynchronize(s));
         return KZ_CRYPTO_OK;
@@ -254,9 +287,7 @@ int kzg_commit_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, uint8
         uint32_t* d_sc = static_cast<uint32_t*>(e.h2d_a.ensure(32 * n));
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes(srs->curve) + 16));
         KZ_HIP(hipMemcpyAsync(d_sc, coeffs, 32 * n, hipMemcpyHostToDevice, s));
-        MsmParams prm;
-        prm.g1 = srs->g1;
-        msm_device(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), d_sc, n, prm, e.ws, s, d_out);
+        srs_msm(srs, d_sc, n, 0, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(commit_out, d_out + point_wire_bytes(srs->curve), compressed_bytes(srs->curve),
                               hipMemcpyDeviceToHost, s));
         KZ_HIP(hipStreamSynchronize(s));

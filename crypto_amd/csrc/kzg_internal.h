@@ -17,6 +17,8 @@ struct kzg_srs {
     kz::DevBuf inf;   // identity flags
     kz::DevBuf wire;  // wire points (kept for export)
     bool g1 = false;  // every point lies in G1 (generated, or checked at creation)
+    kz::DevBuf table; // optional window table of the points (kzg_srs_precompute)
+    int table_c = 0;  // its window bits, 0 = none
 };
 
 struct kzg_ctx {
@@ -26,6 +28,12 @@ struct kzg_ctx {
 };
 
 namespace kz {
+
+// MSM of the first n SRS points with device scalars: through the SRS's window
+// table when it has one (and window_bits is 0 or the table's), else the
+// prepared points (GLV when they are known to lie in G1).
+void srs_msm(const kzg_srs* srs, const uint32_t* d_scalars, size_t n, int window_bits, MsmWorkspace& ws,
+             hipStream_t s, uint8_t* d_out);
 
 // value < r for a 32-byte LE element of the curve's scalar field
 bool canonical_fr(int curve, const uint8_t* le32);

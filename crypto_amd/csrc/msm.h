@@ -20,7 +20,7 @@ struct MsmWorkspace {
 
 struct MsmParams {
     int c = 0;         // window bits; 0 = choose from n
-    int chunk = 32;    // sorted entries per accumulation thread
+    int chunk = 0;     // sorted entries per accumulation thread; 0 = from the entries per bucket
     bool g1 = false;   // caller guarantees every point lies in G1: BLS12-381 may use the GLV split
     int seg = 0;       // first-level reduction segment; 0 = tuned cell (when c = 0) or rule
     // > 0: d_pts is a window table built with window bits c for table_stride

@@ -230,6 +230,18 @@ struct Shape {
     uint32_t max_chunks() const { return ceil_div(max_entries, chunk); }
 };
 
+// Sorted entries per accumulation thread: 32, doubled (up to 256) while a
+// bucket would still average more than 8 chunks -- so its carries fold
+// serially in a few steps instead of a tree -- and at least 2^18 threads
+// remain.  Only window tables of large point sets (thousands of entries in
+// each of 2^(c-1) shared buckets) get past 32.
+uint32_t auto_chunk(size_t max_entries, uint32_t nb) {
+    const size_t per_bucket = max_entries / (nb ? nb : 1);
+    uint32_t chunk = 32;
+    while (chunk < 256 && per_bucket / chunk > 8 && max_entries / (2 * (size_t)chunk) >= ((size_t)1 << 18)) chunk *= 2;
+    return chunk;
+}
+
 // A bucket holds at most n entries (one per point), so its carry run is at
 // most ceil(n / chunk) + 1 long: log2 of that many tree levels suffice.
 uint32_t fold_levels(size_t max_n, uint32_t chunk) {
@@ -372,9 +384,9 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     sh.seg = reduce_segment(sh.nb, B, table ? prm.seg : tuned_seg);
     sh.t1 = B / sh.seg;
     sh.nseg = bw * sh.t1;
-    sh.chunk = (uint32_t)(prm.chunk > 0 ? prm.chunk : 32);
-    sh.levels = fold_levels(table ? npts * (size_t)W : npts, sh.chunk);
     sh.max_entries = npts * (size_t)W;
+    sh.chunk = prm.chunk > 0 ? (uint32_t)prm.chunk : auto_chunk(sh.max_entries, sh.nb);
+    sh.levels = fold_levels(table ? npts * (size_t)W : npts, sh.chunk);
     sh.level_out = (size_t)bw * parts_of(sh.t1);
     reserve(st, sh, ws);
     MsmJob one{};

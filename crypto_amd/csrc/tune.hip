@@ -168,6 +168,7 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
             throw Mismatch("MSM schedule c=" + std::to_string(c) + " seg=" + std::to_string(seg) +
                            " differs from the built-in schedule at 2^" + std::to_string(lg));
     };
+    const bool verbose = std::getenv("KINET_HIP_TUNE_VERBOSE") != nullptr;
     const size_t msm_n = mode == kMsmGlv ? 2 * n : n;
     const int lo = std::max(5, lg - 9), hi = std::min(mode == kMsmGlv ? 16 : 18, std::max(5, lg - 1 + (mode == kMsmGlv)));
     Choice best;
@@ -175,6 +176,7 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
         if (mode == kMsmGlv && !lds_sort_supported(c, msm_n)) continue;
         const float ms = time_ms([&] { run(c, 0, d_out); }, s);
         check(c, 0);
+        if (verbose) std::fprintf(stderr, "msm curve=%d mode=%d 2^%d c=%d seg=rule %.4f ms\n", curve, mode, lg, c, ms);
         if (best.c == 0 || ms < best.ms) best = Choice{c, 0, ms};
     }
     if (best.c == 0) throw std::invalid_argument("no window width applies to this size class");
@@ -182,6 +184,8 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
         if (seg > (1 << (best.c - 1))) continue;
         const float ms = time_ms([&] { run(best.c, seg, d_out); }, s);
         check(best.c, seg);
+        if (verbose)
+            std::fprintf(stderr, "msm curve=%d mode=%d 2^%d c=%d seg=%d %.4f ms\n", curve, mode, lg, best.c, seg, ms);
         if (best.seg == 0 || ms < best.ms) {
             best.seg = seg;
             best.ms = ms;

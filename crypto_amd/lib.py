@@ -82,6 +82,9 @@ def library():
         lib.kzg_srs_curve.argtypes = [vp]
         lib.kzg_srs_curve.restype = u32
         lib.kzg_srs_export.argtypes = [vp, sz, sz, ctypes.c_char_p]
+        lib.kzg_srs_precompute.argtypes = [vp, i32]
+        lib.kzg_srs_table_bytes.argtypes = [vp]
+        lib.kzg_srs_table_bytes.restype = sz
         lib.kzg_commit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p]
         lib.kzg_commit_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(sz), sz,
                                          ctypes.c_char_p]
@@ -210,6 +213,13 @@ class KzgSrs:
         if rc != 0:
             raise KinetError("kzg_msm", rc, CRYPTO_STATUS)
         return out.raw
+
+    def precompute(self, window_bits: int = 0) -> int:
+        """Builds the SRS's window table (kzg_srs_precompute); returns its device bytes."""
+        rc = library().kzg_srs_precompute(self.handle, window_bits)
+        if rc != 0:
+            raise KinetError("kzg_srs_precompute", rc, CRYPTO_STATUS)
+        return library().kzg_srs_table_bytes(self.handle)
 
     def close(self):
         if self.handle:

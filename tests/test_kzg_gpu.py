@@ -101,3 +101,26 @@ def test_commit_coeffs_matches_evals_path(ca, curve):
     lib.kzg_commit_coeffs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p]
     assert lib.kzg_commit_coeffs(srs.handle, coeffs, n, out) == 0
     assert out.raw == srs.commit(ev)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_precomputed_srs_commits_unchanged(ca, curve):
+    """An SRS with a window table (kzg_srs_precompute) gives the same
+    commitments, MSMs and coefficient commits, for full size and prefixes."""
+    n = 1 << 13
+    tau = int.from_bytes(O.gen_scalars(curve, 21, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, n)
+    evs = {m: _evals(curve, m, 300 + m) for m in (1, 16, 4096, n)}
+    before = {m: srs.commit(e) for m, e in evs.items()}
+    sc = O.gen_scalars(curve, 7, 5000)
+    msm_before = srs.msm(sc)
+    nbytes = srs.precompute()
+    assert nbytes > 0
+    for m, e in evs.items():
+        assert srs.commit(e) == before[m], m
+    assert srs.msm(sc) == msm_before
+    assert before[n] == O.kzg_commit_closed_form(curve, tau.to_bytes(32, "little"), evs[n])
+    assert srs.precompute(9) > 0  # rebuilt with other window bits
+    assert srs.commit(evs[4096]) == before[4096]
+    with pytest.raises(ca.KinetError):
+        srs.precompute(4)
