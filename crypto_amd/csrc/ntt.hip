@@ -9,8 +9,9 @@
 // Schedule: n = A * B * C (each <= 2^10), one to three passes of the same
 // kernel.  A pass runs independent sub-DFTs of length L <= 1024 held in LDS
 // (DIT: bit-reversed load, log2 L radix-2 stages fused in pairs into radix-4
-// register passes, twiddles w_L^k staged in LDS), optionally multiplies its outputs by the four-step twiddle
-// w_n^(e*k), and writes them back through a strided address map:
+// register passes, twiddles w_L^k read from the resident table), optionally
+// multiplies its outputs by the four-step twiddle w_n^(e*k), and writes them
+// back through a strided address map:
 //   pass 1  columns of length A, stride B*C, twiddle w_n^(m k)      (in place)
 //   pass 2  columns of length B inside each row, twiddle w_(BC)^(c k) (in place)
 //   pass 3  contiguous rows of length C, written transposed to natural order
@@ -66,17 +67,18 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
            const uint32_t* __restrict__ ninv, NttPass d, uint32_t n_mask, int* __restrict__ err) {
     static_assert(F::N == 9, "Fr limbs");
     __shared__ uint32_t sh[kNttTile * F::N];
-    __shared__ uint32_t shtw[(kNttTile / 2) * F::N];
     // blockIdx.y selects one transform of a batch laid back to back (n words x 8)
     const size_t xform = (size_t)blockIdx.y * ((size_t)n_mask + 1) * 8;
     in += xform;
     out += xform;
-    const uint32_t L = 1u << d.logL, half_L = L >> 1, tile = d.T * L;
+    const uint32_t L = 1u << d.logL, tile = d.T * L;
     const uint32_t q_start = blockIdx.x * d.T;
     const uint32_t tid = threadIdx.x;
 
-    for (uint32_t k = tid; k < half_L; k += kNttThreads)
-        fe_store_words9<F>(shtw + k * F::N, fe_load_words9<F>(tw + (size_t)(k << d.root_shift) * F::N));
+    // twiddles w_L^k = w_n^(k << root_shift) come straight from the resident
+    // table (a few hundred lines, hot in L2): LDS holds only the tile, so four
+    // workgroups fit on a CU
+    const uint32_t tws = d.root_shift;
 
     bool bad = false;
     for (uint32_t e = tid; e < tile; e += kNttThreads) {
@@ -96,7 +98,7 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
     __syncthreads();
 
     // Butterflies: radix-2 DIT stages s = 0 .. logL-1 (stage s pairs elements
-    // h = 2^s apart with twiddle w_(2h)^i = shtw[i << (logL-1-s)]), fused two
+    // h = 2^s apart with twiddle w_(2h)^i = w_L^(i << (logL-1-s))), fused two
     // at a time: a group of 4h elements holds, for each i < h, the four
     // elements e, e+h, e+2h, e+3h that stages s and s+1 combine among
     // themselves, so they stay in registers across both stages and LDS is
@@ -122,14 +124,15 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
             Fe<F> x2 = fe_load_words9<F>(sh + (e + 2 * h) * F::N);
             Fe<F> x3 = fe_load_words9<F>(sh + (e + 3 * h) * F::N);
             if (s != 0) {  // stage s: w_(2h)^i (= 1 at s = 0)
-                const Fe<F> w = fe_load_words9<F>(shtw + (i << sh1) * F::N);
+                const Fe<F> w = fe_load_words9<F>(tw + ((size_t)(i << sh1) << tws) * F::N);
                 x1 = fe_mul<F>(x1, w);
                 x3 = fe_mul<F>(x3, w);
             }
             const Fe<F> a0 = fe_add<F>(x0, x1), a1 = fe_sub<F>(x0, x1);
             // stage s+1: pairs (e, e+2h) with w_(4h)^i, (e+h, e+3h) with w_(4h)^(i+h)
-            const Fe<F> a2 = fe_mul<F>(fe_add<F>(x2, x3), fe_load_words9<F>(shtw + (i << sh2) * F::N));
-            const Fe<F> a3 = fe_mul<F>(fe_sub<F>(x2, x3), fe_load_words9<F>(shtw + ((i + h) << sh2) * F::N));
+            const Fe<F> a2 = fe_mul<F>(fe_add<F>(x2, x3), fe_load_words9<F>(tw + ((size_t)(i << sh2) << tws) * F::N));
+            const Fe<F> a3 =
+                fe_mul<F>(fe_sub<F>(x2, x3), fe_load_words9<F>(tw + ((size_t)((i + h) << sh2) << tws) * F::N));
             fe_store_words9<F>(sh + e * F::N, fe_add<F>(a0, a2));
             fe_store_words9<F>(sh + (e + 2 * h) * F::N, fe_sub<F>(a0, a2));
             fe_store_words9<F>(sh + (e + h) * F::N, fe_add<F>(a1, a3));
