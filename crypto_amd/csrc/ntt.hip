@@ -72,7 +72,13 @@ k_ntt_pass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const ui
     in += xform;
     out += xform;
     const uint32_t L = 1u << d.logL, tile = d.T * L;
-    const uint32_t q_start = blockIdx.x * d.T;
+    // Workgroup ids go round-robin over the 8 XCDs (each with its own L2).
+    // Hand each XCD a contiguous range of tiles instead, so the neighbouring
+    // columns of a strided pass -- which share 128-byte lines -- are read
+    // through one L2.
+    const uint32_t ng = gridDim.x, bid = blockIdx.x;
+    const uint32_t tile_id = (ng % 8 == 0) ? (bid % 8) * (ng / 8) + bid / 8 : bid;
+    const uint32_t q_start = tile_id * d.T;
     const uint32_t tid = threadIdx.x;
 
     // twiddles w_L^k = w_n^(k << root_shift) come straight from the resident
