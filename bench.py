@@ -47,8 +47,9 @@ def parse_args():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-versus-plain job set")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="open-loop serving run length (0 = skip)")
     ap.add_argument("--profile-run", action="store_true",
-                    help="for rocprofv3 counter runs: 2 MSMs of the workload plus one 1 GiB device copy "
-                         "(the HBM-counter calibration kernel), no JSON line")
+                    help="for rocprofv3 counter runs: 2 MSMs of the workload, one forward Fr NTT per field "
+                         "at 2^20 and 2^22, and one 1 GiB device copy (the HBM-counter calibration kernel); "
+                         "no JSON line")
     return ap.parse_args()
 
 
@@ -548,13 +549,23 @@ def main():
 
     if args.profile_run:
         timed_msm(lib, torch, D, curve, n, 20261203, 2, 0, args.window_bits, profile=False)
+        # one forward Fr NTT per field at 2^20 and 2^22 (canonical random inputs)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(11)
+        for field in (0, 1):
+            for logn in (20, 22):
+                m = 1 << logn
+                d_in = torch.randint(0, 256, (32 * m,), dtype=torch.uint8, device="cuda", generator=gen)
+                d_in.view(-1, 32)[:, 31] &= 0x1F
+                D.ntt_batch(field, d_in, m, 1)
+                del d_in
         # calibration: a plain 1 GiB device copy of initialised data (1 GiB read + 1 GiB written)
         src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        print("profile run done: 2 MSMs + 1 GiB copy", flush=True)
+        print("profile run done: 2 MSMs + 4 NTTs + 1 GiB copy", flush=True)
         return
 
     ms, stage, result, check, ext = timed_msm(lib, torch, D, curve, n, 20261203, args.steps, args.warmup,
