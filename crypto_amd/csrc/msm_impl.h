@@ -122,19 +122,22 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
 // bucket is large -- the top window's few buckets, GLV's carry window, or the
 // single shared bucket set of a window table (about rows * n / 2^(c-1)
 // entries per bucket).  So:
-//   k_carry_runs   one thread per bucket reads its run [head, head + len) off
+//   k_carry_runs   one quad per bucket reads its run [head, head + len) off
 //                  the offsets; runs of at most kShortRun carries are folded
-//                  serially by that thread; a longer run is recorded as
-//                  {head, len} (and the longest length is kept);
+//                  serially by that quad (quad-cooperative additions); a
+//                  longer run is recorded as {head, len} (and the longest
+//                  length is kept);
 //   k_list_runs    a workgroup per recorded run reserves len slots of a
 //                  compact list (one atomic) and its threads write {chunk j,
 //                  position r in the run, len} for the run's chunks (one run
 //                  can hold most of a window's chunks);
-//   k_fold_level   pairwise tree over the listed chunks: level L adds
-//                  carry[j + 2^L] into carry[j] for run positions
-//                  r = 0 mod 2^(L+1); levels past log2(longest run) return at
-//                  once.  Grid-stride over the list, so a launch is small
-//                  whatever the number of chunks.
+//   k_fold_level   pairwise tree over the listed chunks, one quad per list
+//                  entry: level L adds carry[j + 2^L] into carry[j] for run
+//                  positions r = 0 mod 2^(L+1); levels past log2(longest run)
+//                  return at once.  Grid-stride over the list, so a launch is
+//                  small whatever the number of chunks.  A level's cost is one
+//                  point addition's latency, so the additions are the
+//                  4-lane ones.
 // ctl[0] = list length, ctl[1] = longest listed run, ctl[2] = recorded runs
 // (all zeroed before).
 // (Templated only so that each per-curve translation unit owns its copy.)
@@ -145,8 +148,8 @@ template <class F>
 __global__ void __launch_bounds__(256)
 k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint2* __restrict__ runs,
              uint32_t* __restrict__ ctl, uint32_t* __restrict__ carry) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nb) return;
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    if (g >= nb) return;  // whole quads leave together
     const uint32_t lo = offsets[g], hi = offsets[g + 1];
     if (hi <= lo) return;  // empty bucket
     const uint32_t head = lo / chunk + 1, len = (hi - 1) / chunk + 1 - head;
@@ -154,12 +157,14 @@ k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
     if (len <= kShortRun) {
         Xyzz<F> acc = xyzz_load<F>(carry, head);
 #pragma unroll 1
-        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add<F, MulCall>(acc, xyzz_load<F>(carry, head + k));
-        xyzz_store<F>(carry, head, acc);
+        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add_quad<F>(acc, xyzz_load<F>(carry, head + k));
+        if (quad_lane() == 0) xyzz_store<F>(carry, head, acc);
         return;
     }
-    runs[atomicAdd(&ctl[2], 1u)] = make_uint2(head, len);
-    atomicMax(&ctl[1], len);
+    if (quad_lane() == 0) {
+        runs[atomicAdd(&ctl[2], 1u)] = make_uint2(head, len);
+        atomicMax(&ctl[1], len);
+    }
 }
 
 template <class F>
@@ -184,12 +189,14 @@ k_fold_level(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, u
     const uint32_t s = 1u << level;
     if (s >= ctl[1]) return;
     const uint32_t count = ctl[0];
+    const uint32_t q0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 2, nq = (gridDim.x * blockDim.x) >> 2;
 #pragma unroll 1
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = q0; i < count; i += nq) {  // the four lanes of a quad take the same entries
         const uint4 e = list[i];
         const uint32_t j = e.x, r = e.y, len = e.z;
         if ((r & (2 * s - 1)) != 0 || r + s >= len) continue;
-        xyzz_store<F>(carry, j, xyzz_add<F, MulCall>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s)));
+        const Xyzz<F> sum = xyzz_add_quad<F>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s));
+        if (quad_lane() == 0) xyzz_store<F>(carry, j, sum);
     }
 }
 

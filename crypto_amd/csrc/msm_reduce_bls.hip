@@ -10,10 +10,11 @@ void launch_fold_bls(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32
     device_fill_u32(ctl, 0, 3, s);
     uint4* l4 = reinterpret_cast<uint4*>(list);
     uint2* r2 = reinterpret_cast<uint2*>(runs);
-    hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div(nb, 256)), dim3(256), 0, s, offsets, nb, chunk, r2, ctl,
-                       carry);
+    hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div((size_t)nb * 4, 256)), dim3(256), 0, s, offsets, nb,
+                       chunk, r2, ctl, carry);
     hipLaunchKernelGGL(k_list_runs<CurveBLS12381::Fp>, dim3(256), dim3(256), 0, s, r2, l4, ctl);
-    const unsigned grid = ceil_div(nchunks < kFoldThreads ? nchunks : kFoldThreads, 64);
+    // one quad per list entry, at most kFoldThreads quads
+    const unsigned grid = ceil_div(4 * (size_t)(nchunks < kFoldThreads ? nchunks : kFoldThreads), 64);
     for (uint32_t level = 0; level < levels; ++level)
         hipLaunchKernelGGL(k_fold_level<CurveBLS12381::Fp>, dim3(grid), dim3(64), 0, s, l4, ctl, level, carry);
 }
