@@ -15,6 +15,7 @@ struct MsmWorkspace {
     DevBuf lvA[2], lvB[2];  // reduction levels (ping-pong)
     DevBuf sort_cnt, sort_pre;  // LDS sort: per-(window, part) histograms
     DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 128-bit scalars
+    DevBuf sort_dig;                  // batch LDS sort: 16-bit digit codes
     DevBuf out;
 };
 
@@ -43,6 +44,19 @@ struct MsmJob {
     uint32_t gb_base;   // first global bucket
     uint32_t blk_base;  // first block of the digit kernels
     uint32_t pad;
+};
+
+// LDS-histogram sort of a batch: per job, the signed digits are written once
+// (16-bit codes, window-major: digit w of point i at dig_base + w n + i), then
+// one workgroup per (job, digit window, part of the points) counts them in an
+// LDS histogram and scatters them through LDS cursors.
+struct SortJob {
+    uint64_t dig_base;   // first digit code of the job
+    uint64_t cnt_base;   // first histogram word: item (w, p) at cnt_base + (w parts + p) B
+    uint32_t item_base;  // first workgroup of the job
+    uint32_t parts;      // parts per digit window
+    uint32_t per_part;   // points per part
+    uint32_t wd;         // digit windows
 };
 
 // Inputs per workgroup (one part) of a bucket-reduction level.
@@ -109,7 +123,12 @@ struct MsmBatchPlan {
     // (nwin entries), launched on level_grid[l] workgroups
     std::vector<uint32_t> level_off, level_grid;
     uint32_t level_max_out = 0;  // largest output count of a non-final level
-    DevBuf d_jobs, d_levels;
+    // LDS-histogram sort (every job's c <= 16): workgroups, histogram words,
+    // digit codes, largest 2^(c-1)
+    bool lds_sort = false;
+    uint32_t sort_items = 0, max_b = 0;
+    size_t sort_words = 0, sort_digits = 0;
+    DevBuf d_jobs, d_levels, d_sjobs;
 };
 
 // Tuning modes: one MSM, a job of a batch, one MSM over G1 points through

@@ -358,4 +358,195 @@ void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8
     }
 }
 
+// ------------------------------------------------------------ batch LDS sort
+// The global-atomic passes of k_digits_multi make the sort the slowest stage
+// of a batch of small and mid-size MSMs (hundreds of millions of digits, all
+// through memory-side atomics).  The batch form of the LDS sort:
+//
+//   k_batch_digits   one thread per point writes its signed digits as 16-bit
+//                    codes, window-major per job (coalesced)
+//   k_batch_count    one workgroup per (job, digit window w, part p): LDS
+//                    histogram of 2^(c-1) words -> cnt[item]
+//   k_batch_colscan  per global bucket: prefix over the job's parts -> pre,
+//                    and the bucket's count -> tot
+//   scan             offsets
+//   k_batch_scatter  per item: LDS cursor[b] = offsets[gb] + pre[item][b];
+//                    every digit takes a slot by an LDS atomic
+//
+// Digit code: d > 0 -> d - 1; d < 0 -> 0x8000 | (|d| - 1); d = 0 -> 0xFFFF.
+// (|d| <= 2^(c-1) <= 2^15 for positive digits and < 2^(c-1) for negative
+// ones, so the three ranges do not meet.)
+constexpr uint16_t kNoDigit = 0xFFFF;
+
+__device__ __forceinline__ uint32_t find_job_by_block(const MsmJob* __restrict__ jobs, uint32_t njobs, uint32_t b) {
+    uint32_t lo = 0, hi = njobs;  // invariant: jobs[lo].blk_base <= b
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].blk_base <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_digits(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf, const MsmJob* __restrict__ jobs,
+               const SortJob* __restrict__ sjobs, uint32_t njobs, uint16_t* __restrict__ dig) {
+    __shared__ uint32_t sj;
+    if (threadIdx.x == 0) sj = find_job_by_block(jobs, njobs, blockIdx.x);
+    __syncthreads();
+    const MsmJob jd = jobs[sj];
+    const SortJob so = sjobs[sj];
+    const uint32_t i = (blockIdx.x - jd.blk_base) * blockDim.x + threadIdx.x;
+    if (i >= jd.n) return;
+    uint16_t* out = dig + so.dig_base + i;
+    const size_t stride = jd.n;
+    if (inf && inf[jd.pt_base + i]) {
+        for (uint32_t w = 0; w < so.wd; ++w) out[w * stride] = kNoDigit;
+        return;
+    }
+    uint32_t s[8];
+    load_scalar(scalars + jd.sc_base * 8, i, s);
+    const uint32_t c = jd.c, B = 1u << (c - 1), mask = (1u << c) - 1u;
+    uint32_t w = 0, carry = 0;
+    auto emit = [&](uint32_t v) {
+        int32_t d = (int32_t)(v + carry);
+        carry = 0;
+        if (d > (int32_t)B) {
+            d -= (int32_t)(1u << c);
+            carry = 1;
+        }
+        const uint16_t code = d > 0 ? (uint16_t)(d - 1) : d < 0 ? (uint16_t)(0x8000u | (uint32_t)(-d - 1)) : kNoDigit;
+        out[w * stride] = code;
+        ++w;
+    };
+    uint64_t buf = 0;
+    uint32_t nbits = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        buf |= (uint64_t)s[k] << nbits;
+        nbits += 32;
+        while (nbits >= c) {
+            emit((uint32_t)buf & mask);
+            buf >>= c;
+            nbits -= c;
+        }
+    }
+    while (w < so.wd) {  // the remaining < c bits, then the carry window
+        emit((uint32_t)buf & mask);
+        buf >>= c;
+    }
+}
+
+// Job of sort item `item` (workgroup), read by thread 0 into LDS.
+__device__ __forceinline__ uint32_t find_job_by_item(const SortJob* __restrict__ sjobs, uint32_t njobs, uint32_t item) {
+    uint32_t lo = 0, hi = njobs;  // invariant: sjobs[lo].item_base <= item
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sjobs[mid].item_base <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_batch_count(const uint16_t* __restrict__ dig, const MsmJob* __restrict__ jobs, const SortJob* __restrict__ sjobs,
+              uint32_t njobs, uint32_t* __restrict__ cnt) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t sj;
+    if (threadIdx.x == 0) sj = find_job_by_item(sjobs, njobs, blockIdx.x);
+    __syncthreads();
+    const MsmJob jd = jobs[sj];
+    const SortJob so = sjobs[sj];
+    const uint32_t item = blockIdx.x - so.item_base, w = item / so.parts, p = item - w * so.parts;
+    const uint32_t B = 1u << (jd.c - 1);
+    for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) hist[b] = 0;
+    __syncthreads();
+    const uint16_t* d = dig + so.dig_base + (size_t)w * jd.n;
+    const uint32_t lo = p * so.per_part, hi = min(jd.n, lo + so.per_part);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
+        const uint16_t code = d[i];
+        if (code != kNoDigit) atomicAdd(&hist[code & 0x7FFFu], 1u);
+    }
+    __syncthreads();
+    uint32_t* out = cnt + so.cnt_base + (size_t)item * B;
+    for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) out[b] = hist[b];
+}
+
+// One thread per global bucket gb = gb_base + w B + b of job j.
+__global__ void __launch_bounds__(256)
+k_batch_colscan(const uint32_t* __restrict__ cnt, const MsmJob* __restrict__ jobs, const SortJob* __restrict__ sjobs,
+                uint32_t njobs, uint32_t nb, uint32_t* __restrict__ pre, uint32_t* __restrict__ tot) {
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gb >= nb) return;
+    uint32_t lo = 0, hi = njobs;  // invariant: jobs[lo].gb_base <= gb
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].gb_base <= gb) lo = mid;
+        else hi = mid;
+    }
+    const MsmJob jd = jobs[lo];
+    const SortJob so = sjobs[lo];
+    const uint32_t B = 1u << (jd.c - 1), r = gb - jd.gb_base, w = r >> (jd.c - 1), b = r & (B - 1);
+    uint32_t run = 0;
+    for (uint32_t p = 0; p < so.parts; ++p) {
+        const size_t idx = so.cnt_base + ((size_t)w * so.parts + p) * B + b;
+        const uint32_t c = cnt[idx];
+        pre[idx] = run;
+        run += c;
+    }
+    tot[gb] = run;
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_batch_scatter(const uint16_t* __restrict__ dig, const MsmJob* __restrict__ jobs, const SortJob* __restrict__ sjobs,
+                uint32_t njobs, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pre,
+                uint32_t* __restrict__ entries) {
+    extern __shared__ uint32_t cur[];
+    __shared__ uint32_t sj;
+    if (threadIdx.x == 0) sj = find_job_by_item(sjobs, njobs, blockIdx.x);
+    __syncthreads();
+    const MsmJob jd = jobs[sj];
+    const SortJob so = sjobs[sj];
+    const uint32_t item = blockIdx.x - so.item_base, w = item / so.parts, p = item - w * so.parts;
+    const uint32_t B = 1u << (jd.c - 1);
+    const uint32_t* pp = pre + so.cnt_base + (size_t)item * B;
+    const uint32_t* wo = offsets + jd.gb_base + w * B;
+    for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) cur[b] = wo[b] + pp[b];
+    __syncthreads();
+    const uint16_t* d = dig + so.dig_base + (size_t)w * jd.n;
+    const uint32_t lo = p * so.per_part, hi = min(jd.n, lo + so.per_part);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
+        const uint16_t code = d[i];
+        if (code != kNoDigit) {
+            const uint32_t pos = atomicAdd(&cur[code & 0x7FFFu], 1u);
+            entries[pos] = (jd.pt_base + i) | ((uint32_t)(code & 0x8000u) << 16);
+        }
+    }
+}
+
+bool batch_lds_sort_prepare(size_t dyn_lds) {
+    if (dyn_lds > 160 * 1024) return false;
+    if (dyn_lds <= 64 * 1024) return true;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_count),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) == hipSuccess &&
+           hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_scatter),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) == hipSuccess;
+}
+
+void batch_lds_sort_dispatch(const uint32_t* d_scalars, const uint8_t* d_inf, const MsmJob* d_jobs,
+                             const SortJob* d_sjobs, uint32_t njobs, uint32_t dig_blocks, uint32_t items, uint32_t nb,
+                             size_t dyn_lds, uint16_t* dig, uint32_t* cnt, uint32_t* pre, uint32_t* tot,
+                             uint32_t* offsets, uint32_t* block_sums, uint32_t* total, uint32_t* entries,
+                             hipStream_t s) {
+    if (dig_blocks == 0 || items == 0) return;
+    hipLaunchKernelGGL(k_batch_digits, dim3(dig_blocks), dim3(256), 0, s, d_scalars, d_inf, d_jobs, d_sjobs, njobs,
+                       dig);
+    hipLaunchKernelGGL(k_batch_count, dim3(items), dim3(kSortThreads), dyn_lds, s, dig, d_jobs, d_sjobs, njobs, cnt);
+    hipLaunchKernelGGL(k_batch_colscan, dim3(ceil_div(nb, 256)), dim3(256), 0, s, cnt, d_jobs, d_sjobs, njobs, nb,
+                       pre, tot);
+    device_exclusive_scan(tot, offsets, nullptr, nb, block_sums, total, s);
+    hipLaunchKernelGGL(k_batch_scatter, dim3(items), dim3(kSortThreads), dyn_lds, s, dig, d_jobs, d_sjobs, njobs,
+                       offsets, pre, entries);
+}
+
 }  // namespace kz

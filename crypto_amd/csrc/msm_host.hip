@@ -493,6 +493,42 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     plan.nseg = (uint32_t)nseg;
     plan.dig_blocks = (uint32_t)blocks;
     plan.max_entries = entries;
+    // LDS-histogram sort when every job's histogram fits the LDS: parts of
+    // >= 8192 points (at most 64 per window), one workgroup per (job, digit
+    // window, part)
+    {
+        std::vector<SortJob> sj(plan.jobs.size());
+        uint64_t dig = 0, cnt = 0, items = 0;
+        uint32_t max_b = 0;
+        bool fits = true;
+        for (size_t k = 0; k < plan.jobs.size(); ++k) {
+            const MsmJob& j = plan.jobs[k];
+            const uint32_t wd = (uint32_t)msm_num_windows((int)j.c), B = 1u << (j.c - 1);
+            uint32_t parts = 1;
+            while (parts < 64 && (uint64_t)parts * 2 * 8192 <= j.n) parts *= 2;
+            SortJob& so = sj[k];
+            so.dig_base = dig;
+            so.cnt_base = cnt;
+            so.item_base = (uint32_t)items;
+            so.parts = parts;
+            so.per_part = (j.n + parts - 1) / parts;
+            so.wd = wd;
+            dig += (uint64_t)wd * j.n;
+            cnt += (uint64_t)wd * parts * B;
+            items += (uint64_t)wd * parts;
+            if (B > max_b) max_b = B;
+            fits = fits && j.c <= 16;
+        }
+        plan.lds_sort = fits && items < 0x80000000ull && batch_lds_sort_prepare((size_t)max_b * 4);
+        if (plan.lds_sort) {
+            plan.sort_items = (uint32_t)items;
+            plan.max_b = max_b;
+            plan.sort_words = cnt;
+            plan.sort_digits = dig;
+            plan.d_sjobs.ensure(sj.size() * sizeof(SortJob));
+            KZ_HIP(hipMemcpy(plan.d_sjobs.ptr, sj.data(), sj.size() * sizeof(SortJob), hipMemcpyHostToDevice));
+        }
+    }
     plan.levels = fold_levels(max_n, plan.chunk);
     // reduction levels until every window is down to one part; the last level
     // writes window w's sum at index w
@@ -529,15 +565,28 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
 
 void msm_batch_reserve(const MsmBatchPlan& plan, MsmWorkspace& ws) {
     reserve(stages_of(plan.curve), batch_shape(plan), ws);
+    if (plan.lds_sort) {
+        ws.sort_cnt.ensure(plan.sort_words * sizeof(uint32_t));
+        ws.sort_pre.ensure(plan.sort_words * sizeof(uint32_t));
+        ws.sort_dig.ensure(plan.sort_digits * sizeof(uint16_t));
+    }
 }
 
 void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_t* d_inf,
                    const uint32_t* d_scalars, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(plan.curve);
     const Shape sh = batch_shape(plan);
-    reserve(st, sh, ws);
+    msm_batch_reserve(plan, ws);
     const MsmJob* d_jobs = plan.d_jobs.as<MsmJob>();
     auto sort = [&] {
+        if (plan.lds_sort) {
+            batch_lds_sort_dispatch(d_scalars, d_inf, d_jobs, plan.d_sjobs.as<SortJob>(), plan.njobs, plan.dig_blocks,
+                                    plan.sort_items, plan.nb, (size_t)plan.max_b * 4, ws.sort_dig.as<uint16_t>(),
+                                    ws.sort_cnt.as<uint32_t>(), ws.sort_pre.as<uint32_t>(), ws.counts.as<uint32_t>(),
+                                    ws.offsets.as<uint32_t>(), ws.block_sums.as<uint32_t>(), ws.total.as<uint32_t>(),
+                                    ws.entries.as<uint32_t>(), s);
+            return;
+        }
         atomic_sort(sh, [&](bool scatter, uint32_t* cc, uint32_t* entries) {
             digits_multi_dispatch(scatter, d_scalars, d_inf, d_jobs, plan.njobs, plan.dig_blocks, cc, entries, s);
         }, ws, s);

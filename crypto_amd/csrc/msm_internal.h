@@ -33,4 +33,15 @@ void digits_multi_dispatch(bool scatter, const uint32_t* d_scalars, const uint8_
                            uint32_t njobs, uint32_t nblocks, uint32_t* counts_or_cursor, uint32_t* entries,
                            hipStream_t s);
 
+// Batch LDS-histogram sort (plan.lds_sort): digit codes, per-item counts,
+// column scan (tot = the bucket counts, nb words), scan into offsets, scatter.
+// dyn_lds = largest 2^(c-1) words of the batch, in bytes.
+void batch_lds_sort_dispatch(const uint32_t* d_scalars, const uint8_t* d_inf, const MsmJob* d_jobs,
+                             const SortJob* d_sjobs, uint32_t njobs, uint32_t dig_blocks, uint32_t items, uint32_t nb,
+                             size_t dyn_lds, uint16_t* dig, uint32_t* cnt, uint32_t* pre, uint32_t* tot,
+                             uint32_t* offsets, uint32_t* block_sums, uint32_t* total, uint32_t* entries, hipStream_t s);
+// Allows the batch sort kernels dyn_lds bytes of dynamic LDS; false if the
+// runtime refuses (the batch then sorts with global atomics).
+bool batch_lds_sort_prepare(size_t dyn_lds);
+
 }  // namespace kz

@@ -46,8 +46,13 @@ def _scalar_block(rnd, n, wide):
 def test_msm_batch_matches_closed_form(D, curve, window_bits):
     k0, d = 0xC0FFEE, 0x1357913579
     npts = 5000
-    points = _progression(curve, k0, d, npts)
+    points = bytearray(_progression(curve, k0, d, npts))
     ks = [(k0 + i * d) for i in range(npts)]
+    pb = O.POINT_BYTES[curve]
+    for i in (7, 99, 1500, 2500, 4999):  # identity points (all-zero wire bytes)
+        points[i * pb:(i + 1) * pb] = bytes(pb)
+        ks[i] = 0
+    points = bytes(points)
     sizes = [1, 0, 2, 3, 64, 100, 777, 1024, 3000, 5000, 17]
     rnd = random.Random(curve * 100 + window_bits)
     blocks = [_scalar_block(rnd, n, wide=(j % 2 == 0)) for j, n in enumerate(sizes)]
@@ -56,7 +61,6 @@ def test_msm_batch_matches_closed_form(D, curve, window_bits):
     b = D.MsmBatch(curve, sizes, window_bits=window_bits)
     print(f"msm batch curve={curve} c={window_bits} jobs={len(sizes)}", flush=True)
     out = b.run(pp, D.to_device(scalars)).cpu().numpy().tobytes()
-    pb = O.POINT_BYTES[curve]
     rec = pb + pb // 2
     for j, n in enumerate(sizes):
         want = _closed(curve, blocks[j], ks[:n]) if n else bytes(pb)
