@@ -238,6 +238,8 @@ Choice tune_batched(Engine& e, int curve, int lg, const MsmInputs& in, size_t jo
         if (!device_equal(d_ref, d_out, ob * jobs, d_flag, s))
             throw Mismatch("batched MSM window bits " + std::to_string(c) +
                            " differ from the built-in schedule at 2^" + std::to_string(lg));
+        if (std::getenv("KINET_HIP_TUNE_VERBOSE"))
+            std::fprintf(stderr, "msm curve=%d batched 2^%d x %zu jobs c=%d %.4f ms\n", curve, lg, jobs, c, ms);
         if (best.c == 0 || ms < best.ms) best = Choice{c, 0, ms};
     }
     KZ_HIP(hipStreamSynchronize(s));  // ws is freed on return
