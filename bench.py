@@ -349,14 +349,17 @@ def batched_set(lib, torch, D, reps: int, seed: int = 4242):
     }
 
 
-def serving_run(lib, torch, seconds: float, seed: int = 77):
+def serving_run(lib, torch, seconds: float, seed: int = 77, tables: bool = False):
     """Commitment serving: 70% BLS12-381 requests with n = 4096, 30% BN254 with
     n = 2^K, K uniform on {14..20}; evaluations uniform canonical Fr.
     Capacity is measured closed-loop (a burst of requests, all submitted at
     once) for the server and for plain one-at-a-time kzg_commit on the same
     request sample; then an open-loop Poisson run at 75% of the server's
     capacity gives commits/s and latency percentiles (submit -> completion,
-    timed by the server)."""
+    timed by the server).  tables: the SRSs first get resident window tables
+    (kzg_srs_precompute) for the mix's window widths -- BLS12-381 c = 8 (its
+    4096-point commits), BN254 c = 9, 12, 15 (2^14 .. 2^20) -- used by the
+    server's batches and by the plain commits alike."""
     import numpy as np
 
     import crypto_amd
@@ -364,6 +367,12 @@ def serving_run(lib, torch, seconds: float, seed: int = 77):
 
     tau = 0x5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED5EED
     srs = {2: crypto_amd.KzgSrs.generate(2, tau, 4096), 1: crypto_amd.KzgSrs.generate(1, tau, 1 << 20)}
+    table_bytes = 0
+    if tables:
+        table_bytes += srs[2].precompute(8)
+        for c in (9, 12, 15):
+            srs[1].precompute(c)
+        table_bytes += srs[1].precompute(15)
     pool = {2: inputs.scalars(2, seed, 4096).view(np.uint8).reshape(-1).copy(),
             1: inputs.scalars(1, seed + 1, 1 << 20).view(np.uint8).reshape(-1).copy()}
 
@@ -418,6 +427,8 @@ def serving_run(lib, torch, seconds: float, seed: int = 77):
                        "bls12_381_n4096": {"p50": pct(lat_bls, 50), "p99": pct(lat_bls, 99)},
                        "bn254": {"p50": pct(lat_bn, 50), "p99": pct(lat_bn, 99)}},
         "alone_vs_mix_bytes_identical": identical,
+        "srs_window_tables": ({"bls12_381": [8], "bn254": [9, 12, 15], "gb": round(table_bytes / 1e9, 2)}
+                              if tables else None),
         "server_stats": st,
     }
 
@@ -634,6 +645,7 @@ def main():
         line["batched"] = batched_set(lib, torch, D, reps=3)
     if args.serve_seconds > 0:
         line["serving"] = serving_run(lib, torch, args.serve_seconds)
+        line["serving_with_srs_tables"] = serving_run(lib, torch, args.serve_seconds, tables=True)
     if not args.no_extras:
         extras = {}
         pms, _, pres, pcheck, _ = timed_msm(lib, torch, D, curve, n, 20261203, max(3, args.steps // 2), 2,

@@ -1,5 +1,6 @@
 // KZG commitments C = MSM(SRS, iNTT(evals)) on the device, and the SRS /
 // context handles of include/kinet/crypto/kzg.h.
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -68,13 +69,27 @@ static void srs_upload(kzg_srs* srs, const uint8_t* d_wire_src, bool src_on_host
                      s);
 }
 
+int srs_table_near(const kzg_srs* srs, int c, int max_gap) {
+    int best = -1;
+    for (size_t k = 0; k < srs->table_c.size(); ++k) {
+        const int gap = std::abs(srs->table_c[k] - c);
+        if (gap > max_gap) continue;
+        if (best < 0 || gap < std::abs(srs->table_c[best] - c) ||
+            (gap == std::abs(srs->table_c[best] - c) && srs->table_c[k] > srs->table_c[best]))
+            best = (int)k;
+    }
+    return best;
+}
+
 void srs_msm(const kzg_srs* srs, const uint32_t* d_scalars, size_t n, int window_bits, MsmWorkspace& ws,
              hipStream_t s, uint8_t* d_out) {
     MsmParams prm;
-    if (srs->table_c && (window_bits == 0 || window_bits == srs->table_c)) {
-        prm.c = srs->table_c;
+    const int k = srs_table_near(srs, window_bits ? window_bits : msm_table_window_bits(n), window_bits ? 0 : 64);
+    if (k >= 0) {
+        prm.c = srs->table_c[k];
         prm.table_stride = (uint32_t)srs->n;
-        msm_device(srs->curve, srs->table.as<uint32_t>(), srs->inf.as<uint8_t>(), d_scalars, n, prm, ws, s, d_out);
+        const uint32_t* tab = srs->tables.as<uint32_t>() + (size_t)srs->table_base[k] * affine_words(srs->curve);
+        msm_device(srs->curve, tab, srs->inf.as<uint8_t>(), d_scalars, n, prm, ws, s, d_out);
         return;
     }
     prm.g1 = srs->g1;
@@ -187,13 +202,28 @@ int kzg_srs_precompute(kzg_srs* srs, int window_bits) {
         Engine& e = engine();
         std::lock_guard<std::mutex> g(e.mu);
         const int c = window_bits ? window_bits : msm_table_window_bits(srs->n);
-        if ((uint64_t)msm_table_rows(c) * srs->n >= 0x80000000ull) return KZ_CRYPTO_ERR_INPUT;
-        srs->table_c = 0;  // unusable until rebuilt
-        srs->table.ensure((size_t)msm_table_rows(c) * srs->n * affine_words(srs->curve) * sizeof(uint32_t));
-        points_table_build(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), srs->n, c,
-                           srs->table.as<uint32_t>(), e.stream);
+        for (int have : srs->table_c)
+            if (have == c) return KZ_CRYPTO_OK;
+        std::vector<int> cs = srs->table_c;
+        cs.push_back(c);
+        uint64_t records = 0;
+        std::vector<uint32_t> base;
+        for (int k : cs) {
+            base.push_back((uint32_t)records);
+            records += (uint64_t)msm_table_rows(k) * srs->n;
+        }
+        if (records >= 0x80000000ull) return KZ_CRYPTO_ERR_INPUT;  // entry indices are 31-bit
+        // rebuild every table into one buffer (the old one is released first)
+        srs->table_c.clear();
+        srs->table_base.clear();
+        const size_t rec = (size_t)affine_words(srs->curve);
+        srs->tables.ensure(records * rec * sizeof(uint32_t));
+        for (size_t k = 0; k < cs.size(); ++k)
+            points_table_build(srs->curve, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), srs->n, cs[k],
+                               srs->tables.as<uint32_t>() + (size_t)base[k] * rec, e.stream);
         KZ_HIP(hipStreamSynchronize(e.stream));
-        srs->table_c = c;
+        srs->table_c = cs;
+        srs->table_base = base;
         return KZ_CRYPTO_OK;
     } catch (const std::exception& ex) {
         set_last_error(ex.what());
@@ -202,7 +232,10 @@ int kzg_srs_precompute(kzg_srs* srs, int window_bits) {
 }
 
 size_t kzg_srs_table_bytes(const kzg_srs* srs) {
-    return srs && srs->table_c ? (size_t)msm_table_rows(srs->table_c) * srs->n * affine_words(srs->curve) * 4 : 0;
+    if (!srs) return 0;
+    size_t records = 0;
+    for (int c : srs->table_c) records += (size_t)msm_table_rows(c) * srs->n;
+    return records * affine_words(srs->curve) * sizeof(uint32_t);
 }
 
 uint32_t kzg_srs_curve(const kzg_srs* srs) { return srs ? (uint32_t)srs->curve : 0; }

@@ -53,7 +53,13 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
         off += (size_t)n[i] * (j - i);
         i = j;
     }
-    auto key = std::make_pair(curve, n);
+    // Plans are cached per (curve, SRS size and window tables, job sizes).
+    std::vector<uint32_t> kv;
+    kv.push_back((uint32_t)srs->n);
+    kv.push_back((uint32_t)srs->table_c.size());
+    for (int c : srs->table_c) kv.push_back((uint32_t)c);
+    kv.insert(kv.end(), n.begin(), n.end());
+    auto key = std::make_pair(curve, kv);
     auto it = lane.plans.find(key);
     if (it == lane.plans.end()) {
         if (lane.plans.size() >= 256) lane.plans.clear();
@@ -64,12 +70,28 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
             specs[i].n = n[i];
             specs[i].sc_base = sc;
             sc += n[i];
+            if (!srs->table_c.empty()) {
+                // every job indexes the tables buffer: a table within 2 bits
+                // of the job's own window width, else ordinary windows over
+                // row 0 (the points) of the first table
+                const int want = msm_window_bits(curve, n[i], kMsmBatched);
+                const int k = srs_table_near(srs, want, 2);
+                if (k >= 0) {
+                    specs[i].c = (uint32_t)srs->table_c[k];
+                    specs[i].table = (uint32_t)srs->n;
+                    specs[i].tab_base = srs->table_base[k];
+                } else {
+                    specs[i].c = (uint32_t)want;
+                    specs[i].tab_base = srs->table_base[0];
+                }
+            }
         }
         auto plan = std::make_unique<MsmBatchPlan>();
         msm_batch_plan(curve, specs.data(), count, 0, *plan);
         it = lane.plans.emplace(key, std::move(plan)).first;
     }
-    msm_batch_run(*it->second, srs->pts.as<uint32_t>(), srs->inf.as<uint8_t>(), co, lane.ws, s, out);
+    const uint32_t* pts = srs->table_c.empty() ? srs->pts.as<uint32_t>() : srs->tables.as<uint32_t>();
+    msm_batch_run(*it->second, pts, srs->inf.as<uint8_t>(), co, lane.ws, s, out);
     KZ_HIP(hipMemcpyAsync(lane.pin_out.ensure(commit_batch_out_bytes(curve, count)), out, count * rec + sizeof(int),
                           hipMemcpyDeviceToHost, s));
     KZ_HIP(hipEventRecord(lane.done, s));

@@ -17,8 +17,13 @@ struct kzg_srs {
     kz::DevBuf inf;   // identity flags
     kz::DevBuf wire;  // wire points (kept for export)
     bool g1 = false;  // every point lies in G1 (generated, or checked at creation)
-    kz::DevBuf table; // optional window table of the points (kzg_srs_precompute)
-    int table_c = 0;  // its window bits, 0 = none
+    // Optional window tables of the points (kzg_srs_precompute), one after
+    // another in `tables` (records of affine_words each): table k has window
+    // bits table_c[k] and starts at record table_base[k]; its row 0 is the
+    // points themselves.
+    kz::DevBuf tables;
+    std::vector<int> table_c;
+    std::vector<uint32_t> table_base;
 };
 
 struct kzg_ctx {
@@ -29,11 +34,16 @@ struct kzg_ctx {
 
 namespace kz {
 
-// MSM of the first n SRS points with device scalars: through the SRS's window
-// table when it has one (and window_bits is 0 or the table's), else the
-// prepared points (GLV when they are known to lie in G1).
+// MSM of the first n SRS points with device scalars: through the SRS window
+// table whose window bits are nearest to msm_table_window_bits(n) (or equal
+// window_bits when given), else the prepared points (GLV when they are known
+// to lie in G1).
 void srs_msm(const kzg_srs* srs, const uint32_t* d_scalars, size_t n, int window_bits, MsmWorkspace& ws,
              hipStream_t s, uint8_t* d_out);
+
+// Index of the SRS table whose window bits are nearest to c (ties: the wider),
+// -1 if the SRS has none within max_gap bits.
+int srs_table_near(const kzg_srs* srs, int c, int max_gap);
 
 // value < r for a 32-byte LE element of the curve's scalar field
 bool canonical_fr(int curve, const uint8_t* le32);

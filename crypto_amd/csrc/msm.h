@@ -43,7 +43,11 @@ struct MsmJob {
     uint32_t win_base;  // first global window
     uint32_t gb_base;   // first global bucket
     uint32_t blk_base;  // first block of the digit kernels
-    uint32_t pad;
+    uint32_t table;     // > 0: the points are a window table of this row stride
+                        // (row w = 2^(c w) P at tab_base + pt_base + w * table
+                        // + i); the job's digits then share one window of buckets
+    uint32_t tab_base;  // added to every entry's point index (not to the
+                        // identity-flag index pt_base + i)
 };
 
 // LDS-histogram sort of a batch: per job, the signed digits are written once

@@ -272,7 +272,9 @@ void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t
 // read by streaming the scalar's words through a 64-bit bit buffer (static
 // word indices only: no scratch).  Block b serves job j = last job with
 // blk_base <= b; its global bucket ids start at the job's gb_base and its
-// entries carry the point's index in the batch's point array.
+// entries carry the point's index in the batch's point array.  A window-table
+// job (jd.table = T > 0) sends the digit of every window w to its one set of
+// buckets, as the entry of point tab_base + pt_base + w T + i.
 template <bool SCATTER>
 __global__ void __launch_bounds__(256)
 k_digits_multi(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__ inf,
@@ -297,6 +299,8 @@ k_digits_multi(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
     uint32_t s[8];
     load_scalar(scalars + jd.sc_base * 8, i, s);
     const uint32_t c = jd.c, B = 1u << (c - 1), mask = (1u << c) - 1u;
+    const uint32_t w0 = (256u + c - 1u) / c, wd = w0 + (w0 * c == 256u ? 1u : 0u);  // digit windows
+    const uint32_t wb = jd.table ? 0u : B;  // bucket-id step per digit window
     uint32_t w = 0, carry = 0;
     auto emit = [&](uint32_t v) {
         int32_t d = (int32_t)(v + carry);
@@ -306,12 +310,12 @@ k_digits_multi(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
             carry = 1;
         }
         if (d != 0) {
-            const uint32_t gb = jd.gb_base + w * B + (uint32_t)(d > 0 ? d : -d) - 1u;
+            const uint32_t gb = jd.gb_base + w * wb + (uint32_t)(d > 0 ? d : -d) - 1u;
             if (!SCATTER) {
                 atomicAdd(&counts_or_cursor[gb], 1u);
             } else {
                 const uint32_t pos = atomicAdd(&counts_or_cursor[gb], 1u);
-                entries[pos] = p | (d < 0 ? 0x80000000u : 0u);
+                entries[pos] = (jd.tab_base + p + w * jd.table) | (d < 0 ? 0x80000000u : 0u);
             }
         }
         ++w;
@@ -328,7 +332,7 @@ k_digits_multi(const uint32_t* __restrict__ scalars, const uint8_t* __restrict__
             nb -= c;
         }
     }
-    while (w < jd.nwin) {  // the remaining < c bits, then the carry window
+    while (w < wd) {  // the remaining < c bits, then the carry window
         emit((uint32_t)buf & mask);
         buf >>= c;
     }
@@ -367,8 +371,10 @@ void digits_dispatch(int c, bool scatter, const uint32_t* d_scalars, const uint8
 //                    codes, window-major per job (coalesced)
 //   k_batch_count    one workgroup per (job, digit window w, part p): LDS
 //                    histogram of 2^(c-1) words -> cnt[item]
-//   k_batch_colscan  per global bucket: prefix over the job's parts -> pre,
-//                    and the bucket's count -> tot
+//   k_batch_colscan  per global bucket: prefix over the job's parts (over all
+//                    its (window, part) items for a window-table job, whose
+//                    windows share one set of buckets) -> pre, and the
+//                    bucket's count -> tot
 //   scan             offsets
 //   k_batch_scatter  per item: LDS cursor[b] = offsets[gb] + pre[item][b];
 //                    every digit takes a slot by an LDS atomic
@@ -486,10 +492,13 @@ k_batch_colscan(const uint32_t* __restrict__ cnt, const MsmJob* __restrict__ job
     }
     const MsmJob jd = jobs[lo];
     const SortJob so = sjobs[lo];
-    const uint32_t B = 1u << (jd.c - 1), r = gb - jd.gb_base, w = r >> (jd.c - 1), b = r & (B - 1);
+    const uint32_t B = 1u << (jd.c - 1), r = gb - jd.gb_base, b = r & (B - 1);
+    // items (w, p) feeding this bucket: one window's parts, or every item
+    const uint32_t first = jd.table ? 0u : (r >> (jd.c - 1)) * so.parts;
+    const uint32_t last = jd.table ? so.wd * so.parts : first + so.parts;
     uint32_t run = 0;
-    for (uint32_t p = 0; p < so.parts; ++p) {
-        const size_t idx = so.cnt_base + ((size_t)w * so.parts + p) * B + b;
+    for (uint32_t it = first; it < last; ++it) {
+        const size_t idx = so.cnt_base + (size_t)it * B + b;
         const uint32_t c = cnt[idx];
         pre[idx] = run;
         run += c;
@@ -510,7 +519,8 @@ k_batch_scatter(const uint16_t* __restrict__ dig, const MsmJob* __restrict__ job
     const uint32_t item = blockIdx.x - so.item_base, w = item / so.parts, p = item - w * so.parts;
     const uint32_t B = 1u << (jd.c - 1);
     const uint32_t* pp = pre + so.cnt_base + (size_t)item * B;
-    const uint32_t* wo = offsets + jd.gb_base + w * B;
+    const uint32_t* wo = offsets + jd.gb_base + (jd.table ? 0u : w * B);
+    const uint32_t base = jd.tab_base + jd.pt_base + w * jd.table;
     for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) cur[b] = wo[b] + pp[b];
     __syncthreads();
     const uint16_t* d = dig + so.dig_base + (size_t)w * jd.n;
@@ -519,7 +529,7 @@ k_batch_scatter(const uint16_t* __restrict__ dig, const MsmJob* __restrict__ job
         const uint16_t code = d[i];
         if (code != kNoDigit) {
             const uint32_t pos = atomicAdd(&cur[code & 0x7FFFu], 1u);
-            entries[pos] = (jd.pt_base + i) | ((uint32_t)(code & 0x8000u) << 16);
+            entries[pos] = (base + i) | ((uint32_t)(code & 0x8000u) << 16);
         }
     }
 }

@@ -460,7 +460,11 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
         const int c = j.c ? (int)j.c : msm_window_bits(curve, j.n, kMsmBatched);
         if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
         j.c = (uint32_t)c;
-        total_buckets += (uint64_t)msm_num_windows(c) << (c - 1);
+        total_buckets += (uint64_t)(j.table ? 1 : msm_num_windows(c)) << (c - 1);
+        const uint64_t span = j.table ? (uint64_t)msm_table_rows(c) * j.table : (uint64_t)j.n;
+        if ((j.table && j.pt_base + (uint64_t)j.n > j.table) ||
+            (uint64_t)j.tab_base + j.pt_base + span > 0x80000000ull)
+            throw std::invalid_argument("job beyond its table or 2^31 points");
         if ((1u << (c - 1)) < min_b) min_b = 1u << (c - 1);
     }
     plan.seg = reduce_segment(total_buckets, min_b);
@@ -469,12 +473,12 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
     size_t entries = 0, max_n = 0;
     for (auto& j : plan.jobs) {
         const int c = (int)j.c;
-        const uint32_t W = (uint32_t)msm_num_windows(c), B = 1u << (c - 1), t1 = B / plan.seg;
+        const uint32_t Wd = (uint32_t)msm_num_windows(c);                            // digit windows
+        const uint32_t W = j.table ? 1u : Wd, B = 1u << (c - 1), t1 = B / plan.seg;  // bucket windows
         j.nwin = W;
         j.win_base = (uint32_t)nwin;
         j.gb_base = (uint32_t)nb;
         j.blk_base = (uint32_t)blocks;
-        j.pad = 0;
         for (uint32_t w = 0; w < W; ++w) {
             lv_base.push_back((uint32_t)(nseg + (uint64_t)w * t1));
             lv_n.push_back(t1);
@@ -483,9 +487,11 @@ void msm_batch_plan(int curve, const MsmJob* specs, size_t njobs, int chunk, Msm
         nwin += W;
         nseg += (uint64_t)W * t1;
         blocks += (j.n + 255) / 256;
-        entries += (size_t)j.n * W;
-        if (j.n > max_n) max_n = j.n;
-        if ((uint64_t)j.pt_base + j.n > 0x80000000ull) throw std::invalid_argument("point index beyond 2^31");
+        entries += (size_t)j.n * Wd;
+        // a bucket holds at most one digit per point and digit window of its job
+        const size_t per_bucket = (size_t)j.n * (j.table ? Wd : 1u);
+        if (per_bucket > max_n) max_n = per_bucket;
+
     }
     if (nb >= 0xFFFFFFFFull || blocks >= 0x80000000ull) throw std::invalid_argument("batch too large");
     plan.nb = (uint32_t)nb;

@@ -55,8 +55,12 @@ uint64_t batch_footprint(int curve, const std::vector<uint32_t>& n) {
     for (uint32_t v : n) {
         const int c = msm_window_bits(curve, v, kMsmBatched);
         const uint64_t W = (uint64_t)msm_num_windows(c), B = 1ull << (c - 1);
+        uint64_t parts = 1;  // the batch sort's parts per window
+        while (parts < 64 && parts * 2 * 8192 <= v) parts *= 2;
         bytes += 3ull * 32 * v;                     // evals, coeffs, scratch
         bytes += (uint64_t)v * W * (4 + rec / 32);  // entries + carries
+        bytes += (uint64_t)v * W * 2;               // the sort's digit codes
+        bytes += W * parts * B * 8;                 // the sort's per-part counts and prefixes
         bytes += W * B * (12 + rec + rec / 8);      // counts/offsets/cursor, bucket sums, segments
     }
     return bytes;

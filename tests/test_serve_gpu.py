@@ -131,3 +131,41 @@ def test_commit_batch_mixed_sizes_and_handles(env, curve):
     for h, ev, g in zip(handles, evs, got):
         assert g == h.commit(ev)
     other.close()
+
+
+def test_serve_and_batch_with_srs_window_tables():
+    """SRSs holding several window tables (kzg_srs_precompute with different
+    window bits): batched, served and single commitments keep the bytes of the
+    table-free SRS, for every job size (jobs near a table's window bits use
+    it; the others run ordinary windows over the tables' points)."""
+    import crypto_amd
+    from crypto_amd import serve
+
+    crypto_amd.lib.require_device()
+    plain = {1: crypto_amd.KzgSrs.generate(1, TAU, 8192), 2: crypto_amd.KzgSrs.generate(2, TAU, 8192)}
+    tabled = {1: crypto_amd.KzgSrs.generate(1, TAU, 8192), 2: crypto_amd.KzgSrs.generate(2, TAU, 8192)}
+    for curve in (1, 2):
+        b0 = tabled[curve].precompute(8)
+        b1 = tabled[curve].precompute(12)
+        assert b1 > b0 > 0
+        assert tabled[curve].precompute(12) == b1  # already there
+    rnd = random.Random(11)
+    reqs = []
+    for i in range(40):
+        curve = rnd.choice([1, 2])
+        n = 1 << rnd.randint(0, 13)
+        reqs.append((curve, _evals(curve, 9000 + i, n)))
+    want = [plain[c].commit(ev.tobytes()) for c, ev in reqs]
+    for (c, ev), w in zip(reqs, want):
+        assert tabled[c].commit(ev.tobytes()) == w
+    for curve in (1, 2):
+        sub = [(ev, w) for (c, ev), w in zip(reqs, want) if c == curve]
+        got = crypto_amd.kzg_commit_batch([tabled[curve]] * len(sub), [ev.tobytes() for ev, _ in sub])
+        assert got == [w for _, w in sub]
+    server = serve.KzgServer(tabled, lanes=2, max_batch=16)
+    tickets = [server.submit(c, ev) for c, ev in reqs]
+    got = [server.wait(t)[0] for t in tickets]
+    server.close()
+    assert got == want
+    for s in list(plain.values()) + list(tabled.values()):
+        s.close()
