@@ -38,17 +38,23 @@ int kzg_srs_generate(uint32_t curve, const uint8_t tau[32], size_t n, kzg_srs** 
 void kzg_srs_destroy(kzg_srs* srs);
 size_t kzg_srs_size(const kzg_srs* srs);
 uint32_t kzg_srs_curve(const kzg_srs* srs);
-/* Builds a window table of the SRS on the device: rows w = 0..R-1 hold
- * 2^(c w) [tau^i] G (R = windows of c-bit digits over 256 bits, 17 at c = 16),
- * so every later MSM against this SRS (kzg_commit, kzg_commit_coeffs, kzg_msm,
- * kzg_commit_async; any prefix n <= size) reduces one window and combines none.
- * Costs R * size * 112 (BLS12-381) or 80 (BN254) bytes of device memory: 7.5 GB
- * for a 2^22-point BLS12-381 SRS at c = 16.  window_bits 0 = the library's
- * choice for the SRS size, else 5..18.  Synchronous; call it before the SRS is
- * shared with other threads.  Results are unchanged byte for byte.  No
- * counterpart in the reference. */
+/* Adds a window table of the SRS on the device for window bits c: rows
+ * w = 0..R-1 hold 2^(c w) [tau^i] G (R = windows of c-bit digits over 256
+ * bits, 17 at c = 16).  An SRS may hold tables for several c (one call per c;
+ * a c already present is a no-op; all tables are rebuilt into one buffer).
+ * Every later MSM against the SRS uses them: single commits (kzg_commit,
+ * kzg_commit_coeffs, kzg_msm, kzg_commit_async) take the table whose c is
+ * nearest to the window width the library picks for their size; batched and
+ * served commits take, per request, a table within 2 bits of the request's
+ * batched window width (else ordinary windows).  A table MSM reduces one
+ * window and combines none.  Costs R * size * 112 (BLS12-381) or 80 (BN254)
+ * bytes of device memory per table: 7.5 GB for a 2^22-point BLS12-381 SRS at
+ * c = 16.  window_bits 0 = the library's choice for the SRS size, else 5..18;
+ * all tables together must stay under 2^31 points.  Synchronous; call it
+ * before the SRS is shared with other threads.  Results are unchanged byte
+ * for byte.  No counterpart in the reference. */
 int kzg_srs_precompute(kzg_srs* srs, int window_bits);
-/* Device bytes of the SRS's window table (0 = none). */
+/* Device bytes of the SRS's window tables (0 = none). */
 size_t kzg_srs_table_bytes(const kzg_srs* srs);
 /* Copies points [first, first + count) back to the host in wire format. */
 int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* points_out);
