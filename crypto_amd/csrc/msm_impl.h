@@ -131,18 +131,17 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
 //                  compact list (one atomic) and its threads write {chunk j,
 //                  position r in the run, len} for the run's chunks (one run
 //                  can hold most of a window's chunks);
-//   k_fold_level   pairwise tree over the listed chunks, one quad per list
-//                  entry: level L adds carry[j + 2^L] into carry[j] for run
-//                  positions r = 0 mod 2^(L+1); levels past log2(longest run)
-//                  return at once.  Grid-stride over the list, so a launch is
-//                  small whatever the number of chunks.  A level's cost is one
-//                  point addition's latency, so the additions are the
-//                  4-lane ones.
+//   k_fold_groups  tree of fan-in 64 over the listed chunks: at level L (stride
+//                  S = 64^L) the entry at run position r = 0 mod 64 S leads a
+//                  group, whose workgroup (one quad per element) sums the
+//                  carries at positions r + e S, e < 64, through a 6-step LDS
+//                  tree into carry[j].  Levels past the longest run return at
+//                  once; a 2^21-point MSM needs 3 launches instead of 17
+//                  pairwise ones.  Grid-stride over the list.
 // ctl[0] = list length, ctl[1] = longest listed run, ctl[2] = recorded runs
 // (all zeroed before).
 // (Templated only so that each per-curve translation unit owns its copy.)
 constexpr uint32_t kShortRun = kFoldShortRun;
-constexpr uint32_t kFoldThreads = 64 * 1024;
 
 template <class F>
 __global__ void __launch_bounds__(256)
@@ -182,21 +181,30 @@ k_list_runs(const uint2* __restrict__ runs, uint4* __restrict__ list, uint32_t* 
     }
 }
 
+constexpr uint32_t kFoldFan = 64;
+
 template <class F>
-__global__ void __launch_bounds__(64)
-k_fold_level(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, uint32_t level,
-             uint32_t* __restrict__ carry) {
-    const uint32_t s = 1u << level;
-    if (s >= ctl[1]) return;
-    const uint32_t count = ctl[0];
-    const uint32_t q0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 2, nq = (gridDim.x * blockDim.x) >> 2;
+__global__ void __launch_bounds__(4 * kFoldFan)
+k_fold_groups(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, uint32_t stride,
+              uint32_t* __restrict__ carry) {
+    __shared__ Xyzz<F> sh[kFoldFan];
+    if (stride >= ctl[1]) return;  // every run is one element at this level
+    const uint32_t count = ctl[0], q = threadIdx.x >> 2;
+    const uint64_t span = (uint64_t)stride * kFoldFan;
 #pragma unroll 1
-    for (uint32_t i = q0; i < count; i += nq) {  // the four lanes of a quad take the same entries
-        const uint4 e = list[i];
-        const uint32_t j = e.x, r = e.y, len = e.z;
-        if ((r & (2 * s - 1)) != 0 || r + s >= len) continue;
-        const Xyzz<F> sum = xyzz_add_quad<F>(xyzz_load<F>(carry, j), xyzz_load<F>(carry, j + s));
-        if (quad_lane() == 0) xyzz_store<F>(carry, j, sum);
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {  // one entry per workgroup step
+        const uint4 en = list[i];                                 // {chunk j, run position r, run length}
+        if (en.y % span != 0) continue;                           // not a group leader at this level
+        const uint64_t pos = en.y + (uint64_t)q * stride;
+        Xyzz<F> p = pos < en.z ? xyzz_load<F>(carry, en.x + (size_t)q * stride) : xyzz_inf<F>();
+#pragma unroll 1
+        for (uint32_t st = 1; st < kFoldFan; st <<= 1) {
+            __syncthreads();  // the previous step's readers of sh are done
+            if ((q & (2 * st - 1)) == st && quad_lane() == 0) sh[q] = p;
+            __syncthreads();
+            if ((q & (2 * st - 1)) == 0) p = xyzz_add_quad<F>(p, sh[q + st]);
+        }
+        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, p);
     }
 }
 

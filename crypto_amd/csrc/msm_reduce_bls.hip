@@ -13,10 +13,14 @@ void launch_fold_bls(hipStream_t s, const uint32_t* offsets, uint32_t nb, uint32
     hipLaunchKernelGGL(k_carry_runs<CurveBLS12381::Fp>, dim3(ceil_div((size_t)nb * 4, 256)), dim3(256), 0, s, offsets, nb,
                        chunk, r2, ctl, carry);
     hipLaunchKernelGGL(k_list_runs<CurveBLS12381::Fp>, dim3(256), dim3(256), 0, s, r2, l4, ctl);
-    // one quad per list entry, at most kFoldThreads quads
-    const unsigned grid = ceil_div(4 * (size_t)(nchunks < kFoldThreads ? nchunks : kFoldThreads), 64);
-    for (uint32_t level = 0; level < levels; ++level)
-        hipLaunchKernelGGL(k_fold_level<CurveBLS12381::Fp>, dim3(grid), dim3(64), 0, s, l4, ctl, level, carry);
+    // levels = pairwise levels a run of this MSM can need; each group level
+    // covers six of them.  Group leaders sit every 64^(level+1) list entries:
+    // an odd grid stride spreads them over all workgroups (a multiple of 64
+    // would hand every leader of a run to the same few).
+    const unsigned grid = nchunks < 2047 ? (nchunks ? nchunks | 1u : 1u) : 2047;
+    uint32_t stride = 1;
+    for (uint32_t level = 0; level < levels; level += 6, stride *= kFoldFan)
+        hipLaunchKernelGGL(k_fold_groups<CurveBLS12381::Fp>, dim3(grid), dim3(4 * kFoldFan), 0, s, l4, ctl, stride, carry);
 }
 
 void launch_reduce_seg_bls(unsigned grid, hipStream_t s, const uint32_t* offsets, uint32_t chunk,
