@@ -89,12 +89,20 @@ struct MulInline {
     static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
         return fe_mul<F>(a, b);
     }
+    template <class F>
+    static KZ_DEV Fe<F> sqr(const Fe<F>& a) {
+        return fe_sqr<F>(a);
+    }
 };
 
 struct MulCall {
     template <class F>
     static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
         return fe_mul_call<F>(a, b);
+    }
+    template <class F>
+    static KZ_DEV Fe<F> sqr(const Fe<F>& a) {  // the one out-of-line product
+        return fe_mul_call<F>(a, a);
     }
 };
 
@@ -151,13 +159,13 @@ KZ_DEV Xyzz<F> xyzz_dbl_affine(const Affine<F>& a) {
 template <class F, class M = MulInline>
 KZ_DEV Xyzz<F> xyzz_dbl(const Xyzz<F>& p) {
     Fe<F> U = fe_dbl<F>(p.Y);
-    Fe<F> V = M::template mul<F>(U, U);
+    Fe<F> V = M::template sqr<F>(U);
     Fe<F> W = M::template mul<F>(U, V);
     Fe<F> S = M::template mul<F>(p.X, V);
-    Fe<F> xx = M::template mul<F>(p.X, p.X);
+    Fe<F> xx = M::template sqr<F>(p.X);
     Fe<F> xx3 = fe_add<F>(fe_dbl<F>(xx), xx);
     Xyzz<F> r;
-    r.X = fe_sub<F>(M::template mul<F>(xx3, xx3), fe_dbl<F>(S));
+    r.X = fe_sub<F>(M::template sqr<F>(xx3), fe_dbl<F>(S));
     r.Y = fe_sub<F>(M::template mul<F>(xx3, fe_sub<F>(S, r.X)), M::template mul<F>(W, p.Y));
     r.ZZ = M::template mul<F>(V, p.ZZ);
     r.ZZZ = M::template mul<F>(W, p.ZZZ);
@@ -182,10 +190,10 @@ KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
         else acc = xyzz_inf<F>();
         return;
     }
-    Fe<F> PP = M::template mul<F>(P, P);
+    Fe<F> PP = M::template sqr<F>(P);
     Fe<F> PPP = M::template mul<F>(P, PP);
     Fe<F> Q = M::template mul<F>(acc.X, PP);
-    Fe<F> X3 = fe_sub<F>(fe_sub<F>(M::template mul<F>(R, R), PPP), fe_dbl<F>(Q));
+    Fe<F> X3 = fe_sub<F>(fe_sub<F>(M::template sqr<F>(R), PPP), fe_dbl<F>(Q));
     Fe<F> Y3 = fe_sub<F>(M::template mul<F>(R, fe_sub<F>(Q, X3)), M::template mul<F>(acc.Y, PPP));
     acc.X = X3;
     acc.Y = Y3;
@@ -208,11 +216,11 @@ KZ_DEV Xyzz<F> xyzz_add(const Xyzz<F>& a, const Xyzz<F>& b) {
         if (fe_is_zero<F>(R)) return xyzz_dbl<F, M>(a);
         return xyzz_inf<F>();
     }
-    Fe<F> PP = M::template mul<F>(P, P);
+    Fe<F> PP = M::template sqr<F>(P);
     Fe<F> PPP = M::template mul<F>(P, PP);
     Fe<F> Q = M::template mul<F>(U1, PP);
     Xyzz<F> r;
-    r.X = fe_sub<F>(fe_sub<F>(M::template mul<F>(R, R), PPP), fe_dbl<F>(Q));
+    r.X = fe_sub<F>(fe_sub<F>(M::template sqr<F>(R), PPP), fe_dbl<F>(Q));
     r.Y = fe_sub<F>(M::template mul<F>(R, fe_sub<F>(Q, r.X)), M::template mul<F>(S1, PPP));
     r.ZZ = M::template mul<F>(M::template mul<F>(a.ZZ, b.ZZ), PP);
     r.ZZZ = M::template mul<F>(M::template mul<F>(a.ZZZ, b.ZZZ), PPP);

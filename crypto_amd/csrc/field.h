@@ -270,9 +270,44 @@ KZ_DEV Fe<F> fe_mul(const Fe<F>& a, const Fe<F>& b) {
     return fe_mul_k<F, KZ_MUL_ILP>(a, b);
 }
 
+// Montgomery square a^2 2^(-29N) mod p: the product scan of fe_mul_k with
+// each cross product a_i a_j (i < j) taken once against the doubled limb
+// 2 a_i (< 2^30) -- (N^2 + N) / 2 limb products for the a*a part instead of
+// N^2 (105 instead of 196 for BLS12-381 Fp), with no extra work per column.
+// A column holds at most N/2 cross products below 2^59, one square below 2^58
+// and N reduction products below 2^58: below 2^63.
 template <class F>
 KZ_DEV Fe<F> fe_sqr(const Fe<F>& a) {
-    return fe_mul<F>(a, a);
+    constexpr int N = F::N;
+    uint32_t m[N], d[N];
+    Fe<F> t;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = a.v[i] << 1;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int i = 0; 2 * i < k; ++i) acc += (uint64_t)d[i] * a.v[k - i];
+        if ((k & 1) == 0) acc += (uint64_t)a.v[k / 2] * a.v[k / 2];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * F::P[k - i];
+        m[k] = ((uint32_t)acc * F::PINV) & kLimbMask;
+        acc += (uint64_t)m[k] * F::P[0];
+        acc >>= kLimbBits;
+    }
+#pragma unroll
+    for (int k = N; k < 2 * N - 1; ++k) {
+#pragma unroll
+        for (int i = k - N + 1; 2 * i < k; ++i) acc += (uint64_t)d[i] * a.v[k - i];
+        if ((k & 1) == 0) acc += (uint64_t)a.v[k / 2] * a.v[k / 2];
+#pragma unroll
+        for (int i = k - N + 1; i < N; ++i) acc += (uint64_t)m[i] * F::P[k - i];
+        t.v[k - N] = (uint32_t)acc & kLimbMask;
+        acc >>= kLimbBits;
+    }
+    t.v[N - 1] = (uint32_t)acc;  // < 2p < 2^(29N): fits the top limb
+    fe_reduce_once<F>(t);
+    return t;
 }
 
 template <class F>

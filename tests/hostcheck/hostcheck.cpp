@@ -23,7 +23,7 @@ void to_le(const Fe<F>& a, uint8_t* b, int nbytes) {
     for (int k = 0; k < nbytes; ++k) b[k] = (uint8_t)(w[k / 4] >> (8 * (k % 4)));
 }
 
-// op: 0 add 1 sub 2 mul 3 inv; canonical LE in/out
+// op: 0 add 1 sub 2 mul 3 inv 4 square of a; canonical LE in/out
 template <class F>
 void field_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, int nbytes) {
     Fe<F> x = fe_to_mont<F>(from_le<F>(a, nbytes)), y = fe_to_mont<F>(from_le<F>(b, nbytes)), r;
@@ -31,6 +31,7 @@ void field_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, int nbyt
         case 0: r = fe_add<F>(x, y); break;
         case 1: r = fe_sub<F>(x, y); break;
         case 2: r = fe_mul<F>(x, y); break;
+        case 4: r = fe_sqr<F>(x); break;
         default: r = fe_inv<F>(x); break;
     }
     to_le<F>(fe_from_mont<F>(r), out, nbytes);

@@ -34,7 +34,7 @@ def test_field_ops(hc, field):
     for i, a in enumerate(vals):
         b = vals[(i * 7 + 3) % len(vals)] if i < len(special) else rnd.randrange(p)
         A, B = a.to_bytes(nb, "little"), b.to_bytes(nb, "little")
-        for op, want in ((0, (a + b) % p), (1, (a - b) % p), (2, a * b % p)):
+        for op, want in ((0, (a + b) % p), (1, (a - b) % p), (2, a * b % p), (4, a * a % p)):
             hc.hc_field_op(field, op, A, B, out)
             assert int.from_bytes(out.raw, "little") == want, (field, op, a, b)
         if a:
