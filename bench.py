@@ -55,19 +55,29 @@ def parse_args():
 
 # --------------------------------------------------------------------- work
 GLV_LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF  # BLS12-381 G1 endomorphism eigenvalue
+# BN254: short lattice basis (A1, -B1), (A2, B2) of (r, lambda) and the
+# 2^256-scaled Babai estimates, as in the library's split
+BN_A1, BN_B1 = 0x6F4D8248EEB859FC8211BBEB7D4F1128, 0x89D3256894D213E3
+BN_A2, BN_B2 = 0x89D3256894D213E3, 0x6F4D8248EEB859FD0BE4E1541221250B
 
 
-def glv_halves(scalars):
-    """(n, 4) LE limbs -> (2n, 4) limbs [s1; s2] with s = s1 + lambda s2 mod r,
-    the split the library's GLV path makes (for the work count below)."""
+def glv_halves(scalars, curve: int = 2):
+    """(n, 4) LE limbs -> (2n, 4) limbs [|k1|; |k2|] with s = k1 + lambda k2
+    mod r, the split the library's GLV path makes (for the work count below)."""
     import numpy as np
 
     from crypto_amd import inputs
 
-    r = inputs.ORDER[2]
+    r = inputs.ORDER[curve]
+    g1, g2 = (BN_B2 << 256) // r, (BN_B1 << 256) // r
     lo, hi = [], []
     for v in inputs.to_int(scalars):
-        q, rem = divmod(v % r, GLV_LAMBDA)
+        k = v % r
+        if curve == 2:
+            q, rem = divmod(k, GLV_LAMBDA)
+        else:
+            b1, b2 = (k * g1) >> 256, (k * g2) >> 256
+            rem, q = abs(k - b1 * BN_A1 - b2 * BN_A2), abs(b1 * BN_B1 - b2 * BN_B2)
         lo.append(rem)
         hi.append(q)
     m = (1 << 64) - 1
@@ -186,7 +196,7 @@ def timed_msm(lib, torch, D, curve, n, seed, steps, warmup, window_bits, profile
         d_t = torch.from_numpy(tk.view("u1").reshape(-1).copy()).to("cuda")
         check = gpu_fixed_base(lib, torch, curve, d_t, 1, stream).cpu().numpy().tobytes()
         extra = {"points_gen_s": gen_s, "scalars": ss, "points_wire": d_pts_wire,
-                 "path": "table" if table else ("g1_glv" if (in_g1 and curve == 2) else "plain")}
+                 "path": "table" if table else ("g1_glv" if in_g1 else "plain")}
         if tab is not None:
             extra.update(table_s=table_s, table_gb=tab.table.numel() / 1e9, table_c=tab.c)
             del tab
@@ -589,7 +599,7 @@ def main():
     n_eff = 2 * n if glv else n
     c = args.window_bits or lib.kinet_hip_msm_window_bits(curve, n, 2 if glv else 0)
     cell = T.msm_cell(curve, 2 if glv else 0, args.log2n)
-    ws = window_stats(glv_halves(ext["scalars"]) if glv else ext["scalars"], c, chunk, 128 if glv else 256)
+    ws = window_stats(glv_halves(ext["scalars"], curve) if glv else ext["scalars"], c, chunk, 128 if glv else 256)
     nl = 12 if curve == 2 else 8          # 32-bit limbs of Fp: 2n^2 + n multiply-adds per product
     mads_per_mul = 2 * nl * nl + nl
     acc_mads = ws["madds"] * 10 * mads_per_mul       # XYZZ mixed addition = 8M + 2S

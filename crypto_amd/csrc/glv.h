@@ -5,9 +5,132 @@
 
 namespace kz {
 
-// BLS12-381: d_pts2 = [P_i, phi(P_i)] (2n prepared records), d_inf2 likewise,
-// d_scalars2 = [s1_i, s2_i] with s_i = s1_i + lambda s2_i mod r (128-bit
-// halves in 32-byte records).  Valid only for points of G1.
+// BN254 GLV split (Gallant-Lambert-Vanstone 2001, with the short lattice basis
+// of (r, lambda) from the extended Euclidean algorithm): for any 256-bit s,
+// s = k1 + lambda k2 mod r with |k1|, |k2| < 2^127, lambda =
+// 0x30644e72e131a029048b6e193fd84104cc37a73fec2bc5e9b8ca0b2d36636f23 (a cube
+// root of unity mod r, matched to beta by phi(G) = lambda G).  Basis v1 =
+// (A1, -B1), v2 = (A2, B2) with det r; Babai coefficients estimated as
+// b1 = floor(k G1 / 2^256), b2 = floor(k G2 / 2^256), G1 = floor(2^256 B2 /
+// r), G2 = floor(2^256 B1 / r) (the truncation keeps |k| < 2^127, checked
+// against big integers over 2e5 random scalars and edge values);
+//   k1 = k - b1 A1 - b2 A2,   k2 = b1 B1 - b2 B2   (k = s mod r).
+// Writes |k1|, |k2| (4 LE words each) and their signs.  Host and device.
+namespace glvbn {
+constexpr uint32_t kR[8] = {0xF0000001u, 0x43E1F593u, 0x79B97091u, 0x2833E848u,
+                            0x8181585Du, 0xB85045B6u, 0xE131A029u, 0x30644E72u};
+constexpr uint32_t kG1[5] = {0x00FF6565u, 0x5398FD03u, 0xA773D2D2u, 0x4CCEF014u, 0x00000002u};
+constexpr uint32_t kG2[3] = {0xC7E0B3D7u, 0xD91D232Eu, 0x00000002u};
+constexpr uint32_t kA1[4] = {0x7D4F1128u, 0x8211BBEBu, 0xEEB859FCu, 0x6F4D8248u};
+constexpr uint32_t kA2[2] = {0x94D213E3u, 0x89D32568u};
+constexpr uint32_t kB1[2] = {0x94D213E3u, 0x89D32568u};
+constexpr uint32_t kB2[4] = {0x1221250Bu, 0x0BE4E154u, 0xEEB859FDu, 0x6F4D8248u};
+
+// acc (8 words, mod 2^256) += a (NA words) * b (NB words)
+template <int NA, int NB>
+__host__ __device__ inline void mac8(uint32_t* acc, const uint32_t* a, const uint32_t* b) {
+    for (int i = 0; i < NA; ++i) {
+        uint64_t carry = 0;
+        for (int j = 0; j < NB && i + j < 8; ++j) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + acc[i + j] + carry;
+            acc[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        for (int k = i + NB; k < 8 && carry; ++k) {
+            const uint64_t t = (uint64_t)acc[k] + carry;
+            acc[k] = (uint32_t)t;
+            carry = t >> 32;
+        }
+    }
+}
+
+// floor(k g / 2^256) for k (8 words) and g (NG words), result <= 4 words
+template <int NG>
+__host__ __device__ inline void mul_hi256(const uint32_t* k, const uint32_t* g, uint32_t* out4) {
+    uint32_t prod[8 + NG];
+    for (int i = 0; i < 8 + NG; ++i) prod[i] = 0;
+    for (int i = 0; i < 8; ++i) {
+        uint64_t carry = 0;
+        for (int j = 0; j < NG; ++j) {
+            const uint64_t t = (uint64_t)k[i] * g[j] + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + NG] = (uint32_t)carry;
+    }
+    for (int i = 0; i < 4; ++i) out4[i] = 8 + i < 8 + NG ? prod[8 + i] : 0u;
+}
+
+// a = -a over 8 words (two's complement)
+__host__ __device__ inline void neg8(uint32_t* a) {
+    uint64_t c = 1;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t t = (uint64_t)(~a[i]) + c;
+        a[i] = (uint32_t)t;
+        c = t >> 32;
+    }
+}
+
+__host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32_t* k2, bool& neg1, bool& neg2) {
+    uint32_t k[8];
+    for (int i = 0; i < 8; ++i) k[i] = s_in[i];
+    for (int rep = 0; rep < 6; ++rep) {  // k mod r: 2^256 < 6 r
+        bool ge = true;
+        for (int i = 7; i >= 0; --i)
+            if (k[i] != kR[i]) {
+                ge = k[i] > kR[i];
+                break;
+            }
+        if (!ge) break;
+        uint64_t borrow = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t t = (uint64_t)k[i] - kR[i] - borrow;
+            k[i] = (uint32_t)t;
+            borrow = (t >> 32) & 1u;
+        }
+    }
+    uint32_t b1[4], b2[4];
+    mul_hi256<5>(k, kG1, b1);
+    mul_hi256<3>(k, kG2, b2);
+    // t = b1 A1 + b2 A2;  k1 = k - t
+    uint32_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    mac8<4, 4>(t, b1, kA1);
+    mac8<2, 2>(t, b2, kA2);
+    neg8(t);
+    uint32_t x[8];
+    {
+        uint64_t c = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t v = (uint64_t)k[i] + t[i] + c;
+            x[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+    }
+    neg1 = (x[7] >> 31) != 0;
+    if (neg1) neg8(x);
+    for (int i = 0; i < 4; ++i) k1[i] = x[i];
+    // k2 = b1 B1 - b2 B2
+    uint32_t u[8] = {0, 0, 0, 0, 0, 0, 0, 0}, v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    mac8<4, 2>(u, b1, kB1);
+    mac8<2, 4>(v, b2, kB2);
+    neg8(v);
+    uint64_t c = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t w = (uint64_t)u[i] + v[i] + c;
+        x[i] = (uint32_t)w;
+        c = w >> 32;
+    }
+    neg2 = (x[7] >> 31) != 0;
+    if (neg2) neg8(x);
+    for (int i = 0; i < 4; ++i) k2[i] = x[i];
+}
+}  // namespace glvbn
+
+// d_pts2 = [P_i, phi(P_i)] (2n prepared records), d_inf2 likewise, d_scalars2
+// = the two 128-bit parts (32-byte records) of s_i = s1_i + lambda s2_i mod r.
+// BLS12-381: s1 = s mod r mod lambda, s2 = s mod r div lambda.  BN254: the
+// signed lattice split above, the signs folded into the points (-P, -phi(P)).
+// Valid only for points of G1 (all curve points for BN254).
 void glv_expand(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 uint32_t* d_pts2, uint8_t* d_inf2, uint32_t* d_scalars2, hipStream_t s);
 

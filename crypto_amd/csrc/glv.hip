@@ -27,7 +27,11 @@ namespace kz {
 
 namespace {
 
-// beta R mod p (radix 2^29, R = 2^406)
+// BN254: beta R mod p (radix 2^29, R = 2^261) for
+// beta = 0x30644e72e131a0295e6dd9e7e0acccb0c28f069fbb966e3de4bd44e5607cfd48
+constexpr uint32_t kBetaMontBn[9] = {0x18CCB791u, 0x175B1C3Au, 0x0B83D6E2u, 0x0E8ED071u, 0x1282BEE2u,
+                                     0x04220E84u, 0x1FE4017Fu, 0x15084D4Au, 0x00169119u};
+// BLS12-381: beta R mod p (radix 2^29, R = 2^406)
 constexpr uint32_t kBetaMont[14] = {0x1195DFEBu, 0x1B04E484u, 0x06026044u, 0x086070A2u, 0x1FD68858u,
                                     0x137E9670u, 0x06871E67u, 0x1E736664u, 0x083B24F6u, 0x08A70373u,
                                     0x02A012FDu, 0x0112F94Bu, 0x18A2733Cu, 0x00000003u};
@@ -143,6 +147,42 @@ k_glv_points_bls(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ i
     out_inf[(size_t)n + i] = f;
 }
 
+// BN254 (signed split, glv.h): out_pts = [+-P_i, +-phi(P_i)], out_inf
+// likewise, out_sc = [|k1_i|, |k2_i|] (32-byte records, upper half zero).
+__global__ void __launch_bounds__(256)
+k_glv_expand_bn(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, const uint32_t* __restrict__ scalars,
+                uint32_t n, uint32_t* __restrict__ out_pts, uint8_t* __restrict__ out_inf, uint32_t* __restrict__ out_sc) {
+    using F = BnFp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s[8];
+    {
+        const uint4* q = reinterpret_cast<const uint4*>(scalars + (size_t)i * 8);
+        const uint4 a = q[0], b = q[1];
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+        s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    }
+    uint32_t k1[4], k2[4];
+    bool neg1, neg2;
+    glvbn::split(s, k1, k2, neg1, neg2);
+    Affine<F> a = affine_load<F>(pts, i);
+    const Fe<F> y = a.y, ny = fe_neg<F>(a.y);
+    a.y = neg1 ? ny : y;
+    affine_store<F>(out_pts, i, a);
+    a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontBn));
+    a.y = neg2 ? ny : y;
+    affine_store<F>(out_pts, (size_t)n + i, a);
+    const uint8_t f = inf ? inf[i] : 0;
+    out_inf[i] = f;
+    out_inf[(size_t)n + i] = f;
+    uint4* o1 = reinterpret_cast<uint4*>(out_sc + (size_t)i * 8);
+    uint4* o2 = reinterpret_cast<uint4*>(out_sc + ((size_t)n + i) * 8);
+    o1[0] = make_uint4(k1[0], k1[1], k1[2], k1[3]);
+    o1[1] = make_uint4(0, 0, 0, 0);
+    o2[0] = make_uint4(k2[0], k2[1], k2[2], k2[3]);
+    o2[1] = make_uint4(0, 0, 0, 0);
+}
+
 // *bad |= 1 if a non-identity point is off the curve or (BLS12-381) outside G1.
 template <class C>
 __global__ void __launch_bounds__(64)
@@ -175,8 +215,14 @@ k_check_g1(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, ui
 
 void glv_expand(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 uint32_t* d_pts2, uint8_t* d_inf2, uint32_t* d_scalars2, hipStream_t s) {
-    if (curve != 2) throw std::invalid_argument("GLV split: BLS12-381 only");
     const unsigned grid = ceil_div(n, 256);
+    if (curve == 1) {
+        hipLaunchKernelGGL(k_glv_expand_bn, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
+                           d_inf2, d_scalars2);
+        KZ_HIP(hipGetLastError());
+        return;
+    }
+    if (curve != 2) throw std::invalid_argument("GLV split: unknown curve");
     hipLaunchKernelGGL(k_glv_points_bls, dim3(grid), dim3(256), 0, s, d_pts, d_inf, (uint32_t)n, d_pts2, d_inf2);
     hipLaunchKernelGGL(k_glv_split_bls, dim3(grid), dim3(256), 0, s, d_scalars, (uint32_t)n, d_scalars2);
     KZ_HIP(hipGetLastError());

@@ -22,7 +22,7 @@ struct MsmWorkspace {
 struct MsmParams {
     int c = 0;         // window bits; 0 = choose from n
     int chunk = 0;     // sorted entries per accumulation thread; 0 = from the entries per bucket
-    bool g1 = false;   // caller guarantees every point lies in G1: BLS12-381 may use the GLV split
+    bool g1 = false;   // caller guarantees every point lies in G1: the GLV split may be used
     int seg = 0;       // first-level reduction segment; 0 = tuned cell (when c = 0) or rule
     // > 0: d_pts is a window table built with window bits c for table_stride
     // points (row w = 2^(c w) P_i at w * table_stride + i, n <= table_stride);
@@ -136,7 +136,7 @@ struct MsmBatchPlan {
 };
 
 // Tuning modes: one MSM, a job of a batch, one MSM over G1 points through
-// the GLV split (BLS12-381; 2n points of 128-bit scalars).
+// the GLV split (2n points of 128-bit scalars).
 enum MsmMode { kMsmPlain = 0, kMsmBatched = 1, kMsmGlv = 2, kMsmModes = 3 };
 // Window bits used for an n-point MSM in `mode`: the tuning table entry of
 // (curve, mode, floor(log2 n)) when set, else msm_default_window_bits(n)

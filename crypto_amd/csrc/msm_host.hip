@@ -344,7 +344,7 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(curve);
-    // GLV (BLS12-381, points in G1): 2n points with 128-bit scalars
+    // GLV (points in G1): 2n points with 128-bit scalars
     int bits = 256;
     size_t npts = n;
     int c = prm.c;
@@ -355,7 +355,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
         if ((uint64_t)msm_table_rows(c) * prm.table_stride >= 0x80000000ull)
             throw std::invalid_argument("window table beyond 2^31 points");
     }
-    if (prm.g1 && !table && curve == 2 && n <= (1u << 29)) {
+    if (prm.g1 && !table && n <= (1u << 29)) {
         const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
             bits = 128;

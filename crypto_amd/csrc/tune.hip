@@ -249,7 +249,6 @@ Choice tune_batched(Engine& e, int curve, int lg, const MsmInputs& in, size_t jo
 void tune_msm(int curve, int mode, int log_min, int log_max) {
     if (curve != 1 && curve != 2) throw std::invalid_argument("curve");
     if (mode < 0 || mode >= kMsmModes) throw std::invalid_argument("mode");
-    if (mode == kMsmGlv && curve != 2) throw std::invalid_argument("the GLV mode is BLS12-381 only");
     if (log_min < 1 || log_max > 24 || log_min > log_max) throw std::invalid_argument("size classes out of range");
     if (mode == kMsmGlv && log_min < 11) throw std::invalid_argument("GLV size classes start at 2^11");
     Engine& e = engine();

@@ -80,7 +80,8 @@ def tune_all(log_min: int = 10, log_max: int = 22, ntt_min: int = 4, ntt_max: in
         jobs.append((f"msm curve={curve} single", lambda c=curve: tune_msm(c, MODE_SINGLE, log_min, log_max)))
         jobs.append((f"msm curve={curve} batched",
                      lambda c=curve: tune_msm(c, MODE_BATCHED, max(1, log_min - 2), min(log_max, 20))))
-    jobs.append(("msm curve=2 glv", lambda: tune_msm(2, MODE_GLV, max(11, log_min), log_max)))
+    for curve in (2, 1):
+        jobs.append((f"msm curve={curve} glv", lambda c=curve: tune_msm(c, MODE_GLV, max(11, log_min), log_max)))
     for field in (0, 1):
         jobs.append((f"ntt field={field}", lambda f=field: tune_ntt(f, ntt_min, ntt_max)))
     for name, fn in jobs:

@@ -38,11 +38,13 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
 
 /* The same MSM for points the caller guarantees to lie in G1 (for example
  * checked once by kinet_hip_points_check_g1, or generated as multiples of the
- * generator).  BLS12-381 then uses the GLV endomorphism phi(x, y) = (beta x, y)
- * = lambda (x, y): each 256-bit scalar is split as s = s1 + lambda s2 mod r
- * with 128-bit halves, giving a 2n-point MSM over half the windows.  Results
- * equal kinet_hip_msm_prepared's for points of G1; for other points they are
- * unspecified.  BN254 (cofactor 1) runs the plain path. */
+ * generator; for BN254, cofactor 1, every curve point).  It then uses the GLV
+ * endomorphism phi(x, y) = (beta x, y) = lambda (x, y): each 256-bit scalar is
+ * split as s = k1 + lambda k2 mod r with parts below 2^128 (BLS12-381: s mod r
+ * divided by the 128-bit lambda; BN254: a signed split over a short lattice
+ * basis, the signs moved onto the points), giving a 2n-point MSM over half
+ * the windows.  Results equal kinet_hip_msm_prepared's for points of G1; for
+ * other points they are unspecified. */
 int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                               size_t n, int window_bits, int chunk, void* d_out, void* stream);
 

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "curve.h"
+#include "glv.h"
 
 using namespace kz;
 
@@ -92,6 +93,21 @@ void hc_field_op(int field, int op, const uint8_t* a, const uint8_t* b, uint8_t*
         case 2: field_op<BnFp>(op, a, b, out, 32); break;
         default: field_op<BnFr>(op, a, b, out, 32); break;
     }
+}
+
+// BN254 GLV split of a 32-byte LE scalar: out = |k1| (16 B LE) || |k2| (16 B)
+// || sign of k1 || sign of k2
+void hc_glv_split_bn(const uint8_t* s32, uint8_t* out) {
+    uint32_t s[8] = {0}, k1[4], k2[4];
+    for (int k = 0; k < 32; ++k) s[k / 4] |= (uint32_t)s32[k] << (8 * (k % 4));
+    bool n1, n2;
+    glvbn::split(s, k1, k2, n1, n2);
+    for (int k = 0; k < 16; ++k) {
+        out[k] = (uint8_t)(k1[k / 4] >> (8 * (k % 4)));
+        out[16 + k] = (uint8_t)(k2[k / 4] >> (8 * (k % 4)));
+    }
+    out[32] = n1 ? 1 : 0;
+    out[33] = n2 ? 1 : 0;
 }
 
 void hc_point_op(int curve, int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {

@@ -67,3 +67,28 @@ def test_point_ops(hc, curve):
         assert out.raw == Qw
         hc.hc_point_op(curve, 1, inf, Qw, out)
         assert out.raw == Qw
+
+
+BN_LAMBDA = 0x30644E72E131A029048B6E193FD84104CC37A73FEC2BC5E9B8CA0B2D36636F23
+
+
+def test_glv_split_bn(hc):
+    """The BN254 GLV split of the library (host build): s = k1 + lambda k2 mod r
+    with |k1|, |k2| < 2^127 for random and edge scalars, lambda a cube root of
+    unity mod r; and lambda G = phi(G) = (beta x_G, y_G) on the curve."""
+    r = M.CURVES[1]["r"]
+    p = M.CURVES[1]["p"]
+    assert pow(BN_LAMBDA, 3, r) == 1 and BN_LAMBDA != 1
+    beta = 0x30644E72E131A0295E6DD9E7E0ACCCB0C28F069FBB966E3DE4BD44E5607CFD48
+    G = M.CURVES[1]["g"]
+    assert M.mul(1, G, BN_LAMBDA) == (beta * G[0] % p, G[1])
+    rnd = random.Random(5)
+    out = ctypes.create_string_buffer(34)
+    edge = [0, 1, 2, r - 1, r, r + 1, (1 << 256) - 1, BN_LAMBDA, r - BN_LAMBDA, (r - 1) // 2, 4 * r, 5 * r - 1]
+    for s in edge + [rnd.getrandbits(256) for _ in range(3000)]:
+        hc.hc_glv_split_bn(s.to_bytes(32, "little"), out)
+        raw = out.raw
+        k1 = int.from_bytes(raw[:16], "little") * (-1 if raw[32] else 1)
+        k2 = int.from_bytes(raw[16:32], "little") * (-1 if raw[33] else 1)
+        assert abs(k1) < 1 << 127 and abs(k2) < 1 << 127, hex(s)
+        assert (k1 + BN_LAMBDA * k2 - s) % r == 0, hex(s)

@@ -261,3 +261,36 @@ def test_msm_g1_golden_bls(ca):
         assert D.check_g1(pp)
         out = D.msm_prepared(pp, D.to_device(scalars), g1=True).cpu().numpy().tobytes()
         assert out[:96].hex() == case["expected"], case["n"]
+
+
+@pytest.mark.parametrize("n,c", [(2048, 0), (5000, 0), (5000, 9), (5000, 16), (1 << 16, 0)])
+def test_msm_g1_glv_bn254(ca, n, c):
+    """BN254 through the signed GLV split (kinet_hip_msm_prepared_g1): the plain
+    path's bytes and the closed form, for scalars 0, 1, r - 1, r, r + 1,
+    2^256 - 1, lambda, lambda^2, values near multiples of r and random ones,
+    with identity points among the inputs."""
+    from crypto_amd import device as D
+
+    curve = 1
+    k0, d = 0x31337, 0xBEEF1234567
+    pb = O.POINT_BYTES[curve]
+    points = bytearray(O.gen_points_progression(curve, k0, d, n))
+    zero_at = set(range(5, n, 197))
+    for i in zero_at:
+        points[i * pb:(i + 1) * pb] = bytes(pb)
+    r = M.CURVES[curve]["r"]
+    lam = 0x30644E72E131A029048B6E193FD84104CC37A73FEC2BC5E9B8CA0B2D36636F23
+    rnd = random.Random(n * 31 + c)
+    sc = [rnd.getrandbits(256) for _ in range(n)]
+    sc[:12] = [0, 1, r - 1, r, r + 1, (1 << 256) - 1, lam, lam * lam % r, 2 * r - 1, 4 * r + 7, r - lam, (r + 1) // 2]
+    scalars = b"".join(s.to_bytes(32, "little") for s in sc)
+    t = sum(s * (k0 + i * d) for i, s in enumerate(sc) if i not in zero_at) % r
+    want = O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+    pp = D.PreparedPoints(curve, bytes(points))
+    assert D.check_g1(pp)
+    d_sc = D.to_device(scalars)
+    print(f"bn glv n={n} c={c}", flush=True)
+    got = D.msm_prepared(pp, d_sc, window_bits=c, g1=True).cpu().numpy().tobytes()
+    plain = D.msm_prepared(pp, d_sc).cpu().numpy().tobytes()
+    assert got[:pb] == want
+    assert got == plain
