@@ -138,16 +138,23 @@ struct MsmBatchPlan {
 // Tuning modes: one MSM, a job of a batch, one MSM over G1 points through
 // the GLV split (2n points of 128-bit scalars).
 enum MsmMode { kMsmPlain = 0, kMsmBatched = 1, kMsmGlv = 2, kMsmModes = 3 };
+// A GLV cell holding kMsmGlvOff instead of window bits records that the
+// plain MSM beat the split at that size class: G1 MSMs of that class without
+// explicit window bits then run the plain path (same result).
+constexpr int kMsmGlvOff = 1;
 // Window bits used for an n-point MSM in `mode`: the tuning table entry of
 // (curve, mode, floor(log2 n)) when set, else msm_default_window_bits(n)
-// (of 2n for GLV).
+// (of 2n for GLV); for a GLV cell of kMsmGlvOff, the plain MSM's.
 int msm_window_bits(int curve, size_t n, int mode);
+// False when the GLV cell of n's size class is kMsmGlvOff.
+bool msm_glv_enabled(int curve, size_t n);
 // Tuned first-level reduction segment of that size class, 0 = built-in rule.
 int msm_tuned_segment(int curve, size_t n, int mode);
 // Installs window bits: c_by_log[k] for 2^k <= n < 2^(k+1) (0 = default
 // rule), k < count <= 32; count = 0 clears them.  Segments are kept.
 void msm_set_window_table(int curve, int mode, const int* c_by_log, int count);
-// One tuning table cell (lg = floor(log2 n)); c or seg 0 = built-in rule.
+// One tuning table cell (lg = floor(log2 n)); c or seg 0 = built-in rule; c
+// may be kMsmGlvOff in mode kMsmGlv.
 void msm_set_tuned(int curve, int mode, int lg, int c, int seg);
 void msm_get_tuned(int curve, int mode, int lg, int& c, int& seg);
 

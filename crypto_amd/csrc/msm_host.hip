@@ -47,13 +47,22 @@ void check_mode(int curve, int mode) {
 
 int msm_window_bits(int curve, size_t n, int mode) {
     const int lg = floor_log2(n);
+    int c = 0;
     if (curve >= 1 && curve <= 2 && mode >= 0 && mode < kMsmModes && lg < 32) {
         std::lock_guard<std::mutex> g(g_table_mu);
-        const int c = g_table[curve][mode][lg].c;
-        if (c) return c;
+        c = g_table[curve][mode][lg].c;
     }
+    if (c == kMsmGlvOff && mode == kMsmGlv) return msm_window_bits(curve, n, kMsmPlain);
+    if (c) return c;
     // GLV runs 2n points
     return msm_default_window_bits(mode == kMsmGlv ? 2 * n : n);
+}
+
+bool msm_glv_enabled(int curve, size_t n) {
+    const int lg = floor_log2(n);
+    if (curve < 1 || curve > 2 || lg >= 32) return true;
+    std::lock_guard<std::mutex> g(g_table_mu);
+    return g_table[curve][kMsmGlv][lg].c != kMsmGlvOff;
 }
 
 int msm_tuned_segment(int curve, size_t n, int mode) {
@@ -67,7 +76,8 @@ void msm_set_window_table(int curve, int mode, const int* c_by_log, int count) {
     check_mode(curve, mode);
     if (count < 0 || count > 32) throw std::invalid_argument("bad window table");
     for (int k = 0; k < count; ++k)
-        if (c_by_log[k] != 0 && (c_by_log[k] < 5 || c_by_log[k] > 18))
+        if (c_by_log[k] != 0 && (c_by_log[k] < 5 || c_by_log[k] > 18) &&
+            !(mode == kMsmGlv && c_by_log[k] == kMsmGlvOff))
             throw std::invalid_argument("window bits out of range [5, 18]");
     std::lock_guard<std::mutex> g(g_table_mu);
     for (int k = 0; k < 32; ++k) g_table[curve][mode][k].c = k < count ? c_by_log[k] : 0;
@@ -76,7 +86,8 @@ void msm_set_window_table(int curve, int mode, const int* c_by_log, int count) {
 void msm_set_tuned(int curve, int mode, int lg, int c, int seg) {
     check_mode(curve, mode);
     if (lg < 0 || lg >= 32) throw std::invalid_argument("size class out of range");
-    if (c != 0 && (c < 5 || c > 18)) throw std::invalid_argument("window bits out of range [5, 18]");
+    if (c != 0 && (c < 5 || c > 18) && !(mode == kMsmGlv && c == kMsmGlvOff))
+        throw std::invalid_argument("window bits out of range [5, 18]");
     if (seg != 0 && (seg > 16 || (seg & (seg - 1)) != 0))
         throw std::invalid_argument("segment must be 0 or a power of two in [1, 16]");
     std::lock_guard<std::mutex> g(g_table_mu);
@@ -355,7 +366,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
         if ((uint64_t)msm_table_rows(c) * prm.table_stride >= 0x80000000ull)
             throw std::invalid_argument("window table beyond 2^31 points");
     }
-    if (prm.g1 && !table && n <= (1u << 29)) {
+    if (prm.g1 && !table && n <= (1u << 29) && (c || msm_glv_enabled(curve, n))) {
         const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
             bits = 128;

@@ -144,11 +144,11 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
     out.ensure(2 * ob);
     uint8_t* d_ref = out.as<uint8_t>();
     uint8_t* d_out = d_ref + ob;
-    auto run = [&](int c, int seg, uint8_t* dst) {
+    auto run = [&](int c, int seg, uint8_t* dst, bool glv = true) {
         MsmParams prm;
         prm.c = c;
         prm.seg = seg;
-        prm.g1 = mode == kMsmGlv;
+        prm.g1 = glv && mode == kMsmGlv;
         msm_device(curve, in.pts.as<uint32_t>(), in.inf.as<uint8_t>(), in.sc.as<uint32_t>(), n, prm, e.ws, s, dst);
     };
     // built-in schedule: the reference output
@@ -190,6 +190,13 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
             best.seg = seg;
             best.ms = ms;
         }
+    }
+    if (mode == kMsmGlv) {
+        // the plain MSM under its own cell of this size class, kept when it beats the split
+        const float ms = time_ms([&] { run(0, 0, d_out, false); }, s);
+        check(0, 0);
+        if (verbose) std::fprintf(stderr, "msm curve=%d mode=%d 2^%d plain %.4f ms\n", curve, mode, lg, ms);
+        if (ms < best.ms) best = Choice{kMsmGlvOff, 0, ms};
     }
     return best;
 }
@@ -348,7 +355,7 @@ std::vector<Cell> parse_table(const std::string& path) {
             c.msm = true;
             ok = (bool)(ls >> c.a >> c.b >> c.lg >> c.x >> c.y) && !(ls >> extra);
             ok = ok && (c.a == 1 || c.a == 2) && c.b >= 0 && c.b < kMsmModes && c.lg >= 0 && c.lg < 32 &&
-                 (c.x == 0 || (c.x >= 5 && c.x <= 18)) &&
+                 (c.x == 0 || (c.x >= 5 && c.x <= 18) || (c.b == kMsmGlv && c.x == kMsmGlvOff)) &&
                  (c.y == 0 || (c.y > 0 && c.y <= 16 && (c.y & (c.y - 1)) == 0));
         } else if (kind == "ntt") {
             c.msm = false;
@@ -373,6 +380,7 @@ void save_table(const std::string& path) {
     std::ostringstream o;
     o << "# MSM and NTT schedules per size class k = floor(log2 n)\n"
       << "# msm <curve 1=BN254 2=BLS12-381> <mode 0 single 1 batched 2 GLV> <k> <window bits> <segment>\n"
+      << "# (a GLV cell of window bits 1: the plain MSM is faster at that size class)\n"
       << "# ntt <field 0=BLS12-381 Fr 1=BN254 Fr> <k> <log2 pass 1> <pass 2> <pass 3>\n";
     for (int curve = 1; curve <= 2; ++curve)
         for (int mode = 0; mode < kMsmModes; ++mode)

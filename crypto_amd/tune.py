@@ -2,7 +2,8 @@
 
 Schedules are chosen per size class k = floor(log2 n): for MSMs per curve and
 mode (single, batched job, GLV single over G1 points) the window bits c and
-the first-level bucket-reduction segment; for the Fr NTT per field the split
+the first-level bucket-reduction segment (a GLV cell of window bits 1: the
+plain MSM beat the split there and runs instead); for the Fr NTT per field the split
 of log2 n into one to three LDS passes.  A schedule changes speed only: the
 tuner refuses any candidate whose output differs from the built-in schedule's
 by a byte.  Tables persist as a text file (``save`` / ``load``); the library

@@ -113,7 +113,9 @@ int kinet_hip_msm_batch_run(kinet_msm_batch* batch, const void* d_scalars, const
  * points through the GLV split, kinet_hip_msm_prepared_g1) and hold the
  * window bits c and the first-level reduction segment; NTT cells are per
  * field and hold the pass split (log2 of the sub-transform lengths of passes
- * 1..3).  Schedules never change results, only speed.
+ * 1..3).  Schedules never change results, only speed.  A mode-2 cell of
+ * window bits 1 records that the plain MSM was faster at that size class:
+ * kinet_hip_msm_prepared_g1 called with window_bits 0 then runs it.
  *
  * Window bits only: c_by_log[k] for 2^k <= n < 2^(k+1), 0 = rule, count <= 32
  * (0 clears).  The rule is c = floor(log2 n) - 4 clamped to [5, 16] (of 2n
@@ -128,7 +130,8 @@ int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode);
  * output equals the built-in schedule's byte for byte (GPUKIT_ERR_DETERMINISM
  * otherwise), and installs the fastest.  Candidates: MSM window bits around
  * log2 n - 4, then reduction segments 16/8/4/2 for the best c (modes 0, 2;
- * batches of equal-size jobs totalling >= 2^20 points for mode 1); NTT every
+ * batches of equal-size jobs totalling >= 2^20 points for mode 1; for mode 2
+ * also the plain MSM under its mode-0 cell); NTT every
  * split of log2 n into one to three passes of at most 2^10.  Synchronous;
  * holds the library's device lock, so other calls wait.  Mode 2 needs
  * log_min >= 11 (smaller GLV MSMs run the plain path).  Sizes are limited to

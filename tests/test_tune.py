@@ -38,11 +38,25 @@ def test_load_save_clear(T, tmp_path):
     assert T.msm_cell(2, 2, 18) == (14, 4)
 
 
+def test_glv_off_cell(T, tmp_path):
+    """A GLV cell of window bits 1 records that the plain MSM wins at that size
+    class: the GLV mode then reports the plain MSM's window bits."""
+    p = tmp_path / "g.tune"
+    p.write_text("msm 2 0 22 15 16\nmsm 2 2 22 1 0\nmsm 2 2 21 16 8\n")
+    T.load(str(p))
+    assert T.msm_cell(2, 2, 22) == (1, 0)
+    assert T.window_bits(2, 1 << 22, T.MODE_GLV) == 15  # the plain cell
+    assert T.window_bits(2, 1 << 21, T.MODE_GLV) == 16
+    q = tmp_path / "h.tune"
+    T.save(str(q))
+    assert "msm 2 2 22 1 0" in q.read_text().splitlines()
+
+
 @pytest.mark.parametrize("line", ["msm 2 0 20 19 8", "msm 2 0 20 4 8", "msm 2 0 20 16 3", "msm 2 0 20 16 32",
                                   "msm 3 0 20 16 8", "msm 2 3 20 16 8", "msm 2 0 32 16 8", "msm 2 0 20 16",
                                   "msm 2 0 20 16 8 1", "ntt 0 20 10 11 0", "ntt 0 20 11 9 0", "ntt 0 20 0 20 0",
                                   "ntt 0 12 4 0 8", "ntt 2 12 6 6 0", "ntt 0 27 9 9 9", "ntx 0 12 6 6 0",
-                                  "msm two 0 20 16 8"])
+                                  "msm two 0 20 16 8", "msm 2 0 20 1 0", "msm 2 1 20 1 0"])
 def test_malformed_lines_install_nothing(T, tmp_path, line):
     import crypto_amd
 
