@@ -385,67 +385,150 @@ KZ_DEV bool fe_gt(const Fe<F>& a, const Fe<F>& b) {
 }
 
 // ------------------------------------------------------------ inversion
-// Variable-time binary extended Euclid (HAC 14.61 for an odd prime modulus)
-// on the plain integer value; used only on a handful of elements (final
-// normalisation), where its ~2*bits short iterations beat a Fermat chain of
-// ~1.5*bits multiplications in latency.
+// Bernstein-Yang divstep inversion ("Fast constant-time gcd computation and
+// modular inversion", 2019), variable time in the number of batches only.
+// State: f odd, g, eta = -delta (start f = p, g = A, eta = -1); one divstep:
+//   g even:          g <- g/2,                    eta <- eta - 1
+//   g odd, eta >= 0: g <- (g + f)/2,              eta <- eta - 1
+//   g odd, eta < 0:  (f, g) <- (g, (g - f)/2),    eta <- -eta - 1
+// g reaches 0 within (49 d + 80)/17 steps for d-bit inputs, and then f = +-1.
+// Steps depend only on the low bits of f and g, so batches of 29 run on the
+// low limbs alone and record the 2x2 matrix T with 2^29 (f', g') = T (f, g);
+// T is then applied to the full values (exact division by 2^29) and to the
+// coefficients (d, e), kept with d A = f, e A = g (mod p) and divided by
+// 2^29 mod p by adding the multiple of p that clears their low limb.
+// Multi-limb values are signed: limbs 0..N-2 in [0, 2^29), the top limb a
+// signed int32.  |T entries| <= 2^29, so every limb sum fits an int64.
 template <class F>
-KZ_DEV void mp_shr1(uint32_t* a) {
+KZ_DEV void dv_apply(int32_t u, int32_t v, int32_t q, int32_t r, int32_t* f, int32_t* g) {
+    int64_t cf = (int64_t)u * f[0] + (int64_t)v * g[0];
+    int64_t cg = (int64_t)q * f[0] + (int64_t)r * g[0];
+    cf >>= kLimbBits;  // low limbs are zero: exact division
+    cg >>= kLimbBits;
 #pragma unroll
-    for (int i = 0; i < F::N - 1; ++i) a[i] = (a[i] >> 1) | ((a[i + 1] & 1u) << (kLimbBits - 1));
-    a[F::N - 1] >>= 1;
-}
-
-template <class F>
-KZ_DEV bool mp_is_one(const uint32_t* a) {
-    uint32_t acc = a[0] ^ 1u;
-#pragma unroll
-    for (int i = 1; i < F::N; ++i) acc |= a[i];
-    return acc == 0;
-}
-
-// x <- x/2 mod p, x < p normalised.
-template <class F>
-KZ_DEV void fe_half_plain(uint32_t* x) {
-    if (x[0] & 1u) {
-        uint32_t carry = 0;
-#pragma unroll
-        for (int i = 0; i < F::N; ++i) {
-            const uint32_t t = x[i] + F::P[i] + carry;
-            carry = t >> kLimbBits;
-            x[i] = t & kLimbMask;
-        }
-        // x + p < 2^(29N): no carry out
+    for (int i = 1; i < F::N; ++i) {
+        cf += (int64_t)u * f[i] + (int64_t)v * g[i];
+        cg += (int64_t)q * f[i] + (int64_t)r * g[i];
+        f[i - 1] = (int32_t)(cf & kLimbMask);
+        g[i - 1] = (int32_t)(cg & kLimbMask);
+        cf >>= kLimbBits;
+        cg >>= kLimbBits;
     }
-    mp_shr1<F>(x);
+    f[F::N - 1] = (int32_t)cf;
+    g[F::N - 1] = (int32_t)cg;
+}
+
+// d <- d - p if d >= p (d signed, |d| < 2^(29N - 1))
+template <class F>
+KZ_DEV void dv_sub_p_if_ge(int32_t* d) {
+    int32_t t[F::N];
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N - 1; ++i) {
+        const int32_t x = d[i] - (int32_t)F::P[i] + c;
+        t[i] = x & (int32_t)kLimbMask;
+        c = x >> kLimbBits;
+    }
+    t[F::N - 1] = d[F::N - 1] - (int32_t)F::P[F::N - 1] + c;
+    const bool ge = t[F::N - 1] >= 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) d[i] = ge ? t[i] : d[i];
+}
+
+// (d, e) <- T (d, e) / 2^29 mod p; d, e in (-p, p) before and after.
+template <class F>
+KZ_DEV void dv_apply_mod(int32_t u, int32_t v, int32_t q, int32_t r, int32_t* d, int32_t* e) {
+    int64_t cd = (int64_t)u * d[0] + (int64_t)v * e[0];
+    int64_t ce = (int64_t)q * d[0] + (int64_t)r * e[0];
+    // md p clears the low limb: md = -cd p^-1 mod 2^29 (PINV = -p^-1)
+    const int64_t md = (int64_t)(((uint32_t)cd * F::PINV) & kLimbMask);
+    const int64_t me = (int64_t)(((uint32_t)ce * F::PINV) & kLimbMask);
+    cd = (cd + md * (int64_t)F::P[0]) >> kLimbBits;
+    ce = (ce + me * (int64_t)F::P[0]) >> kLimbBits;
+#pragma unroll
+    for (int i = 1; i < F::N; ++i) {
+        cd += (int64_t)u * d[i] + (int64_t)v * e[i] + md * (int64_t)F::P[i];
+        ce += (int64_t)q * d[i] + (int64_t)r * e[i] + me * (int64_t)F::P[i];
+        d[i - 1] = (int32_t)(cd & kLimbMask);
+        e[i - 1] = (int32_t)(ce & kLimbMask);
+        cd >>= kLimbBits;
+        ce >>= kLimbBits;
+    }
+    d[F::N - 1] = (int32_t)cd;
+    e[F::N - 1] = (int32_t)ce;
+    // |u d + v e| < 2^29 p and 0 <= md p < 2^29 p: now in (-p, 2p)
+    dv_sub_p_if_ge<F>(d);
+    dv_sub_p_if_ge<F>(e);
 }
 
 // Inverse of a Montgomery-form element (a != 0): a^-1 in Montgomery form.
 template <class F>
 KZ_DEV Fe<F> fe_inv(const Fe<F>& a) {
     if (fe_is_zero<F>(a)) return a;  // no inverse; callers never ask for it
-    Fe<F> u = a, v = fe_const<F>(F::P), x1 = fe_zero<F>(), x2 = fe_zero<F>();
-    x1.v[0] = 1;
-    while (!mp_is_one<F>(u.v) && !mp_is_one<F>(v.v)) {
-        while ((u.v[0] & 1u) == 0) {
-            mp_shr1<F>(u.v);
-            fe_half_plain<F>(x1.v);
-        }
-        while ((v.v[0] & 1u) == 0) {
-            mp_shr1<F>(v.v);
-            fe_half_plain<F>(x2.v);
-        }
-        if (!fe_gt<F>(v, u)) {  // u >= v
-            u = fe_sub<F>(u, v);  // no wrap: u - v >= 0
-            x1 = fe_sub<F>(x1, x2);
-        } else {
-            v = fe_sub<F>(v, u);
-            x2 = fe_sub<F>(x2, x1);
-        }
+    constexpr int kBatch = kLimbBits;
+    constexpr int kMaxBatches = ((49 * F::BITS + 80) / 17 + kBatch - 1) / kBatch + 1;
+    int32_t f[F::N], g[F::N], d[F::N], e[F::N];
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        f[i] = (int32_t)F::P[i];
+        g[i] = (int32_t)a.v[i];
+        d[i] = 0;
+        e[i] = 0;
     }
-    const Fe<F> r = mp_is_one<F>(u.v) ? x1 : x2;
-    // r = (a R)^-1 as a plain value; its Montgomery product with R^3 is a^-1 R.
-    return fe_mul<F>(r, fe_const<F>(F::R3));
+    e[0] = 1;
+    int32_t eta = -1;
+#pragma unroll 1
+    for (int b = 0; b < kMaxBatches; ++b) {
+        int32_t gz = 0;
+#pragma unroll
+        for (int i = 0; i < F::N; ++i) gz |= g[i];
+        if (gz == 0) break;
+        // 29 divsteps on the low limbs, branch-free
+        uint32_t fl = (uint32_t)f[0], gl = (uint32_t)g[0];
+        int32_t u = 1, v = 0, q = 0, r = 1;
+#pragma unroll 1
+        for (int i = 0; i < kBatch; ++i) {
+            const int32_t neg = eta >> 31;            // eta < 0
+            const int32_t odd = -(int32_t)(gl & 1u);  // g odd
+            // g odd: g += f (eta >= 0) or g -= f (eta < 0); rows likewise
+            gl += ((fl ^ (uint32_t)neg) - (uint32_t)neg) & (uint32_t)odd;
+            q += ((u ^ neg) - neg) & odd;
+            r += ((v ^ neg) - neg) & odd;
+            // swap (g odd, eta < 0): f <- old g = f + (g - f), eta <- -eta - 1
+            const int32_t sw = neg & odd;
+            eta = (eta ^ sw) - (sw + 1);
+            fl += gl & (uint32_t)sw;
+            u += q & sw;
+            v += r & sw;
+            gl >>= 1;
+            u *= 2;
+            v *= 2;
+        }
+        dv_apply<F>(u, v, q, r, f, g);
+        dv_apply_mod<F>(u, v, q, r, d, e);
+    }
+    // f = +-1: A^-1 = f d, d in (-p, p)
+    const int32_t fneg = f[F::N - 1] >> 31;
+    int32_t c = 0;
+    Fe<F> res;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t x = ((d[i] ^ fneg) - fneg) + c;  // limbwise negation, signed carries
+        const int32_t lim = i + 1 < F::N ? (x & (int32_t)kLimbMask) : x;
+        c = i + 1 < F::N ? (x >> kLimbBits) : 0;
+        d[i] = lim;
+    }
+    // now in (-p, p): add p if negative
+    const int32_t dneg = d[F::N - 1] >> 31;
+    c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t x = d[i] + ((int32_t)F::P[i] & dneg) + c;
+        res.v[i] = (uint32_t)(i + 1 < F::N ? (x & (int32_t)kLimbMask) : x);
+        c = x >> kLimbBits;
+    }
+    // res = (a R)^-1 as a plain value; its Montgomery product with R^3 is a^-1 R.
+    return fe_mul<F>(res, fe_const<F>(F::R3));
 }
 
 }  // namespace kz

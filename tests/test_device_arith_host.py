@@ -42,6 +42,26 @@ def test_field_ops(hc, field):
             assert int.from_bytes(out.raw, "little") == pow(a, p - 2, p), (field, a)
 
 
+@pytest.mark.parametrize("field", [0, 1, 2, 3])
+def test_field_inverse_many(hc, field):
+    """Divstep inversion against Fermat's a^(p-2) over random elements and
+    inputs whose gcd runs are long or short: powers of two, 2^k - 1, values
+    next to p and p/2, small integers."""
+    p, nb = FIELDS[field]
+    rnd = random.Random(700 + field)
+    vals = [1 << k for k in range(p.bit_length())] + [(1 << k) - 1 for k in range(2, p.bit_length())]
+    vals += [p - k for k in range(1, 40)] + [p // 2 + k for k in range(-5, 6)] + list(range(1, 60))
+    vals += [rnd.randrange(1, p) for _ in range(1500)]
+    out = ctypes.create_string_buffer(nb)
+    zero = bytes(nb)
+    for a in vals:
+        a %= p
+        if not a:
+            continue
+        hc.hc_field_op(field, 3, a.to_bytes(nb, "little"), zero, out)
+        assert int.from_bytes(out.raw, "little") == pow(a, p - 2, p), (field, a)
+
+
 @pytest.mark.parametrize("curve", [1, 2])
 def test_point_ops(hc, curve):
     pb = O.POINT_BYTES[curve]
