@@ -75,6 +75,11 @@ def glv_halves(scalars, curve: int = 2):
         k = v % r
         if curve == 2:
             q, rem = divmod(k, GLV_LAMBDA)
+            if q > (GLV_LAMBDA + 1) // 2:  # s - r, r = lambda^2 + lambda + 1
+                rem, q = rem - 1, q - GLV_LAMBDA - 1
+            if rem > GLV_LAMBDA // 2:
+                rem, q = rem - GLV_LAMBDA, q + 1
+            rem, q = abs(rem), abs(q)
         else:
             b1, b2 = (k * g1) >> 256, (k * g2) >> 256
             rem, q = abs(k - b1 * BN_A1 - b2 * BN_A2), abs(b1 * BN_B1 - b2 * BN_B2)

@@ -126,10 +126,114 @@ __host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32
 }
 }  // namespace glvbn
 
+// BLS12-381 GLV split: lambda = 0xac45a4010001a40200000000ffffffff with r =
+// lambda^2 + lambda + 1.  For s mod r = k1 + lambda k2 (k1 = s mod lambda,
+// k2 = s div lambda <= lambda + 1, both near 2^127.4), two exact moves make
+// both parts signed and below 2^127:
+//   k2 > (lambda + 1)/2:  (k1, k2) <- (k1 - 1, k2 - lambda - 1)   (s - r)
+//   k1 > lambda/2:        (k1, k2) <- (k1 - lambda, k2 + 1)
+// so |k1| <= lambda/2 and |k2| <= (lambda + 1)/2 + 1.  With 16-bit windows
+// the eight windows of a part are then full and the carry window next to
+// them is almost empty, instead of ~26% of the points carrying into it.
+// Writes |k1|, |k2| (4 LE words each) and their signs.  Host and device.
+namespace glvbls {
+constexpr uint32_t kLam[4] = {0xFFFFFFFFu, 0x00000000u, 0x0001A402u, 0xAC45A401u};
+constexpr uint32_t kLam1[4] = {0x00000000u, 0x00000001u, 0x0001A402u, 0xAC45A401u};  // lambda + 1
+constexpr uint32_t kHalf[4] = {0x7FFFFFFFu, 0x00000000u, 0x8000D201u, 0x5622D200u};  // floor(lambda / 2)
+constexpr uint32_t kHalf1[4] = {0x80000000u, 0x00000000u, 0x8000D201u, 0x5622D200u}; // (lambda + 1) / 2
+constexpr uint32_t kMu[5] = {0xF6CFEE30u, 0x63F6E522u, 0xE01FAADDu, 0x7C6BECF1u, 0x00000001u};  // 2^256 div lambda
+constexpr uint32_t kR[8] = {0x00000001u, 0xFFFFFFFFu, 0xFFFE5BFEu, 0x53BDA402u,
+                            0x09A1D805u, 0x3339D808u, 0x299D7D48u, 0x73EDA753u};
+
+// a > b over K words
+template <int K>
+__host__ __device__ inline bool gt(const uint32_t* a, const uint32_t* b) {
+    for (int i = K - 1; i >= 0; --i)
+        if (a[i] != b[i]) return a[i] > b[i];
+    return false;
+}
+
+// a += sign * b over 8 words mod 2^256 (b: K words, zero-extended)
+template <int K>
+__host__ __device__ inline void addsub8(uint32_t* a, const uint32_t* b, bool subtract) {
+    uint64_t c = subtract ? 1u : 0u;
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t w = i < K ? b[i] : 0u;
+        const uint64_t t = (uint64_t)a[i] + (subtract ? ~w : w) + c;
+        a[i] = (uint32_t)t;
+        c = t >> 32;
+    }
+}
+
+__host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32_t* k2, bool& neg1, bool& neg2) {
+    uint32_t s[8];
+    for (int i = 0; i < 8; ++i) s[i] = s_in[i];
+    for (int rep = 0; rep < 2; ++rep)  // 2^256 < 3 r
+        if (!gt<8>(kR, s)) addsub8<8>(s, kR, true);
+    // q = floor(s mu / 2^256) <= s div lambda <= q + 2
+    uint32_t prod[13];
+    for (int i = 0; i < 13; ++i) prod[i] = 0;
+    for (int a = 0; a < 8; ++a) {
+        uint64_t carry = 0;
+        for (int b = 0; b < 5; ++b) {
+            const uint64_t t = (uint64_t)s[a] * kMu[b] + prod[a + b] + carry;
+            prod[a + b] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[a + 5] = (uint32_t)carry;
+    }
+    uint32_t q[8] = {prod[8], prod[9], prod[10], prod[11], prod[12], 0, 0, 0};
+    // s <- s - q lambda, then at most two more subtractions of lambda
+    uint32_t ql[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 5; ++a) {
+        uint64_t carry = 0;
+        for (int b = 0; b < 4 && a + b < 8; ++b) {
+            const uint64_t t = (uint64_t)q[a] * kLam[b] + ql[a + b] + carry;
+            ql[a + b] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        if (a + 4 < 8) ql[a + 4] = (uint32_t)carry;
+    }
+    addsub8<8>(s, ql, true);
+    const uint32_t one[1] = {1u};
+    const uint32_t lam8[8] = {kLam[0], kLam[1], kLam[2], kLam[3], 0, 0, 0, 0};
+    for (int rep = 0; rep < 2; ++rep)
+        if (!gt<8>(lam8, s)) {
+            addsub8<4>(s, kLam, true);
+            addsub8<1>(q, one, false);
+        }
+    // k1 = s (< lambda), k2 = q (<= lambda + 1), as 8-word two's complement
+    if (gt<4>(q, kHalf1) || q[4] != 0) {
+        addsub8<1>(s, one, true);
+        addsub8<4>(q, kLam1, true);
+    }
+    if (!(s[7] >> 31) && gt<4>(s, kHalf)) {  // s >= 0 here is below 2^128
+        addsub8<4>(s, kLam, true);
+        addsub8<1>(q, one, false);
+    }
+    neg1 = (s[7] >> 31) != 0;
+    neg2 = (q[7] >> 31) != 0;
+    if (neg1) {
+        uint32_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        addsub8<8>(t, s, true);
+        for (int i = 0; i < 8; ++i) s[i] = t[i];
+    }
+    if (neg2) {
+        uint32_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        addsub8<8>(t, q, true);
+        for (int i = 0; i < 8; ++i) q[i] = t[i];
+    }
+    for (int i = 0; i < 4; ++i) {
+        k1[i] = s[i];
+        k2[i] = q[i];
+    }
+}
+}  // namespace glvbls
+
 // d_pts2 = [P_i, phi(P_i)] (2n prepared records), d_inf2 likewise, d_scalars2
-// = the two 128-bit parts (32-byte records) of s_i = s1_i + lambda s2_i mod r.
-// BLS12-381: s1 = s mod r mod lambda, s2 = s mod r div lambda.  BN254: the
-// signed lattice split above, the signs folded into the points (-P, -phi(P)).
+// = the two 128-bit parts (32-byte records) of s_i = s1_i + lambda s2_i mod r:
+// the signed splits above (magnitudes below 2^127), the signs folded into the
+// points (-P, -phi(P)).
 // Valid only for points of G1 (all curve points for BN254).
 void glv_expand(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 uint32_t* d_pts2, uint8_t* d_inf2, uint32_t* d_scalars2, hipStream_t s);

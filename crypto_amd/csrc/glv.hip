@@ -5,9 +5,9 @@
 // of unity in Fp, acting on G1 as multiplication by lambda =
 // 0xac45a4010001a40200000000ffffffff (a cube root of unity mod r, 128 bits;
 // the pair (beta, lambda) below was matched by checking phi(G) = lambda G).
-// For P in G1 and any 256-bit s, with s' = s mod r = s1 + lambda s2
-// (s1 = s' mod lambda, s2 = s' div lambda, both < 2^128):
-//     s P = s1 P + s2 phi(P),
+// For P in G1 and any 256-bit s, with s = k1 + lambda k2 mod r (the signed
+// split of glv.h, |k1|, |k2| < 2^127):
+//     s P = |k1| (+-P) + |k2| (+-phi(P)),
 // so an n-point MSM with 256-bit scalars becomes a 2n-point MSM with 128-bit
 // scalars: half the windows, hence half the bucket reduction and half the
 // Horner doublings of the window combine, for the same number of additions.
@@ -17,6 +17,8 @@
 // curves holds exactly for the points of G1 (Scott, "A note on group
 // membership tests for G1, G2 and GT on BLS pairing-friendly curves", 2021).
 // BN254 G1 has cofactor 1: its check is the curve equation alone.
+#include <type_traits>
+
 #include "curve.h"
 #include "engine.h"
 #include "glv.h"
@@ -35,124 +37,15 @@ constexpr uint32_t kBetaMontBn[9] = {0x18CCB791u, 0x175B1C3Au, 0x0B83D6E2u, 0x0E
 constexpr uint32_t kBetaMont[14] = {0x1195DFEBu, 0x1B04E484u, 0x06026044u, 0x086070A2u, 0x1FD68858u,
                                     0x137E9670u, 0x06871E67u, 0x1E736664u, 0x083B24F6u, 0x08A70373u,
                                     0x02A012FDu, 0x0112F94Bu, 0x18A2733Cu, 0x00000003u};
-// lambda, floor(2^256 / lambda) (129 bits) and r as 32-bit LE limbs
-constexpr uint32_t kLambda[4] = {0xFFFFFFFFu, 0x00000000u, 0x0001A402u, 0xAC45A401u};
-constexpr uint32_t kMu[5] = {0xF6CFEE30u, 0x63F6E522u, 0xE01FAADDu, 0x7C6BECF1u, 0x00000001u};
-constexpr uint32_t kR[8] = {0x00000001u, 0xFFFFFFFFu, 0xFFFE5BFEu, 0x53BDA402u,
-                            0x09A1D805u, 0x3339D808u, 0x299D7D48u, 0x73EDA753u};
-
-// a >= b over K limbs
-template <int K>
-__device__ __forceinline__ bool geq(const uint32_t* a, const uint32_t* b) {
-#pragma unroll
-    for (int i = K - 1; i >= 0; --i)
-        if (a[i] != b[i]) return a[i] > b[i];
-    return true;
-}
-
-// a -= b over K limbs (a >= b)
-template <int K>
-__device__ __forceinline__ void sub_in(uint32_t* a, const uint32_t* b) {
-    uint64_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const uint64_t t = (uint64_t)a[i] - b[i] - borrow;
-        a[i] = (uint32_t)t;
-        borrow = (t >> 32) & 1u;
-    }
-}
-
 }  // namespace
 
-// out[i] = s1_i, out[n + i] = s2_i (32-byte records, upper 16 bytes zero).
+// Signed GLV split (glv.h): out_pts = [+-P_i, +-phi(P_i)], out_inf likewise,
+// out_sc = [|k1_i|, |k2_i|] (32-byte records, upper half zero).
+template <int CURVE>
 __global__ void __launch_bounds__(256)
-k_glv_split_bls(const uint32_t* __restrict__ scalars, uint32_t n, uint32_t* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s[8];
-    {
-        const uint4* q = reinterpret_cast<const uint4*>(scalars + (size_t)i * 8);
-        const uint4 a = q[0], b = q[1];
-        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
-        s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
-    }
-    // s < 2^256 < 3r
-    if (geq<8>(s, kR)) sub_in<8>(s, kR);
-    if (geq<8>(s, kR)) sub_in<8>(s, kR);
-    // q = floor(s mu / 2^256): limbs 8..11 of the 13-limb product (q < 2^128)
-    uint32_t prod[13];
-#pragma unroll
-    for (int k = 0; k < 13; ++k) prod[k] = 0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        uint64_t carry = 0;
-#pragma unroll
-        for (int b = 0; b < 5; ++b) {
-            const uint64_t t = (uint64_t)s[a] * kMu[b] + prod[a + b] + carry;
-            prod[a + b] = (uint32_t)t;
-            carry = t >> 32;
-        }
-        prod[a + 5] = (uint32_t)carry;
-    }
-    uint32_t q[4] = {prod[8], prod[9], prod[10], prod[11]};
-    // rem = s - q lambda  (0 <= rem < 3 lambda)
-    uint32_t ql[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ql[k] = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        uint64_t carry = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint64_t t = (uint64_t)q[a] * kLambda[b] + ql[a + b] + carry;
-            ql[a + b] = (uint32_t)t;
-            carry = t >> 32;
-        }
-        ql[a + 4] = (uint32_t)carry;
-    }
-    sub_in<8>(s, ql);  // s now holds rem (< 2^130)
-    const uint32_t lam8[8] = {kLambda[0], kLambda[1], kLambda[2], kLambda[3], 0, 0, 0, 0};
-#pragma unroll 1
-    for (int k = 0; k < 2 && geq<8>(s, lam8); ++k) {
-        sub_in<8>(s, lam8);
-        uint64_t c = 1;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const uint64_t v = (uint64_t)q[t] + c;
-            q[t] = (uint32_t)v;
-            c = v >> 32;
-        }
-    }
-    uint4* o1 = reinterpret_cast<uint4*>(out + (size_t)i * 8);
-    uint4* o2 = reinterpret_cast<uint4*>(out + ((size_t)n + i) * 8);
-    o1[0] = make_uint4(s[0], s[1], s[2], s[3]);
-    o1[1] = make_uint4(0, 0, 0, 0);
-    o2[0] = make_uint4(q[0], q[1], q[2], q[3]);
-    o2[1] = make_uint4(0, 0, 0, 0);
-}
-
-// out_pts = [P_0 .. P_{n-1}, phi(P_0) .. phi(P_{n-1})], out_inf likewise.
-__global__ void __launch_bounds__(256)
-k_glv_points_bls(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, uint32_t n,
-                 uint32_t* __restrict__ out_pts, uint8_t* __restrict__ out_inf) {
-    using F = BlsFp;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Affine<F> a = affine_load<F>(pts, i);
-    affine_store<F>(out_pts, i, a);
-    a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMont));
-    affine_store<F>(out_pts, (size_t)n + i, a);
-    const uint8_t f = inf ? inf[i] : 0;
-    out_inf[i] = f;
-    out_inf[(size_t)n + i] = f;
-}
-
-// BN254 (signed split, glv.h): out_pts = [+-P_i, +-phi(P_i)], out_inf
-// likewise, out_sc = [|k1_i|, |k2_i|] (32-byte records, upper half zero).
-__global__ void __launch_bounds__(256)
-k_glv_expand_bn(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, const uint32_t* __restrict__ scalars,
-                uint32_t n, uint32_t* __restrict__ out_pts, uint8_t* __restrict__ out_inf, uint32_t* __restrict__ out_sc) {
-    using F = BnFp;
+k_glv_expand(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, const uint32_t* __restrict__ scalars,
+             uint32_t n, uint32_t* __restrict__ out_pts, uint8_t* __restrict__ out_inf, uint32_t* __restrict__ out_sc) {
+    using F = std::conditional_t<CURVE == 1, BnFp, BlsFp>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t s[8];
@@ -164,12 +57,14 @@ k_glv_expand_bn(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ in
     }
     uint32_t k1[4], k2[4];
     bool neg1, neg2;
-    glvbn::split(s, k1, k2, neg1, neg2);
+    if constexpr (CURVE == 1) glvbn::split(s, k1, k2, neg1, neg2);
+    else glvbls::split(s, k1, k2, neg1, neg2);
     Affine<F> a = affine_load<F>(pts, i);
     const Fe<F> y = a.y, ny = fe_neg<F>(a.y);
     a.y = neg1 ? ny : y;
     affine_store<F>(out_pts, i, a);
-    a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontBn));
+    if constexpr (CURVE == 1) a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontBn));
+    else a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMont));
     a.y = neg2 ? ny : y;
     affine_store<F>(out_pts, (size_t)n + i, a);
     const uint8_t f = inf ? inf[i] : 0;
@@ -205,7 +100,7 @@ k_check_g1(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, ui
 #pragma unroll 1
         for (int bit = 126; bit >= 0; --bit) {
             acc = xyzz_dbl<F, MulCall>(acc);
-            if ((kLambda[bit >> 5] >> (bit & 31)) & 1u) xyzz_madd<F, MulCall>(acc, a);
+            if ((glvbls::kLam[bit >> 5] >> (bit & 31)) & 1u) xyzz_madd<F, MulCall>(acc, a);
         }
         const Fe<F> bx = fe_mul<F>(a.x, fe_const<F>(kBetaMont));
         ok = !xyzz_is_inf<F>(acc) && fe_eq<F>(acc.X, fe_mul<F>(bx, acc.ZZ)) && fe_eq<F>(acc.Y, fe_mul<F>(a.y, acc.ZZZ));
@@ -216,15 +111,14 @@ k_check_g1(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, ui
 void glv_expand(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 uint32_t* d_pts2, uint8_t* d_inf2, uint32_t* d_scalars2, hipStream_t s) {
     const unsigned grid = ceil_div(n, 256);
-    if (curve == 1) {
-        hipLaunchKernelGGL(k_glv_expand_bn, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
+    if (curve == 1)
+        hipLaunchKernelGGL(k_glv_expand<1>, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
                            d_inf2, d_scalars2);
-        KZ_HIP(hipGetLastError());
-        return;
-    }
-    if (curve != 2) throw std::invalid_argument("GLV split: unknown curve");
-    hipLaunchKernelGGL(k_glv_points_bls, dim3(grid), dim3(256), 0, s, d_pts, d_inf, (uint32_t)n, d_pts2, d_inf2);
-    hipLaunchKernelGGL(k_glv_split_bls, dim3(grid), dim3(256), 0, s, d_scalars, (uint32_t)n, d_scalars2);
+    else if (curve == 2)
+        hipLaunchKernelGGL(k_glv_expand<2>, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
+                           d_inf2, d_scalars2);
+    else
+        throw std::invalid_argument("GLV split: unknown curve");
     KZ_HIP(hipGetLastError());
 }
 

@@ -186,7 +186,7 @@ Choice tune_single(Engine& e, int curve, int mode, int lg, const MsmInputs& in, 
         check(best.c, seg);
         if (verbose)
             std::fprintf(stderr, "msm curve=%d mode=%d 2^%d c=%d seg=%d %.4f ms\n", curve, mode, lg, best.c, seg, ms);
-        if (best.seg == 0 || ms < best.ms) {
+        if (ms < best.ms) {  // best.seg 0: the built-in rule, kept when fastest
             best.seg = seg;
             best.ms = ms;
         }

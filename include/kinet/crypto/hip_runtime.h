@@ -40,9 +40,10 @@ int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_
  * checked once by kinet_hip_points_check_g1, or generated as multiples of the
  * generator; for BN254, cofactor 1, every curve point).  It then uses the GLV
  * endomorphism phi(x, y) = (beta x, y) = lambda (x, y): each 256-bit scalar is
- * split as s = k1 + lambda k2 mod r with parts below 2^128 (BLS12-381: s mod r
- * divided by the 128-bit lambda; BN254: a signed split over a short lattice
- * basis, the signs moved onto the points), giving a 2n-point MSM over half
+ * split as s = k1 + lambda k2 mod r with |k1|, |k2| < 2^127 (BLS12-381: s mod
+ * r divided by the 128-bit lambda, then moved by r = lambda^2 + lambda + 1
+ * and by lambda to signed parts; BN254: a signed split over a short lattice
+ * basis), the signs moved onto the points, giving a 2n-point MSM over half
  * the windows.  Results equal kinet_hip_msm_prepared's for points of G1; for
  * other points they are unspecified. */
 int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
@@ -129,9 +130,9 @@ int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode);
  * generator, seeded scalars / field elements), checks that every candidate's
  * output equals the built-in schedule's byte for byte (GPUKIT_ERR_DETERMINISM
  * otherwise), and installs the fastest.  Candidates: MSM window bits around
- * log2 n - 4, then reduction segments 16/8/4/2 for the best c (modes 0, 2;
- * batches of equal-size jobs totalling >= 2^20 points for mode 1; for mode 2
- * also the plain MSM under its mode-0 cell); NTT every
+ * log2 n - 4, then reduction segments 16/8/4/2 against the built-in rule for
+ * the best c (modes 0, 2; batches of equal-size jobs totalling >= 2^20 points
+ * for mode 1; for mode 2 also the plain MSM under its mode-0 cell); NTT every
  * split of log2 n into one to three passes of at most 2^10.  Synchronous;
  * holds the library's device lock, so other calls wait.  Mode 2 needs
  * log_min >= 11 (smaller GLV MSMs run the plain path).  Sizes are limited to

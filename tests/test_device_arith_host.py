@@ -112,3 +112,38 @@ def test_glv_split_bn(hc):
         k2 = int.from_bytes(raw[16:32], "little") * (-1 if raw[33] else 1)
         assert abs(k1) < 1 << 127 and abs(k2) < 1 << 127, hex(s)
         assert (k1 + BN_LAMBDA * k2 - s) % r == 0, hex(s)
+
+
+
+def test_glv_split_bls(hc):
+    """The BLS12-381 signed GLV split (host build): s = k1 + lambda k2 mod r with
+    |k1|, |k2| < 2^127 (the magnitude bound that keeps 16-bit windows free of
+    carries into a ninth window), for random scalars, for the boundaries of
+    both moves (k2 near (lambda + 1)/2, k1 near lambda/2, s = lambda^2 +
+    lambda) and for scalars of 2^256 - 1 and multiples of r; r = lambda^2 +
+    lambda + 1 and lambda G = phi(G) pin the constants."""
+    lam = 0xAC45A4010001A40200000000FFFFFFFF
+    r = M.CURVES[2]["r"]
+    p = M.CURVES[2]["p"]
+    assert r == lam * lam + lam + 1
+    G = M.CURVES[2]["g"]
+    lg = M.mul(2, G, lam)
+    assert lg[1] == G[1] and pow(lg[0] * pow(G[0], -1, p) % p, 3, p) == 1
+    rnd = random.Random(6)
+    out = ctypes.create_string_buffer(34)
+    h1, h = (lam + 1) // 2, lam // 2
+    edge = [0, 1, 2, r - 1, r, r + 1, (1 << 256) - 1, lam, lam - 1, lam + 1, lam * lam + lam, lam * lam + lam - 1,
+            h, h + 1, h - 1, h1 * lam, (h1 + 1) * lam, (h1 + 1) * lam - 1, (h1 + 1) * lam + h, (h1 + 1) * lam + h + 1,
+            h1 * lam + h + 1, 2 * r, 3 * r - 1, 2 * r + lam]
+    edge += [q * lam + k for q in (0, 1, h1 - 1, h1, h1 + 1, lam - 1, lam) for k in (0, 1, h - 1, h, h + 1, lam - 1)]
+    big = max_bits = 0
+    for s in edge + [rnd.getrandbits(256) for _ in range(4000)] + [rnd.randrange(r) for _ in range(2000)]:
+        s %= 1 << 256
+        hc.hc_glv_split_bls(s.to_bytes(32, "little"), out)
+        raw = out.raw
+        k1 = int.from_bytes(raw[:16], "little") * (-1 if raw[32] else 1)
+        k2 = int.from_bytes(raw[16:32], "little") * (-1 if raw[33] else 1)
+        assert abs(k1) <= h and abs(k2) <= h1 + 1, hex(s)
+        assert (k1 + lam * k2 - s) % r == 0, hex(s)
+        max_bits = max(max_bits, abs(k1).bit_length(), abs(k2).bit_length())
+    assert max_bits <= 127
