@@ -220,9 +220,14 @@ bool lds_sort_supported(int c, size_t n) { return c >= 5 && c <= 16 && n >= 4096
 void lds_sort_shape(int c, int bits, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part) {
     const uint32_t W = (uint32_t)msm_num_windows(c, bits);
     wpad = (W + 7) / 8 * 8;
-    // about two workgroups per CU over the whole launch, parts of >= 4096 points
+    // parts of >= 4096 points, doubled while the launch has fewer than 128
+    // (window, part) workgroups.  Fewer, longer parts write each bucket's
+    // entries in longer runs: at 2^21 GLV points and 16-bit windows, 64 parts
+    // per window made the scattered 4-byte entry writes cost about 7x their
+    // bytes in HBM writes and the sort took 0.63 ms; 16 parts took 0.46 ms,
+    // 8 parts 0.73 ms (too few workgroups for the histogram passes).
     uint32_t parts = 1;
-    while (parts * W < 512 && ((size_t)parts * 2) * 4096 <= n) parts *= 2;
+    while (parts * W < 128 && ((size_t)parts * 2) * 4096 <= n) parts *= 2;
     nparts = parts;
     per_part = (uint32_t)((n + parts - 1) / parts);
 }
