@@ -106,6 +106,20 @@ struct MulCall {
     }
 };
 
+// Products without the final subtraction: operands and results in [0, 2p)
+// (the lazy-reduction section of field.h).  Only for chains built from the
+// lazy helpers, such as xyzz_madd_lazy.
+struct MulLazy {
+    template <class F>
+    static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
+        return fe_mul_k<F, KZ_MUL_ILP, false>(a, b);
+    }
+    template <class F>
+    static KZ_DEV Fe<F> sqr(const Fe<F>& a) {
+        return fe_sqr<F, false>(a);
+    }
+};
+
 template <class F>
 KZ_DEV Xyzz<F> xyzz_inf() {
     Xyzz<F> r;
@@ -199,6 +213,48 @@ KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
     acc.Y = Y3;
     acc.ZZ = M::template mul<F>(acc.ZZ, PP);
     acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
+}
+
+// xyzz_madd with lazily reduced coordinates: acc's coordinates lie in
+// [0, 2p) (canonical ones included), q is canonical, and the result's lie in
+// [0, 2p) again; xyzz_canon makes them canonical.  The same point as
+// xyzz_madd.  The identity is only ever the canonical xyzz_inf (ZZ is a
+// product of non-zero values otherwise, never 0 or p), so xyzz_is_inf
+// applies unchanged; the doubling case runs canonically on q.
+template <class F>
+KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
+    using M = MulLazy;
+    if (xyzz_is_inf<F>(acc)) {
+        acc = xyzz_from_affine<F>(q);
+        return;
+    }
+    Fe<F> U2 = M::template mul<F>(q.x, acc.ZZ);
+    Fe<F> S2 = M::template mul<F>(q.y, acc.ZZZ);
+    Fe<F> P = fe_sub_lazy<F>(U2, acc.X);
+    Fe<F> R = fe_sub_lazy<F>(S2, acc.Y);
+    if (fe_is_zero_lazy<F>(P)) {
+        if (fe_is_zero_lazy<F>(R)) acc = xyzz_dbl_affine<F, MulInline>(q);
+        else acc = xyzz_inf<F>();
+        return;
+    }
+    Fe<F> PP = M::template sqr<F>(P);
+    Fe<F> PPP = M::template mul<F>(P, PP);
+    Fe<F> Q = M::template mul<F>(acc.X, PP);
+    Fe<F> X3 = fe_sub_lazy<F>(fe_sub_lazy<F>(M::template sqr<F>(R), PPP), fe_dbl_lazy<F>(Q));
+    Fe<F> Y3 = fe_sub_lazy<F>(M::template mul<F>(R, fe_sub_lazy<F>(Q, X3)), M::template mul<F>(acc.Y, PPP));
+    acc.X = X3;
+    acc.Y = Y3;
+    acc.ZZ = M::template mul<F>(acc.ZZ, PP);
+    acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
+}
+
+template <class F>
+KZ_DEV Xyzz<F> xyzz_canon(Xyzz<F> p) {
+    fe_reduce_once<F>(p.X);
+    fe_reduce_once<F>(p.Y);
+    fe_reduce_once<F>(p.ZZ);
+    fe_reduce_once<F>(p.ZZZ);
+    return p;
 }
 
 // a + b, general (add-2008-s), with the doubling and inverse special cases.

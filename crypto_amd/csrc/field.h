@@ -215,7 +215,7 @@ KZ_DEV Fe<F> fe_neg(const Fe<F>& a) {
 // chain of v_mad_u64_u32 stalls on the instruction's latency whenever few
 // waves share a SIMD.  Each accumulator holds at most N products of 29-bit
 // limbs plus the column carry, far below 2^64.
-template <class F, int K>
+template <class F, int K, bool Canon = true>
 KZ_DEV Fe<F> fe_mul_k(const Fe<F>& a, const Fe<F>& b) {
     constexpr int N = F::N;
     uint32_t m[N];
@@ -257,7 +257,7 @@ KZ_DEV Fe<F> fe_mul_k(const Fe<F>& a, const Fe<F>& b) {
         for (int r = 1; r < K; ++r) acc[r] = 0;
     }
     t.v[N - 1] = (uint32_t)acc[0];  // < 2p < 2^(29N): fits the top limb
-    fe_reduce_once<F>(t);
+    if (Canon) fe_reduce_once<F>(t);
     return t;
 }
 
@@ -276,7 +276,7 @@ KZ_DEV Fe<F> fe_mul(const Fe<F>& a, const Fe<F>& b) {
 // N^2 (105 instead of 196 for BLS12-381 Fp), with no extra work per column.
 // A column holds at most N/2 cross products below 2^59, one square below 2^58
 // and N reduction products below 2^58: below 2^63.
-template <class F>
+template <class F, bool Canon = true>
 KZ_DEV Fe<F> fe_sqr(const Fe<F>& a) {
     constexpr int N = F::N;
     uint32_t m[N], d[N];
@@ -306,8 +306,83 @@ KZ_DEV Fe<F> fe_sqr(const Fe<F>& a) {
         acc >>= kLimbBits;
     }
     t.v[N - 1] = (uint32_t)acc;  // < 2p < 2^(29N): fits the top limb
-    fe_reduce_once<F>(t);
+    if (Canon) fe_reduce_once<F>(t);
     return t;
+}
+
+// ------------------------------------------------------- lazy reduction
+// Values in [0, 2p) instead of [0, p), for chains of operations that never
+// compare or store them.  The Montgomery product and square without their
+// final subtraction (fe_mul_k<F, K, false>, fe_sqr<F, false>) map inputs
+// below 2p to outputs below p + 4p^2/R < 2p (R = 2^(29N) > 2^25 p for the
+// base fields), and their 64-bit column bounds hold unchanged since limbs
+// stay below 2^29.  Addition chains correct by 2p (4p < 2^(29N)).  A lazy
+// value is zero mod p exactly when it is 0 or p; fe_reduce_once maps it to
+// the canonical value.
+template <class F>
+constexpr uint32_t two_p_limb(int i) {
+    uint32_t c = 0, t = 0;
+    for (int k = 0; k <= i; ++k) {
+        t = 2 * F::P[k] + c;
+        c = t >> kLimbBits;
+    }
+    return t & kLimbMask;
+}
+
+// a - b for a, b in [0, 2p): result in [0, 2p)
+template <class F>
+KZ_DEV Fe<F> fe_sub_lazy(const Fe<F>& a, const Fe<F>& b) {
+    Fe<F> d;
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = a.v[i] - b.v[i] - borrow;
+        borrow = t >> 31;
+        d.v[i] = t & kLimbMask;
+    }
+    const uint32_t mask = 0u - borrow;  // a - b in (-2p, 0): add 2p
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = d.v[i] + (two_p_limb<F>(i) & mask) + carry;
+        carry = t >> kLimbBits;
+        d.v[i] = t & kLimbMask;
+    }
+    return d;
+}
+
+// 2a for a in [0, 2p): result in [0, 2p)
+template <class F>
+KZ_DEV Fe<F> fe_dbl_lazy(const Fe<F>& a) {
+    Fe<F> s, d;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = (a.v[i] << 1) + carry;
+        carry = t >> kLimbBits;
+        s.v[i] = t & kLimbMask;
+    }
+    uint32_t borrow = 0;  // 2a < 4p < 2^(29N): subtract 2p unless it borrows
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = s.v[i] - two_p_limb<F>(i) - borrow;
+        borrow = t >> 31;
+        d.v[i] = t & kLimbMask;
+    }
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) s.v[i] = borrow ? s.v[i] : d.v[i];
+    return s;
+}
+
+template <class F>
+KZ_DEV bool fe_is_zero_lazy(const Fe<F>& a) {
+    uint32_t z = 0, e = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        z |= a.v[i];
+        e |= a.v[i] ^ F::P[i];
+    }
+    return z == 0 || e == 0;
 }
 
 template <class F>

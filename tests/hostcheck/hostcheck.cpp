@@ -75,6 +75,14 @@ void point_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
         qa.y = q.Y;
         r = p;
         xyzz_madd<F>(r, qa);
+    } else if (op == 3) {  // p + 3 q through three lazily reduced mixed additions
+        Xyzz<F> q = read_wire<C>(b);
+        Affine<F> qa;
+        qa.x = q.X;
+        qa.y = q.Y;
+        r = p;
+        for (int k = 0; k < 3; ++k) xyzz_madd_lazy<F>(r, qa);
+        r = xyzz_canon<F>(r);
     } else {
         r = xyzz_dbl<F>(p);
     }

@@ -63,6 +63,32 @@ def test_field_inverse_many(hc, field):
 
 
 @pytest.mark.parametrize("curve", [1, 2])
+def test_lazy_mixed_addition_chain(hc, curve):
+    """Three mixed additions with coordinates kept in [0, 2p) between them
+    (the accumulation kernel's form), then made canonical: P + 3Q for random
+    points, for Q = P (the doubling case on the first step), Q = -P (the sum
+    passes through the identity), P = -2Q and P = -3Q (identity mid-chain and
+    at the end) and P the identity."""
+    pb = O.POINT_BYTES[curve]
+    rnd = random.Random(40 + curve)
+    out = ctypes.create_string_buffer(pb)
+    G = M.CURVES[curve]["g"]
+    r = M.CURVES[curve]["r"]
+    cases = []
+    for _ in range(40):
+        cases.append((rnd.randrange(1, r), rnd.randrange(1, r)))
+    q = rnd.randrange(1, r)
+    cases += [(q, q), (r - q, q), ((-2 * q) % r, q), ((-3 * q) % r, q), (0, q), ((-1 * q) % r, q)]
+    for kp, kq in cases:
+        P = M.mul(curve, G, kp) if kp else None
+        Q = M.mul(curve, G, kq)
+        Pw = M.to_wire(curve, P) if P is not None else bytes(pb)
+        hc.hc_point_op(curve, 3, Pw, M.to_wire(curve, Q), out)
+        want = (kp + 3 * kq) % r
+        assert out.raw == (M.to_wire(curve, M.mul(curve, G, want)) if want else bytes(pb)), (kp, kq)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
 def test_point_ops(hc, curve):
     pb = O.POINT_BYTES[curve]
     rnd = random.Random(curve)
