@@ -50,7 +50,7 @@ struct Xyzz {
 // (limbs past N are zero), which the calling convention keeps in VGPRs, and
 // returns the product in VGPRs: no stack traffic.  (Passed as two Fe structs,
 // the second one would go through the stack as a byval copy in scratch.)
-template <class F>
+template <class F, bool Canon = true>
 __host__ __device__ __noinline__ Fe<F> fe_mul_flat(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t a4,
                                                   uint32_t a5, uint32_t a6, uint32_t a7, uint32_t a8, uint32_t a9,
                                                   uint32_t a10, uint32_t a11, uint32_t a12, uint32_t a13,
@@ -64,7 +64,7 @@ __host__ __device__ __noinline__ Fe<F> fe_mul_flat(uint32_t a0, uint32_t a1, uin
         a.v[i] = av[i];
         b.v[i] = bv[i];
     }
-    return fe_mul<F>(a, b);
+    return fe_mul_k<F, KZ_MUL_ILP, Canon>(a, b);
 }
 
 template <int I, class F>
@@ -73,9 +73,9 @@ KZ_DEV uint32_t limb_or_zero(const Fe<F>& a) {
     else return 0u;
 }
 
-template <class F>
+template <class F, bool Canon = true>
 KZ_DEV Fe<F> fe_mul_call(const Fe<F>& a, const Fe<F>& b) {
-    return fe_mul_flat<F>(limb_or_zero<0>(a), limb_or_zero<1>(a), limb_or_zero<2>(a), limb_or_zero<3>(a),
+    return fe_mul_flat<F, Canon>(limb_or_zero<0>(a), limb_or_zero<1>(a), limb_or_zero<2>(a), limb_or_zero<3>(a),
                           limb_or_zero<4>(a), limb_or_zero<5>(a), limb_or_zero<6>(a), limb_or_zero<7>(a),
                           limb_or_zero<8>(a), limb_or_zero<9>(a), limb_or_zero<10>(a), limb_or_zero<11>(a),
                           limb_or_zero<12>(a), limb_or_zero<13>(a), limb_or_zero<0>(b), limb_or_zero<1>(b),
@@ -85,6 +85,7 @@ KZ_DEV Fe<F> fe_mul_call(const Fe<F>& a, const Fe<F>& b) {
 }
 
 struct MulInline {
+    static constexpr bool kLazy = false;
     template <class F>
     static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
         return fe_mul<F>(a, b);
@@ -96,6 +97,7 @@ struct MulInline {
 };
 
 struct MulCall {
+    static constexpr bool kLazy = false;
     template <class F>
     static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
         return fe_mul_call<F>(a, b);
@@ -110,6 +112,7 @@ struct MulCall {
 // (the lazy-reduction section of field.h).  Only for chains built from the
 // lazy helpers, such as xyzz_madd_lazy.
 struct MulLazy {
+    static constexpr bool kLazy = true;
     template <class F>
     static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
         return fe_mul_k<F, KZ_MUL_ILP, false>(a, b);
@@ -117,6 +120,19 @@ struct MulLazy {
     template <class F>
     static KZ_DEV Fe<F> sqr(const Fe<F>& a) {
         return fe_sqr<F, false>(a);
+    }
+};
+
+// The out-of-line product without its final subtraction.
+struct MulCallLazy {
+    static constexpr bool kLazy = true;
+    template <class F>
+    static KZ_DEV Fe<F> mul(const Fe<F>& a, const Fe<F>& b) {
+        return fe_mul_call<F, false>(a, b);
+    }
+    template <class F>
+    static KZ_DEV Fe<F> sqr(const Fe<F>& a) {
+        return fe_mul_call<F, false>(a, a);
     }
 };
 

@@ -374,6 +374,29 @@ KZ_DEV Fe<F> fe_dbl_lazy(const Fe<F>& a) {
     return s;
 }
 
+// a + b for a, b in [0, 2p): result in [0, 2p)
+template <class F>
+KZ_DEV Fe<F> fe_add_lazy(const Fe<F>& a, const Fe<F>& b) {
+    Fe<F> s, d;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = a.v[i] + b.v[i] + carry;
+        carry = t >> kLimbBits;
+        s.v[i] = t & kLimbMask;
+    }
+    uint32_t borrow = 0;  // a + b < 4p < 2^(29N): subtract 2p unless it borrows
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = s.v[i] - two_p_limb<F>(i) - borrow;
+        borrow = t >> 31;
+        d.v[i] = t & kLimbMask;
+    }
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) s.v[i] = borrow ? s.v[i] : d.v[i];
+    return s;
+}
+
 template <class F>
 KZ_DEV bool fe_is_zero_lazy(const Fe<F>& a) {
     uint32_t z = 0, e = 0;

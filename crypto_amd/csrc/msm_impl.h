@@ -156,8 +156,8 @@ k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
     if (len <= kShortRun) {
         Xyzz<F> acc = xyzz_load<F>(carry, head);
 #pragma unroll 1
-        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add_quad<F>(acc, xyzz_load<F>(carry, head + k));
-        if (quad_lane() == 0) xyzz_store<F>(carry, head, acc);
+        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add_quad<F, MulCallLazy>(acc, xyzz_load<F>(carry, head + k));
+        if (quad_lane() == 0) xyzz_store<F>(carry, head, xyzz_canon<F>(acc));
         return;
     }
     if (quad_lane() == 0) {
@@ -202,14 +202,16 @@ k_fold_groups(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, 
             __syncthreads();  // the previous step's readers of sh are done
             if ((q & (2 * st - 1)) == st && quad_lane() == 0) sh[q] = p;
             __syncthreads();
-            if ((q & (2 * st - 1)) == 0) p = xyzz_add_quad<F>(p, sh[q + st]);
+            if ((q & (2 * st - 1)) == 0) p = xyzz_add_quad<F, MulCallLazy>(p, sh[q + st]);
         }
-        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, p);
+        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, xyzz_canon<F>(p));
     }
 }
 
 // ---------------------------------------------------------- bucket reduce
 // Window sum W = sum_b (b+1) S_b over the window's bucket sums, in levels.
+// (The chains of point additions below run on lazily reduced coordinates,
+// MulCallLazy, and store canonical ones.)
 //
 //   k_reduce_seg    per segment t of `seg` buckets (a power of two dividing
 //                   the bucket count; one quad each): adds each bucket's folded
@@ -248,14 +250,14 @@ k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
         if (end > off) {
             S = xyzz_load<F>(bsum, gb);
             const uint32_t head = off / chunk + 1;
-            if ((end - 1) / chunk >= head) S = xyzz_add_quad<F>(S, xyzz_load<F>(carry, head));
+            if ((end - 1) / chunk >= head) S = xyzz_add_quad<F, MulCallLazy>(S, xyzz_load<F>(carry, head));
         }
-        run = xyzz_add_quad<F>(run, S);
-        tot = xyzz_add_quad<F>(tot, run);
+        run = xyzz_add_quad<F, MulCallLazy>(run, S);
+        tot = xyzz_add_quad<F, MulCallLazy>(tot, run);
     }
     if (quad_lane() == 0) {
-        xyzz_store<F>(segA, t, run);
-        xyzz_store<F>(segB, t, tot);
+        xyzz_store<F>(segA, t, xyzz_canon<F>(run));
+        xyzz_store<F>(segB, t, xyzz_canon<F>(tot));
     }
 }
 
@@ -307,7 +309,7 @@ k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ in
     for (uint32_t s = 1; s < span; s <<= 1) {
         const Xyzz<F> other = (k + s < span) ? sh[k + s] : xyzz_inf<F>();
         __syncthreads();
-        if (k < span) R = xyzz_add_quad<F>(R, other);
+        if (k < span) R = xyzz_add_quad<F, MulCallLazy>(R, other);
         if (lead) sh[k] = R;
         __syncthreads();
     }
@@ -315,22 +317,22 @@ k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ in
     if (k > 0 && k < span) {
         Xyzz<F> mR = R;
 #pragma unroll 1
-        for (uint32_t j = 0; j < lm; ++j) mR = xyzz_dbl_quad<F>(mR);
-        term = xyzz_add_quad<F>(term, mR);
+        for (uint32_t j = 0; j < lm; ++j) mR = xyzz_dbl_quad<F, MulCallLazy>(mR);
+        term = xyzz_add_quad<F, MulCallLazy>(term, mR);
     }
     if (lead) sh[k] = term;
     __syncthreads();
 #pragma unroll 1
     for (uint32_t s = span >> 1; s > 0; s >>= 1) {
         if (k < s) {
-            const Xyzz<F> v = xyzz_add_quad<F>(sh[k], sh[k + s]);
+            const Xyzz<F> v = xyzz_add_quad<F, MulCallLazy>(sh[k], sh[k + s]);
             if (lead) sh[k] = v;
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        xyzz_store<F>(outA, out_idx, R);
-        xyzz_store<F>(outB, out_idx, sh[0]);
+        xyzz_store<F>(outA, out_idx, xyzz_canon<F>(R));
+        xyzz_store<F>(outB, out_idx, xyzz_canon<F>(sh[0]));
     }
 }
 
@@ -385,11 +387,11 @@ __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ win
     for (int w = (int)jd.nwin - 2; w >= 0; --w) {
         if (!xyzz_is_inf<F>(r)) {
 #pragma unroll 1
-            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F, MulInline>(r);
+            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F, MulLazy>(r);
         }
-        r = xyzz_add_quad<F, MulInline>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
+        r = xyzz_add_quad<F, MulLazy>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
     }
-    if (quad_lane() == 0) write_output<C>(r, out + (size_t)job * 3 * C::FB);
+    if (quad_lane() == 0) write_output<C>(xyzz_canon<F>(r), out + (size_t)job * 3 * C::FB);
 }
 
 // ------------------------------------------------------------ window tables
