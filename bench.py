@@ -669,8 +669,12 @@ def main():
         extras[f"{CURVE_NAMES[curve]}_plain_path_verified"] = pres[:pb] == pcheck
         other = 1 if curve == 2 else 2
         oms, _, ores, ocheck, _ = timed_msm(lib, torch, D, other, n, 777, max(3, args.steps // 2), 2, 0, False)
-        extras[f"{CURVE_NAMES[other]}_mpoints_s"] = round(n / oms / 1e3, 3)
+        extras[f"{CURVE_NAMES[other]}_mpoints_s"] = round(n / oms / 1e3, 3)  # G1 points: GLV path
         extras[f"{CURVE_NAMES[other]}_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
+        oms, _, ores, ocheck, _ = timed_msm(lib, torch, D, other, n, 777, max(3, args.steps // 2), 2, 0, False,
+                                            g1=False)
+        extras[f"{CURVE_NAMES[other]}_plain_path_mpoints_s"] = round(n / oms / 1e3, 3)
+        extras[f"{CURVE_NAMES[other]}_plain_path_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
         # fixed point sets (an SRS): window table built once, then one window per MSM
         for cv, seed in ((curve, 20261203), (other, 777)):
             tms, tst, tres, tcheck, text = timed_msm(lib, torch, D, cv, n, seed, max(3, args.steps // 2), 2, 0,
