@@ -74,9 +74,7 @@ def test_lazy_mixed_addition_chain(hc, curve):
     out = ctypes.create_string_buffer(pb)
     G = M.CURVES[curve]["g"]
     r = M.CURVES[curve]["r"]
-    cases = []
-    for _ in range(40):
-        cases.append((rnd.randrange(1, r), rnd.randrange(1, r)))
+    cases = [(rnd.randrange(1, r), rnd.randrange(1, r)) for _ in range(12)]
     q = rnd.randrange(1, r)
     cases += [(q, q), (r - q, q), ((-2 * q) % r, q), ((-3 * q) % r, q), (0, q), ((-1 * q) % r, q)]
     for kp, kq in cases:
