@@ -119,6 +119,7 @@ def window_stats(scalars, c: int, chunk: int, bits: int = 256):
         carry = over.astype(np.int64)
         nz = d != 0
         counts += np.bincount(w * B + np.abs(d[nz]) - 1, minlength=W * B)
+    assert not carry.any(), "a carry out of the top window"  # 127-bit GLV parts never carry out
     offsets = np.concatenate([[0], np.cumsum(counts)])
     total = int(offsets[-1])
     nchunks = (total + chunk - 1) // chunk
@@ -604,7 +605,7 @@ def main():
     n_eff = 2 * n if glv else n
     c = args.window_bits or lib.kinet_hip_msm_window_bits(curve, n, 2 if glv else 0)
     cell = T.msm_cell(curve, 2 if glv else 0, args.log2n)
-    ws = window_stats(glv_halves(ext["scalars"], curve) if glv else ext["scalars"], c, chunk, 128 if glv else 256)
+    ws = window_stats(glv_halves(ext["scalars"], curve) if glv else ext["scalars"], c, chunk, 127 if glv else 256)
     nl = 12 if curve == 2 else 8          # 32-bit limbs of Fp: 2n^2 + n multiply-adds per product
     mads_per_mul = 2 * nl * nl + nl
     acc_mads = ws["madds"] * 10 * mads_per_mul       # XYZZ mixed addition = 8M + 2S

@@ -12,7 +12,10 @@ namespace kz {
 // root of unity mod r, matched to beta by phi(G) = lambda G).  Basis v1 =
 // (A1, -B1), v2 = (A2, B2) with det r; Babai coefficients estimated as
 // b1 = floor(k G1 / 2^256), b2 = floor(k G2 / 2^256), G1 = floor(2^256 B2 /
-// r), G2 = floor(2^256 B1 / r) (the truncation keeps |k| < 2^127, checked
+// r), G2 = floor(2^256 B1 / r).  Then (k1, k2) = e1 v1 + e2 v2 with rounding
+// errors 0 <= e_i < 1 + k d_i / 2^256 (d_i < 1 the truncation of G_i; for
+// k < r the errors stay below 1.016 and 1.082), so 0 <= k1 < 2^126.83 and
+// -2^63.13 < k2 < 2^126.92: both magnitudes below 2^127 (also checked
 // against big integers over 2e5 random scalars and edge values);
 //   k1 = k - b1 A1 - b2 A2,   k2 = b1 B1 - b2 B2   (k = s mod r).
 // Writes |k1|, |k2| (4 LE words each) and their signs.  Host and device.

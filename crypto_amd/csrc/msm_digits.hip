@@ -23,9 +23,9 @@
 
 namespace kz {
 
-// Windows of c-bit signed digits over BITS-bit scalars (256, or 128 for the
-// halves of a GLV split), with the extra window for the final carry when c
-// divides BITS.
+// Windows of c-bit signed digits over BITS-bit scalars (256, or 127 for the
+// magnitudes of a signed GLV split), with the extra window for the final
+// carry when c divides BITS (never for 127).
 template <int CB, int BITS = 256>
 struct Digits {
     static constexpr int W0 = (BITS + CB - 1) / CB;
@@ -53,7 +53,7 @@ __device__ __forceinline__ void load_scalar(const uint32_t* __restrict__ scalars
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
 }
 
-// Scalars of at most BITS bits: 128-bit ones skip the (zero) upper half.
+// Scalars of at most BITS bits: those of at most 128 skip the (zero) upper half.
 template <int BITS>
 __device__ __forceinline__ void load_scalar_bits(const uint32_t* __restrict__ scalars, uint32_t i, uint32_t (&s)[8]) {
     if (BITS > 128) {
@@ -220,14 +220,17 @@ bool lds_sort_supported(int c, size_t n) { return c >= 5 && c <= 16 && n >= 4096
 void lds_sort_shape(int c, int bits, size_t n, uint32_t& wpad, uint32_t& nparts, uint32_t& per_part) {
     const uint32_t W = (uint32_t)msm_num_windows(c, bits);
     wpad = (W + 7) / 8 * 8;
-    // parts of >= 4096 points, doubled while the launch has fewer than 128
-    // (window, part) workgroups.  Fewer, longer parts write each bucket's
-    // entries in longer runs: at 2^21 GLV points and 16-bit windows, 64 parts
-    // per window made the scattered 4-byte entry writes cost about 7x their
-    // bytes in HBM writes and the sort took 0.63 ms; 16 parts took 0.46 ms,
-    // 8 parts 0.73 ms (too few workgroups for the histogram passes).
+    // parts of >= 4096 points, doubled while the launch stays within 256
+    // (window, part) workgroups: one per CU of the MI355X (a 16-bit window's
+    // histogram takes 128 KiB of LDS, so one workgroup fills a CU).  Fewer,
+    // longer parts write each bucket's entries in longer runs: at 2^21 GLV
+    // points and 16-bit windows (9 windows then), 64 parts per window made
+    // the scattered 4-byte entry writes cost about 7x their bytes in HBM
+    // writes and the sort took 0.63 ms; 16 parts took 0.46 ms, 8 parts
+    // 0.73 ms.  Half the CUs idle also cost: 16 parts of 8 windows at 2^23
+    // points sorted in 2.96 ms against 2.57 ms for 16 parts of 9.
     uint32_t parts = 1;
-    while (parts * W < 128 && ((size_t)parts * 2) * 4096 <= n) parts *= 2;
+    while (parts * 2 * W <= 256 && ((size_t)parts * 2) * 4096 <= n) parts *= 2;
     nparts = parts;
     per_part = (uint32_t)((n + parts - 1) / parts);
 }
@@ -256,7 +259,7 @@ static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uin
 void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
                        uint32_t table_stride, uint32_t* cnt, uint32_t* pre, uint32_t* tot, uint32_t* offsets,
                        uint32_t* block_sums, uint32_t* total, uint32_t* entries, hipStream_t s) {
-    if (bits != 256 && bits != 128) throw std::invalid_argument("LDS sort: scalar bits must be 256 or 128");
+    if (bits != 256 && bits != 127) throw std::invalid_argument("LDS sort: scalar bits must be 256 or 127");
     switch (c) {
 #define KZ_C(CB)                                                                                                     \
     case CB:                                                                                                         \
@@ -264,7 +267,7 @@ void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t
             lds_sort_launch<CB, 256>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
                                      entries, s);                                                                    \
         else                                                                                                         \
-            lds_sort_launch<CB, 128>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
+            lds_sort_launch<CB, 127>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
                                      entries, s);                                                                    \
         break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15) KZ_C(16)

@@ -355,7 +355,7 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(curve);
-    // GLV (points in G1): 2n points with 128-bit scalars
+    // GLV (points in G1): 2n points with scalar magnitudes below 2^127
     int bits = 256;
     size_t npts = n;
     int c = prm.c;
@@ -369,7 +369,11 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     if (prm.g1 && !table && n <= (1u << 29) && (c || msm_glv_enabled(curve, n))) {
         const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
-            bits = 128;
+            // |k1|, |k2| < 2^127 (glv.h): 127-bit windows.  The top window then
+            // holds t < c real bits (c never divides the prime 127), so its digit
+            // plus an incoming carry is at most 2^t <= 2^(c-1) and never carries
+            // out: no carry window (8 windows at c = 16 instead of 9).
+            bits = 127;
             npts = 2 * n;
             c = cg;
             const size_t pw = (size_t)affine_words(curve) * 4;
@@ -385,7 +389,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
     if (!c) c = msm_window_bits(curve, n, kMsmPlain);
     if (c < 5 || c > 18) throw std::invalid_argument("window bits out of range [5, 18]");
     // the tuned segment belongs to the tuned window width
-    const int tuned_seg = prm.seg ? prm.seg : prm.c ? 0 : msm_tuned_segment(curve, n, bits == 128 ? kMsmGlv : kMsmPlain);
+    const int tuned_seg = prm.seg ? prm.seg : prm.c ? 0 : msm_tuned_segment(curve, n, bits != 256 ? kMsmGlv : kMsmPlain);
     const int W = msm_num_windows(c, bits);
     const uint32_t B = 1u << (c - 1);
     const uint32_t bw = table ? 1u : (uint32_t)W;  // bucket windows (a table shares one)
