@@ -234,7 +234,7 @@ __host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32
 }  // namespace glvbls
 
 // d_pts2 = [P_i, phi(P_i)] (2n prepared records), d_inf2 likewise, d_scalars2
-// = the two 128-bit parts (32-byte records) of s_i = s1_i + lambda s2_i mod r:
+// = the two parts (16-byte records) of s_i = s1_i + lambda s2_i mod r:
 // the signed splits above (magnitudes below 2^127), the signs folded into the
 // points (-P, -phi(P)).
 // Valid only for points of G1 (all curve points for BN254).

@@ -40,7 +40,7 @@ constexpr uint32_t kBetaMont[14] = {0x1195DFEBu, 0x1B04E484u, 0x06026044u, 0x086
 }  // namespace
 
 // Signed GLV split (glv.h): out_pts = [+-P_i, +-phi(P_i)], out_inf likewise,
-// out_sc = [|k1_i|, |k2_i|] (32-byte records, upper half zero).
+// out_sc = [|k1_i|, |k2_i|] (16-byte records).
 template <int CURVE>
 __global__ void __launch_bounds__(256)
 k_glv_expand(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, const uint32_t* __restrict__ scalars,
@@ -70,12 +70,9 @@ k_glv_expand(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, 
     const uint8_t f = inf ? inf[i] : 0;
     out_inf[i] = f;
     out_inf[(size_t)n + i] = f;
-    uint4* o1 = reinterpret_cast<uint4*>(out_sc + (size_t)i * 8);
-    uint4* o2 = reinterpret_cast<uint4*>(out_sc + ((size_t)n + i) * 8);
-    o1[0] = make_uint4(k1[0], k1[1], k1[2], k1[3]);
-    o1[1] = make_uint4(0, 0, 0, 0);
-    o2[0] = make_uint4(k2[0], k2[1], k2[2], k2[3]);
-    o2[1] = make_uint4(0, 0, 0, 0);
+    uint4* o = reinterpret_cast<uint4*>(out_sc);
+    o[i] = make_uint4(k1[0], k1[1], k1[2], k1[3]);
+    o[(size_t)n + i] = make_uint4(k2[0], k2[1], k2[2], k2[3]);
 }
 
 // *bad |= 1 if a non-identity point is off the curve or (BLS12-381) outside G1.

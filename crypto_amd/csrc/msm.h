@@ -14,7 +14,7 @@ struct MsmWorkspace {
     DevBuf bsum, carry, run_bucket, segA, segB, winres;
     DevBuf lvA[2], lvB[2];  // reduction levels (ping-pong)
     DevBuf sort_cnt, sort_pre;  // LDS sort: per-(window, part) histograms
-    DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 128-bit scalars
+    DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 16-byte scalar magnitudes
     DevBuf sort_dig;                  // batch LDS sort: 16-bit digit codes
     DevBuf out;
 };

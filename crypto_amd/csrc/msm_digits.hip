@@ -53,13 +53,14 @@ __device__ __forceinline__ void load_scalar(const uint32_t* __restrict__ scalars
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
 }
 
-// Scalars of at most BITS bits: those of at most 128 skip the (zero) upper half.
+// Scalars of at most BITS bits: 32-byte records, or for at most 128 bits
+// (the magnitudes of a GLV split) 16-byte records.
 template <int BITS>
 __device__ __forceinline__ void load_scalar_bits(const uint32_t* __restrict__ scalars, uint32_t i, uint32_t (&s)[8]) {
     if (BITS > 128) {
         load_scalar(scalars, i, s);
     } else {
-        const uint4 a = reinterpret_cast<const uint4*>(scalars + (size_t)i * 8)[0];
+        const uint4 a = reinterpret_cast<const uint4*>(scalars)[i];
         s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
         s[4] = s[5] = s[6] = s[7] = 0;
     }

@@ -379,7 +379,7 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
             const size_t pw = (size_t)affine_words(curve) * 4;
             uint32_t* pts2 = static_cast<uint32_t*>(ws.glv_pts.ensure(2 * n * pw));
             uint8_t* inf2 = static_cast<uint8_t*>(ws.glv_inf.ensure(2 * n));
-            uint32_t* sc2 = static_cast<uint32_t*>(ws.glv_sc.ensure(2 * n * 32));
+            uint32_t* sc2 = static_cast<uint32_t*>(ws.glv_sc.ensure(2 * n * 16));  // 16-byte magnitudes
             glv_expand(curve, d_pts, d_inf, d_scalars, n, pts2, inf2, sc2, s);
             d_pts = pts2;
             d_inf = inf2;
