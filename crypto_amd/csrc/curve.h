@@ -231,12 +231,23 @@ KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
     acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
 }
 
+template <class F>
+KZ_DEV Xyzz<F> xyzz_canon(Xyzz<F> p) {
+    fe_reduce_once<F>(p.X);
+    fe_reduce_once<F>(p.Y);
+    fe_reduce_once<F>(p.ZZ);
+    fe_reduce_once<F>(p.ZZZ);
+    return p;
+}
+
 // xyzz_madd with lazily reduced coordinates: acc's coordinates lie in
 // [0, 2p) (canonical ones included), q is canonical, and the result's lie in
 // [0, 2p) again; xyzz_canon makes them canonical.  The same point as
 // xyzz_madd.  The identity is only ever the canonical xyzz_inf (ZZ is a
 // product of non-zero values otherwise, never 0 or p), so xyzz_is_inf
-// applies unchanged; the doubling case runs canonically on q.
+// applies unchanged.  In the doubling case acc equals q, so acc is doubled
+// (canonically): q is then dead after its two products, which keeps the
+// accumulation loop within 168 registers without spills.
 template <class F>
 KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
     using M = MulLazy;
@@ -249,7 +260,7 @@ KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
     Fe<F> P = fe_sub_lazy<F>(U2, acc.X);
     Fe<F> R = fe_sub_lazy<F>(S2, acc.Y);
     if (fe_is_zero_lazy<F>(P)) {
-        if (fe_is_zero_lazy<F>(R)) acc = xyzz_dbl_affine<F, MulInline>(q);
+        if (fe_is_zero_lazy<F>(R)) acc = xyzz_dbl<F, MulInline>(xyzz_canon<F>(acc));  // acc = q: 2q
         else acc = xyzz_inf<F>();
         return;
     }
@@ -262,15 +273,6 @@ KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
     acc.Y = Y3;
     acc.ZZ = M::template mul<F>(acc.ZZ, PP);
     acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
-}
-
-template <class F>
-KZ_DEV Xyzz<F> xyzz_canon(Xyzz<F> p) {
-    fe_reduce_once<F>(p.X);
-    fe_reduce_once<F>(p.Y);
-    fe_reduce_once<F>(p.ZZ);
-    fe_reduce_once<F>(p.ZZZ);
-    return p;
 }
 
 // a + b, general (add-2008-s), with the doubling and inverse special cases.
