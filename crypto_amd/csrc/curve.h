@@ -240,14 +240,30 @@ KZ_DEV Xyzz<F> xyzz_canon(Xyzz<F> p) {
     return p;
 }
 
-// xyzz_madd with lazily reduced coordinates: acc's coordinates lie in
-// [0, 2p) (canonical ones included), q is canonical, and the result's lie in
-// [0, 2p) again; xyzz_canon makes them canonical.  The same point as
-// xyzz_madd.  The identity is only ever the canonical xyzz_inf (ZZ is a
-// product of non-zero values otherwise, never 0 or p), so xyzz_is_inf
-// applies unchanged.  In the doubling case acc equals q, so acc is doubled
-// (canonically): q is then dead after its two products, which keeps the
+// xyzz_madd for the accumulation loop, with wider bounds (field.h): on entry
+// acc.X < 8p, acc.Y < 4p, acc.ZZ, acc.ZZZ < 2p (canonical values included),
+// q canonical; the result keeps the same bounds, and xyzz_canon_acc makes it
+// canonical.  The same point as xyzz_madd:
+//   P = U2 + 8p - X (< 10p), R = S2 + 4p - Y (< 6p), PP = P^2, RR = R^2,
+//   X3 = RR + 6p - PPP - 2Q (< 8p), Y3 = R (Q + 8p - X3) + 2p - Y PPP (< 4p),
+// every product operand pair below 100 p^2.  P or R is zero mod p exactly
+// when PP or RR (products, below 2p) is 0 or p.  The identity is only ever the
+// canonical xyzz_inf (ZZ is a product of non-zero values otherwise), so
+// xyzz_is_inf applies unchanged.  In the doubling case acc equals q and is
+// doubled canonically: q is dead after its two products, which keeps the
 // accumulation loop within 168 registers without spills.
+template <class F>
+KZ_DEV Xyzz<F> xyzz_canon_acc(Xyzz<F> p) {
+    fe_csub_kp<F, 4>(p.X);
+    fe_csub_kp<F, 2>(p.X);
+    fe_csub_kp<F, 1>(p.X);
+    fe_csub_kp<F, 2>(p.Y);
+    fe_csub_kp<F, 1>(p.Y);
+    fe_reduce_once<F>(p.ZZ);
+    fe_reduce_once<F>(p.ZZZ);
+    return p;
+}
+
 template <class F>
 KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
     using M = MulLazy;
@@ -255,20 +271,18 @@ KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
         acc = xyzz_from_affine<F>(q);
         return;
     }
-    Fe<F> U2 = M::template mul<F>(q.x, acc.ZZ);
-    Fe<F> S2 = M::template mul<F>(q.y, acc.ZZZ);
-    Fe<F> P = fe_sub_lazy<F>(U2, acc.X);
-    Fe<F> R = fe_sub_lazy<F>(S2, acc.Y);
-    if (fe_is_zero_lazy<F>(P)) {
-        if (fe_is_zero_lazy<F>(R)) acc = xyzz_dbl<F, MulInline>(xyzz_canon<F>(acc));  // acc = q: 2q
+    const Fe<F> P = fe_sub_kp<F, 8>(M::template mul<F>(q.x, acc.ZZ), acc.X);
+    const Fe<F> R = fe_sub_kp<F, 4>(M::template mul<F>(q.y, acc.ZZZ), acc.Y);
+    const Fe<F> PP = M::template sqr<F>(P);
+    if (fe_is_zero_lazy<F>(PP)) {
+        if (fe_is_zero_lazy<F>(M::template sqr<F>(R))) acc = xyzz_dbl<F, MulInline>(xyzz_canon_acc<F>(acc));  // 2q
         else acc = xyzz_inf<F>();
         return;
     }
-    Fe<F> PP = M::template sqr<F>(P);
-    Fe<F> PPP = M::template mul<F>(P, PP);
-    Fe<F> Q = M::template mul<F>(acc.X, PP);
-    Fe<F> X3 = fe_sub_lazy<F>(fe_sub_lazy<F>(M::template sqr<F>(R), PPP), fe_dbl_lazy<F>(Q));
-    Fe<F> Y3 = fe_sub_lazy<F>(M::template mul<F>(R, fe_sub_lazy<F>(Q, X3)), M::template mul<F>(acc.Y, PPP));
+    const Fe<F> PPP = M::template mul<F>(P, PP);
+    const Fe<F> Q = M::template mul<F>(acc.X, PP);
+    const Fe<F> X3 = fe_r6p_s_2q<F>(M::template sqr<F>(R), PPP, Q);
+    const Fe<F> Y3 = fe_sub_kp<F, 2>(M::template mul<F>(R, fe_sub_kp<F, 8>(Q, X3)), M::template mul<F>(acc.Y, PPP));
     acc.X = X3;
     acc.Y = Y3;
     acc.ZZ = M::template mul<F>(acc.ZZ, PP);

@@ -397,6 +397,66 @@ KZ_DEV Fe<F> fe_add_lazy(const Fe<F>& a, const Fe<F>& b) {
     return s;
 }
 
+// Wider bounds for the accumulation's mixed additions.  The Montgomery
+// product maps inputs a, b with a b < R p to outputs below p + a b / R < 2p;
+// R / p is 2^25.3 for BLS12-381 Fp and 2^7.4 (about 169) for BN254 Fp, so
+// operands up to 10p (10 * 10 < 169) still give products below 2p.  Values
+// below 10p fit N normalised limbs.  Differences then need no conditional
+// correction: a + k p - b is computed in one pass with signed carries, the
+// caller choosing k p above b's bound.
+template <class F, int K>
+constexpr uint32_t kp_limb(int i) {
+    uint64_t c = 0, t = 0;
+    for (int j = 0; j <= i; ++j) {
+        t = (uint64_t)K * F::P[j] + c;
+        c = t >> kLimbBits;
+    }
+    return (uint32_t)t & kLimbMask;
+}
+
+// a + K p - b for a + K p - b in (0, 2^(29N)) (limbs normalised)
+template <class F, int K>
+KZ_DEV Fe<F> fe_sub_kp(const Fe<F>& a, const Fe<F>& b) {
+    Fe<F> d;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t t = (int32_t)(a.v[i] + kp_limb<F, K>(i)) - (int32_t)b.v[i] + c;
+        c = t >> kLimbBits;
+        d.v[i] = (uint32_t)t & kLimbMask;
+    }
+    return d;
+}
+
+// r + 6p - s - 2q in one pass, for r, s, q < 2p: a value in (0, 8p)
+template <class F>
+KZ_DEV Fe<F> fe_r6p_s_2q(const Fe<F>& r, const Fe<F>& s, const Fe<F>& q) {
+    Fe<F> d;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t t = (int32_t)(r.v[i] + kp_limb<F, 6>(i)) - (int32_t)s.v[i] - (int32_t)(q.v[i] << 1) + c;
+        c = t >> kLimbBits;
+        d.v[i] = (uint32_t)t & kLimbMask;
+    }
+    return d;
+}
+
+// a <- a - K p when a >= K p
+template <class F, int K>
+KZ_DEV void fe_csub_kp(Fe<F>& a) {
+    uint32_t d[F::N];
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = a.v[i] - kp_limb<F, K>(i) - borrow;
+        borrow = t >> 31;
+        d[i] = t & kLimbMask;
+    }
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) a.v[i] = borrow ? a.v[i] : d[i];
+}
+
 template <class F>
 KZ_DEV bool fe_is_zero_lazy(const Fe<F>& a) {
     uint32_t z = 0, e = 0;

@@ -98,8 +98,8 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
     for (uint32_t e = start + 1; e < end; ++e) {
         const Affine<F> q = load_signed_point<F>(pts, entries[e]);
         if (e == next) {
-            if (is_carry) xyzz_store<F>(carry, j, xyzz_canon<F>(acc));
-            else xyzz_store<F>(bsum, g, xyzz_canon<F>(acc));
+            if (is_carry) xyzz_store<F>(carry, j, xyzz_canon_acc<F>(acc));
+            else xyzz_store<F>(bsum, g, xyzz_canon_acc<F>(acc));
             do {
                 ++g;
                 next = offsets[g + 1];
@@ -107,11 +107,11 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
             is_carry = false;
             acc = xyzz_from_affine<F>(q);
         } else {
-            xyzz_madd_lazy<F>(acc, q);  // coordinates in [0, 2p) until stored
+            xyzz_madd_lazy<F>(acc, q);  // coordinates below 8p / 4p / 2p until stored
         }
     }
-    if (is_carry) xyzz_store<F>(carry, j, xyzz_canon<F>(acc));
-    else xyzz_store<F>(bsum, g, xyzz_canon<F>(acc));
+    if (is_carry) xyzz_store<F>(carry, j, xyzz_canon_acc<F>(acc));
+    else xyzz_store<F>(bsum, g, xyzz_canon_acc<F>(acc));
 }
 
 // ------------------------------------------------------------ carry fold

@@ -75,14 +75,22 @@ void point_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
         qa.y = q.Y;
         r = p;
         xyzz_madd<F>(r, qa);
-    } else if (op == 3) {  // p + 3 q through three lazily reduced mixed additions
+    } else if (op == 3) {  // p + 3 q through three mixed additions of the accumulation loop
         Xyzz<F> q = read_wire<C>(b);
         Affine<F> qa;
         qa.x = q.X;
         qa.y = q.Y;
         r = p;
         for (int k = 0; k < 3; ++k) xyzz_madd_lazy<F>(r, qa);
-        r = xyzz_canon<F>(r);
+        r = xyzz_canon_acc<F>(r);
+    } else if (op == 5) {  // p + 20 q: the loop's bounds in their steady state
+        Xyzz<F> q = read_wire<C>(b);
+        Affine<F> qa;
+        qa.x = q.X;
+        qa.y = q.Y;
+        r = p;
+        for (int k = 0; k < 20; ++k) xyzz_madd_lazy<F>(r, qa);
+        r = xyzz_canon_acc<F>(r);
     } else {
         r = xyzz_dbl<F>(p);
     }

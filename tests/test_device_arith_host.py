@@ -64,11 +64,12 @@ def test_field_inverse_many(hc, field):
 
 @pytest.mark.parametrize("curve", [1, 2])
 def test_lazy_mixed_addition_chain(hc, curve):
-    """Three mixed additions with coordinates kept in [0, 2p) between them
-    (the accumulation kernel's form), then made canonical: P + 3Q for random
-    points, for Q = P (the doubling case on the first step), Q = -P (the sum
-    passes through the identity), P = -2Q and P = -3Q (identity mid-chain and
-    at the end) and P the identity."""
+    """Mixed additions of the accumulation loop (coordinates kept below 8p /
+    4p / 2p between them), then made canonical: P + 3Q for random points, for
+    Q = P (the doubling case on the first step), Q = -P (the sum passes
+    through the identity), P = -2Q and P = -3Q (identity mid-chain and at the
+    end) and P the identity; and P + 20Q, where the bounds reach their steady
+    state, including chains through the identity."""
     pb = O.POINT_BYTES[curve]
     rnd = random.Random(40 + curve)
     out = ctypes.create_string_buffer(pb)
@@ -84,6 +85,12 @@ def test_lazy_mixed_addition_chain(hc, curve):
         hc.hc_point_op(curve, 3, Pw, M.to_wire(curve, Q), out)
         want = (kp + 3 * kq) % r
         assert out.raw == (M.to_wire(curve, M.mul(curve, G, want)) if want else bytes(pb)), (kp, kq)
+    for kp, kq in cases[:6] + [((-20 * q) % r, q), ((-7 * q) % r, q)]:  # 20 additions; identity mid-chain
+        P = M.mul(curve, G, kp) if kp else None
+        Pw = M.to_wire(curve, P) if P is not None else bytes(pb)
+        hc.hc_point_op(curve, 5, Pw, M.to_wire(curve, M.mul(curve, G, kq)), out)
+        want = (kp + 20 * kq) % r
+        assert out.raw == (M.to_wire(curve, M.mul(curve, G, want)) if want else bytes(pb)), (kp, kq, 20)
 
 
 @pytest.mark.parametrize("curve", [1, 2])
