@@ -108,9 +108,9 @@ struct MulCall {
     }
 };
 
-// Products without the final subtraction: operands and results in [0, 2p)
-// (the lazy-reduction section of field.h).  Only for chains built from the
-// lazy helpers, such as xyzz_madd_lazy.
+// Products without the final subtraction (the lazy-reduction section of
+// field.h): results below 2p for operand pairs below R p.  Only for chains
+// that keep the bounds of xyzz_madd_lazy / the lazy quad operations.
 struct MulLazy {
     static constexpr bool kLazy = true;
     template <class F>
@@ -229,15 +229,6 @@ KZ_DEV void xyzz_madd(Xyzz<F>& acc, const Affine<F>& q) {
     acc.Y = Y3;
     acc.ZZ = M::template mul<F>(acc.ZZ, PP);
     acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
-}
-
-template <class F>
-KZ_DEV Xyzz<F> xyzz_canon(Xyzz<F> p) {
-    fe_reduce_once<F>(p.X);
-    fe_reduce_once<F>(p.Y);
-    fe_reduce_once<F>(p.ZZ);
-    fe_reduce_once<F>(p.ZZZ);
-    return p;
 }
 
 // xyzz_madd for the accumulation loop, with wider bounds (field.h): on entry

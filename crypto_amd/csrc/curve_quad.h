@@ -14,9 +14,9 @@
 //   addition (add-2008-s)   14 products -> 4 levels
 //
 // The results are the field elements the serial formulas in curve.h
-// produce: canonical ones under MulInline / MulCall, the same elements in
-// [0, 2p) under the lazy policies, so outputs stay bit-identical once made
-// canonical.
+// produce: canonical ones under MulInline / MulCall, the same elements below
+// 8p / 4p / 2p (X / Y / ZZ, ZZZ) under the lazy policies, so outputs stay
+// bit-identical once made canonical (xyzz_canon_acc).
 //
 // Contract: all four lanes of a quad take the same branches (control flow
 // depends only on the point data they share).  Different quads may diverge.
@@ -59,55 +59,40 @@ __device__ __forceinline__ Fe<F> quad_pick(int q, const Fe<F>& a0, const Fe<F>& 
     return r;
 }
 
-// Additive helpers matching the product policy: canonical values, or with a
-// lazy policy (MulLazy, MulCallLazy) values in [0, 2p) (field.h), which the
-// caller makes canonical (xyzz_canon) before storing or comparing them.
-template <class F, bool L>
-__device__ __forceinline__ Fe<F> q_sub(const Fe<F>& a, const Fe<F>& b) {
-    if constexpr (L) return fe_sub_lazy<F>(a, b);
-    else return fe_sub<F>(a, b);
-}
-template <class F, bool L>
-__device__ __forceinline__ Fe<F> q_add(const Fe<F>& a, const Fe<F>& b) {
-    if constexpr (L) return fe_add_lazy<F>(a, b);
-    else return fe_add<F>(a, b);
-}
-template <class F, bool L>
-__device__ __forceinline__ Fe<F> q_dbl(const Fe<F>& a) {
-    if constexpr (L) return fe_dbl_lazy<F>(a);
-    else return fe_dbl<F>(a);
-}
-template <class F, bool L>
-__device__ __forceinline__ bool q_is_zero(const Fe<F>& a) {
-    if constexpr (L) return fe_is_zero_lazy<F>(a);
-    else return fe_is_zero<F>(a);
-}
-
 // 2P, three product levels.  (The curves' groups have odd order: doubling a
 // non-identity point never gives the identity, so ZZ stays non-zero.)
+//
+// Under a lazy policy the coordinates keep the accumulation loop's bounds
+// (field.h): X < 8p, Y < 4p, ZZ, ZZZ < 2p on entry and exit, differences in
+// one pass (a + k p - b), every product operand pair at most 64 p^2; the
+// caller makes stored values canonical with xyzz_canon_acc.
 template <class F, class M = MulCall>
 __device__ Xyzz<F> xyzz_dbl_quad(const Xyzz<F>& p) {
     constexpr bool L = M::kLazy;
     const int q = quad_lane();
-    const Fe<F> U = q_dbl<F, L>(p.Y);
+    const Fe<F> U = L ? fe_small_mul<F, 2>(p.Y) : fe_dbl<F>(p.Y);  // lazy: < 8p
     // level 1: V = U^2, XX = X^2
     Fe<F> m = M::template mul<F>(quad_pick<F>(q, U, p.X, U, p.X), quad_pick<F>(q, U, p.X, U, p.X));
     const Fe<F> V = quad_read<0>(m), XX = quad_read<1>(m);
-    const Fe<F> X3 = q_add<F, L>(q_dbl<F, L>(XX), XX);  // 3 X^2
+    const Fe<F> X3 = L ? fe_small_mul<F, 3>(XX) : fe_add<F>(fe_dbl<F>(XX), XX);  // 3 X^2 (lazy: < 6p)
     // level 2: W = U V, S = X V, ZZ' = V ZZ, (3 X^2)^2
     m = M::template mul<F>(quad_pick<F>(q, U, p.X, V, X3), quad_pick<F>(q, V, V, p.ZZ, X3));
     const Fe<F> W = quad_read<0>(m), S = quad_read<1>(m);
     Xyzz<F> r;
     r.ZZ = quad_read<2>(m);
-    r.X = q_sub<F, L>(quad_read<3>(m), q_dbl<F, L>(S));
+    r.X = L ? fe_kp_2b<F, 4>(quad_read<3>(m), S) : fe_sub<F>(quad_read<3>(m), fe_dbl<F>(S));  // lazy: < 6p
     // level 3: 3 X^2 (S - X'), W Y, W ZZZ
-    m = M::template mul<F>(quad_pick<F>(q, X3, W, W, W), quad_pick<F>(q, q_sub<F, L>(S, r.X), p.Y, p.ZZZ, p.ZZZ));
-    r.Y = q_sub<F, L>(quad_read<0>(m), quad_read<1>(m));
+    const Fe<F> SmX = L ? fe_sub_kp<F, 6>(S, r.X) : fe_sub<F>(S, r.X);  // lazy: < 8p
+    m = M::template mul<F>(quad_pick<F>(q, X3, W, W, W), quad_pick<F>(q, SmX, p.Y, p.ZZZ, p.ZZZ));
+    r.Y = L ? fe_sub_kp<F, 2>(quad_read<0>(m), quad_read<1>(m)) : fe_sub<F>(quad_read<0>(m), quad_read<1>(m));
     r.ZZZ = quad_read<2>(m);
     return r;
 }
 
 // a + b with the doubling and inverse special cases, four product levels.
+// Lazy policy: the bounds of xyzz_dbl_quad; P, R < 4p, X' < 8p, Y' < 4p, and
+// P or R is zero mod p exactly when PP or RR (products, below 2p) is 0 or p,
+// so the special cases are decided after the second level.
 template <class F, class M = MulCall>
 __device__ Xyzz<F> xyzz_add_quad(const Xyzz<F>& a, const Xyzz<F>& b) {
     constexpr bool L = M::kLazy;
@@ -117,24 +102,33 @@ __device__ Xyzz<F> xyzz_add_quad(const Xyzz<F>& a, const Xyzz<F>& b) {
     // level 1: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1
     Fe<F> m = M::template mul<F>(quad_pick<F>(q, a.X, b.X, a.Y, b.Y), quad_pick<F>(q, b.ZZ, a.ZZ, b.ZZZ, a.ZZZ));
     const Fe<F> U1 = quad_read<0>(m), S1 = quad_read<2>(m);
-    const Fe<F> P = q_sub<F, L>(quad_read<1>(m), U1);
-    const Fe<F> R = q_sub<F, L>(quad_read<3>(m), S1);
-    if (q_is_zero<F, L>(P)) {
-        if (q_is_zero<F, L>(R)) return xyzz_dbl_quad<F, M>(a);
-        return xyzz_inf<F>();
+    const Fe<F> P = L ? fe_sub_kp<F, 2>(quad_read<1>(m), U1) : fe_sub<F>(quad_read<1>(m), U1);
+    const Fe<F> R = L ? fe_sub_kp<F, 2>(quad_read<3>(m), S1) : fe_sub<F>(quad_read<3>(m), S1);
+    if constexpr (!L) {
+        if (fe_is_zero<F>(P)) {
+            if (fe_is_zero<F>(R)) return xyzz_dbl_quad<F, M>(a);
+            return xyzz_inf<F>();
+        }
     }
     // level 2: PP = P^2, RR = R^2, ZZ1 ZZ2, ZZZ1 ZZZ2
     m = M::template mul<F>(quad_pick<F>(q, P, R, a.ZZ, a.ZZZ), quad_pick<F>(q, P, R, b.ZZ, b.ZZZ));
     const Fe<F> PP = quad_read<0>(m), RR = quad_read<1>(m), Z12 = quad_read<2>(m), ZZZ12 = quad_read<3>(m);
+    if constexpr (L) {
+        if (fe_is_zero_lazy<F>(PP)) {
+            if (fe_is_zero_lazy<F>(RR)) return xyzz_dbl_quad<F, M>(a);
+            return xyzz_inf<F>();
+        }
+    }
     // level 3: PPP = P PP, Q = U1 PP, ZZ' = ZZ1 ZZ2 PP
     m = M::template mul<F>(quad_pick<F>(q, P, U1, Z12, Z12), quad_pick<F>(q, PP, PP, PP, PP));
     const Fe<F> PPP = quad_read<0>(m), Q = quad_read<1>(m);
     Xyzz<F> r;
     r.ZZ = quad_read<2>(m);
-    r.X = q_sub<F, L>(q_sub<F, L>(RR, PPP), q_dbl<F, L>(Q));
+    r.X = L ? fe_r6p_s_2q<F>(RR, PPP, Q) : fe_sub<F>(fe_sub<F>(RR, PPP), fe_dbl<F>(Q));
     // level 4: R (Q - X'), S1 PPP, ZZZ' = ZZZ1 ZZZ2 PPP
-    m = M::template mul<F>(quad_pick<F>(q, R, S1, ZZZ12, ZZZ12), quad_pick<F>(q, q_sub<F, L>(Q, r.X), PPP, PPP, PPP));
-    r.Y = q_sub<F, L>(quad_read<0>(m), quad_read<1>(m));
+    const Fe<F> QmX = L ? fe_sub_kp<F, 8>(Q, r.X) : fe_sub<F>(Q, r.X);
+    m = M::template mul<F>(quad_pick<F>(q, R, S1, ZZZ12, ZZZ12), quad_pick<F>(q, QmX, PPP, PPP, PPP));
+    r.Y = L ? fe_sub_kp<F, 2>(quad_read<0>(m), quad_read<1>(m)) : fe_sub<F>(quad_read<0>(m), quad_read<1>(m));
     r.ZZZ = quad_read<2>(m);
     return r;
 }

@@ -311,99 +311,14 @@ KZ_DEV Fe<F> fe_sqr(const Fe<F>& a) {
 }
 
 // ------------------------------------------------------- lazy reduction
-// Values in [0, 2p) instead of [0, p), for chains of operations that never
-// compare or store them.  The Montgomery product and square without their
-// final subtraction (fe_mul_k<F, K, false>, fe_sqr<F, false>) map inputs
-// below 2p to outputs below p + 4p^2/R < 2p (R = 2^(29N) > 2^25 p for the
-// base fields), and their 64-bit column bounds hold unchanged since limbs
-// stay below 2^29.  Addition chains correct by 2p (4p < 2^(29N)).  A lazy
-// value is zero mod p exactly when it is 0 or p; fe_reduce_once maps it to
-// the canonical value.
-template <class F>
-constexpr uint32_t two_p_limb(int i) {
-    uint32_t c = 0, t = 0;
-    for (int k = 0; k <= i; ++k) {
-        t = 2 * F::P[k] + c;
-        c = t >> kLimbBits;
-    }
-    return t & kLimbMask;
-}
-
-// a - b for a, b in [0, 2p): result in [0, 2p)
-template <class F>
-KZ_DEV Fe<F> fe_sub_lazy(const Fe<F>& a, const Fe<F>& b) {
-    Fe<F> d;
-    uint32_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = a.v[i] - b.v[i] - borrow;
-        borrow = t >> 31;
-        d.v[i] = t & kLimbMask;
-    }
-    const uint32_t mask = 0u - borrow;  // a - b in (-2p, 0): add 2p
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = d.v[i] + (two_p_limb<F>(i) & mask) + carry;
-        carry = t >> kLimbBits;
-        d.v[i] = t & kLimbMask;
-    }
-    return d;
-}
-
-// 2a for a in [0, 2p): result in [0, 2p)
-template <class F>
-KZ_DEV Fe<F> fe_dbl_lazy(const Fe<F>& a) {
-    Fe<F> s, d;
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = (a.v[i] << 1) + carry;
-        carry = t >> kLimbBits;
-        s.v[i] = t & kLimbMask;
-    }
-    uint32_t borrow = 0;  // 2a < 4p < 2^(29N): subtract 2p unless it borrows
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = s.v[i] - two_p_limb<F>(i) - borrow;
-        borrow = t >> 31;
-        d.v[i] = t & kLimbMask;
-    }
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) s.v[i] = borrow ? s.v[i] : d.v[i];
-    return s;
-}
-
-// a + b for a, b in [0, 2p): result in [0, 2p)
-template <class F>
-KZ_DEV Fe<F> fe_add_lazy(const Fe<F>& a, const Fe<F>& b) {
-    Fe<F> s, d;
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = a.v[i] + b.v[i] + carry;
-        carry = t >> kLimbBits;
-        s.v[i] = t & kLimbMask;
-    }
-    uint32_t borrow = 0;  // a + b < 4p < 2^(29N): subtract 2p unless it borrows
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = s.v[i] - two_p_limb<F>(i) - borrow;
-        borrow = t >> 31;
-        d.v[i] = t & kLimbMask;
-    }
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) s.v[i] = borrow ? s.v[i] : d.v[i];
-    return s;
-}
-
-// Wider bounds for the accumulation's mixed additions.  The Montgomery
-// product maps inputs a, b with a b < R p to outputs below p + a b / R < 2p;
-// R / p is 2^25.3 for BLS12-381 Fp and 2^7.4 (about 169) for BN254 Fp, so
-// operands up to 10p (10 * 10 < 169) still give products below 2p.  Values
-// below 10p fit N normalised limbs.  Differences then need no conditional
-// correction: a + k p - b is computed in one pass with signed carries, the
-// caller choosing k p above b's bound.
+// Chains of point operations that never compare or store their values skip
+// the products' final subtraction (fe_mul_k<F, K, false>, fe_sqr<F, false>).
+// The Montgomery product maps inputs a, b with a b < R p to outputs below
+// p + a b / R < 2p; R / p is 2^25.3 for BLS12-381 Fp and 2^7.4 (about 169)
+// for BN254 Fp, so operands up to 10p (10 * 10 < 169) still give products
+// below 2p.  Values below 10p fit N normalised limbs.  Differences then need
+// no conditional correction: a + k p - b is computed in one pass with signed
+// carries, the caller choosing k p above b's bound.
 template <class F, int K>
 constexpr uint32_t kp_limb(int i) {
     uint64_t c = 0, t = 0;
@@ -442,6 +357,34 @@ KZ_DEV Fe<F> fe_r6p_s_2q(const Fe<F>& r, const Fe<F>& s, const Fe<F>& q) {
     return d;
 }
 
+// a + K p - 2b in one pass (caller: the value lies in (0, 2^(29N)))
+template <class F, int K>
+KZ_DEV Fe<F> fe_kp_2b(const Fe<F>& a, const Fe<F>& b) {
+    Fe<F> d;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t t = (int32_t)(a.v[i] + kp_limb<F, K>(i)) - (int32_t)(b.v[i] << 1) + c;
+        c = t >> kLimbBits;
+        d.v[i] = (uint32_t)t & kLimbMask;
+    }
+    return d;
+}
+
+// m a for m <= 3 and m a < 2^(29N), no reduction
+template <class F, int Mul>
+KZ_DEV Fe<F> fe_small_mul(const Fe<F>& a) {
+    Fe<F> d;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = a.v[i] * Mul + c;
+        c = t >> kLimbBits;
+        d.v[i] = t & kLimbMask;
+    }
+    return d;
+}
+
 // a <- a - K p when a >= K p
 template <class F, int K>
 KZ_DEV void fe_csub_kp(Fe<F>& a) {
@@ -457,6 +400,7 @@ KZ_DEV void fe_csub_kp(Fe<F>& a) {
     for (int i = 0; i < F::N; ++i) a.v[i] = borrow ? a.v[i] : d[i];
 }
 
+// A value below 2p is zero mod p exactly when it is 0 or p.
 template <class F>
 KZ_DEV bool fe_is_zero_lazy(const Fe<F>& a) {
     uint32_t z = 0, e = 0;

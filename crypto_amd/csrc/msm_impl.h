@@ -157,7 +157,7 @@ k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
         Xyzz<F> acc = xyzz_load<F>(carry, head);
 #pragma unroll 1
         for (uint32_t k = 1; k < len; ++k) acc = xyzz_add_quad<F, MulCallLazy>(acc, xyzz_load<F>(carry, head + k));
-        if (quad_lane() == 0) xyzz_store<F>(carry, head, xyzz_canon<F>(acc));
+        if (quad_lane() == 0) xyzz_store<F>(carry, head, xyzz_canon_acc<F>(acc));
         return;
     }
     if (quad_lane() == 0) {
@@ -204,14 +204,14 @@ k_fold_groups(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, 
             __syncthreads();
             if ((q & (2 * st - 1)) == 0) p = xyzz_add_quad<F, MulCallLazy>(p, sh[q + st]);
         }
-        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, xyzz_canon<F>(p));
+        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, xyzz_canon_acc<F>(p));
     }
 }
 
 // ---------------------------------------------------------- bucket reduce
 // Window sum W = sum_b (b+1) S_b over the window's bucket sums, in levels.
 // (The chains of point additions below run on lazily reduced coordinates,
-// MulCallLazy, and store canonical ones.)
+// MulCallLazy, below 8p / 4p / 2p, and store canonical ones.)
 //
 //   k_reduce_seg    per segment t of `seg` buckets (a power of two dividing
 //                   the bucket count; one quad each): adds each bucket's folded
@@ -260,8 +260,8 @@ k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
         tot = xyzz_add_quad<F, MulCallLazy>(tot, run);
     }
     if (quad_lane() == 0) {
-        xyzz_store<F>(segA, t, xyzz_canon<F>(run));
-        xyzz_store<F>(segB, t, xyzz_canon<F>(tot));
+        xyzz_store<F>(segA, t, xyzz_canon_acc<F>(run));
+        xyzz_store<F>(segB, t, xyzz_canon_acc<F>(tot));
     }
 }
 
@@ -335,8 +335,8 @@ k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ in
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        xyzz_store<F>(outA, out_idx, xyzz_canon<F>(R));
-        xyzz_store<F>(outB, out_idx, xyzz_canon<F>(sh[0]));
+        xyzz_store<F>(outA, out_idx, xyzz_canon_acc<F>(R));
+        xyzz_store<F>(outB, out_idx, xyzz_canon_acc<F>(sh[0]));
     }
 }
 
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ win
         }
         r = xyzz_add_quad<F, MulLazy>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
     }
-    if (quad_lane() == 0) write_output<C>(xyzz_canon<F>(r), out + (size_t)job * 3 * C::FB);
+    if (quad_lane() == 0) write_output<C>(xyzz_canon_acc<F>(r), out + (size_t)job * 3 * C::FB);
 }
 
 // ------------------------------------------------------------ window tables
