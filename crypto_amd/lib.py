@@ -62,6 +62,8 @@ def library():
         for name in ("gpukit_multi_pippenger", "gpukit_multi_pippenger_hip"):
             getattr(lib, name).argtypes = [u32, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p]
             getattr(lib, name).restype = i32
+        lib.bls12_381_g1_msm.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
+        lib.bls12_381_g1_msm.restype = i32
         lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
         lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         lib.kinet_hip_msm_prepared_g1.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
@@ -245,6 +247,18 @@ def kzg_commit_batch(srs_list, evals_list) -> list[bytes]:
     if rc != 0:
         raise KinetError("kzg_commit_batch", rc, CRYPTO_STATUS)
     return [out.raw[cb * i: cb * (i + 1)] for i in range(count)]
+
+
+def bls12_381_g1_msm(xycs: bytes) -> tuple[bool, bytes, bytes]:
+    """EIP-2537 G1 MSM (include/kinet/crypto/bls_g1_msm.h; the reference's
+    bls12_381_g1_msm, bls/c-abi/c_bls_pairing.h:153-156): xycs is N tuples of
+    x[64] || y[64] || c[32] (big-endian); returns (ok, rx[64], ry[64]) with the
+    reference's convention of ok = False on any encoding, curve or subgroup
+    failure."""
+    lib = require_device()
+    rx, ry = ctypes.create_string_buffer(64), ctypes.create_string_buffer(64)
+    ok = lib.bls12_381_g1_msm(rx, ry, bytes(xycs), len(xycs)) == 1
+    return ok, rx.raw, ry.raw
 
 
 def multi_pippenger(curve: int, scalars: bytes, points: bytes, backend: str = "dispatch") -> bytes:

@@ -53,3 +53,22 @@ def test_no_silent_cpu_route_without_device():
     assert rc == -3  # GPUKIT_ERR_BACKEND
     with pytest.raises(crypto_amd.HipUnavailable):
         crypto_amd.multi_pippenger(1, bytes(32), bytes(64))
+
+
+def test_eip2537_msm_rejects_without_device_work():
+    """bls12_381_g1_msm checks its encodings before any device call: malformed
+    input returns 0 here too, and a well-formed one returns 0 without a device
+    (no CPU route)."""
+    import crypto_amd
+
+    if not os.path.exists(crypto_amd.library_path()):
+        pytest.skip("HIP library not built")
+    lib = crypto_amd.library()
+    rx, ry = ctypes.create_string_buffer(64), ctypes.create_string_buffer(64)
+    assert lib.bls12_381_g1_msm(rx, ry, bytes(159), 159) == 0  # not a whole entry
+    bad = bytearray(160)
+    bad[0] = 1  # non-zero padding byte
+    assert lib.bls12_381_g1_msm(rx, ry, bytes(bad), 160) == 0
+    assert lib.bls12_381_g1_msm(rx, ry, b"", 0) == 1 and rx.raw == bytes(64) and ry.raw == bytes(64)
+    if not lib.kinet_hip_available():
+        assert lib.bls12_381_g1_msm(rx, ry, bytes(160), 160) == 0
