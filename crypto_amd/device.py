@@ -110,6 +110,22 @@ def check_g1(pp: PreparedPoints) -> bool:
     return int(bad.item()) == 0
 
 
+def points_decompress(curve: int, d_comp: torch.Tensor, stream=None):
+    """Compressed G1 encodings on the device (n x 48 / 32 bytes) -> (wire
+    records n x 96 / 64 bytes, all valid?) through kinet_hip_points_decompress."""
+    lib = require_device()
+    cb = point_bytes(curve) // 2
+    n = d_comp.numel() // cb
+    wire = torch.empty(n * 2 * cb, dtype=torch.uint8, device=d_comp.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=d_comp.device)
+    rc = lib.kinet_hip_points_decompress(curve, d_comp.data_ptr(), n, wire.data_ptr(), bad.data_ptr(),
+                                         stream_handle(stream))
+    if rc != GPUKIT_OK:
+        raise KinetError("kinet_hip_points_decompress", rc, GPUKIT_STATUS)
+    (stream or torch.cuda.current_stream()).synchronize()
+    return wire, int(bad.item()) == 0
+
+
 class MsmBatch:
     """A plan of many MSMs run as one pipeline (kinet_hip_msm_batch_*).
 

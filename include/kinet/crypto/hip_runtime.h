@@ -80,6 +80,17 @@ int kinet_hip_msm_table(uint32_t curve, const void* d_scalars, const void* d_tab
 int kinet_hip_points_check_g1(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int* d_bad,
                               void* stream);
 
+/* Decompresses n G1 points (48-byte ZCash BLS12-381 / 32-byte BN254
+ * encodings, the compressed form of the MSM outputs) into wire records
+ * (big-endian x || y, all zero for infinity).  Sets *d_bad (device int, zeroed
+ * by the caller) nonzero for any rejected encoding: missing compression flag,
+ * infinity with other bits set, x >= p, or x^3 + b not a square (the rejected
+ * record is written as zeros).  Subgroup membership is checked separately
+ * (kinet_hip_points_check_g1).  No first-party counterpart in the reference
+ * (it uses blst's codec, bls/cpp/bls_signature.cpp:58).  Asynchronous on
+ * `stream`; gpukit codes. */
+int kinet_hip_points_decompress(uint32_t curve, const void* d_comp, size_t n, void* d_wire, int* d_bad, void* stream);
+
 /* ------------------------------------------------------------- batches
  * Many independent MSMs as one pipeline: every stage is launched once for
  * all jobs (bucket ids, windows and outputs of the jobs laid end to end), so

@@ -302,9 +302,14 @@ int orc_g1_decompress(int curve, const uint8_t* in, uint8_t* out_wire) {
     uint8_t xb[48];
     int larger;
     memcpy(xb, in, fb);
+    /* infinity: the flag byte alone (0xC0 BLS12-381, 0x40 BN254), every
+       other bit zero */
+    int rest_zero = 1;
+    for (int k = 1; k < fb; ++k) rest_zero &= in[k] == 0;
     if (curve == ORC_CURVE_BLS12_381) {
         if (!(in[0] & 0x80)) return -1;
         if (in[0] & 0x40) {
+            if (in[0] != 0xC0 || !rest_zero) return -1;
             memset(out_wire, 0, 2 * fb);
             return 0;
         }
@@ -312,7 +317,11 @@ int orc_g1_decompress(int curve, const uint8_t* in, uint8_t* out_wire) {
         xb[0] &= 0x1F;
     } else {
         int flag = in[0] >> 6;
-        if (flag == 1) { memset(out_wire, 0, 2 * fb); return 0; }
+        if (flag == 1) {
+            if (in[0] != 0x40 || !rest_zero) return -1;
+            memset(out_wire, 0, 2 * fb);
+            return 0;
+        }
         if (flag < 2) return -1;
         larger = (flag == 3);
         xb[0] &= 0x3F;

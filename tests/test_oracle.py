@@ -347,3 +347,17 @@ def test_package_scalar_generator_matches_oracle(curve):
     r = M.CURVES[curve]["r"]
     assert all(0 <= v < r for v in inputs.to_int(x))
     assert inputs.dot_mod(curve, x, x) == sum(v * v for v in inputs.to_int(x)) % r
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_decompress_rejects_loose_infinity(curve):
+    """The infinity encoding is the flag byte alone (0xC0 BLS12-381, 0x40
+    BN254) with every other bit zero; anything else with the infinity flag is
+    rejected."""
+    cb = O.COMP_BYTES[curve]
+    flag = 0xC0 if curve == 2 else 0x40
+    assert O.decompress(curve, bytes([flag]) + bytes(cb - 1)) == bytes(2 * cb)
+    assert O.decompress(curve, bytes([flag]) + bytes(cb - 2) + b"\x01") is None
+    assert O.decompress(curve, bytes([flag | 0x01]) + bytes(cb - 1)) is None
+    if curve == 2:
+        assert O.decompress(curve, bytes([0xE0]) + bytes(cb - 1)) is None  # sign bit with infinity
