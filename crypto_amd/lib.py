@@ -77,6 +77,8 @@ def library():
         lib.kinet_hip_ntt_fr.argtypes = [u32, i32, vp, vp, vp, sz, vp, vp]
         pp = ctypes.POINTER(ctypes.c_void_p)
         lib.kzg_srs_create.argtypes = [u32, ctypes.c_char_p, sz, pp]
+        lib.kzg_open.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+        lib.kzg_open.restype = i32
         lib.kzg_srs_generate.argtypes = [u32, ctypes.c_char_p, sz, pp]
         lib.kzg_srs_destroy.argtypes = [vp]
         lib.kzg_srs_destroy.restype = None
@@ -209,6 +211,15 @@ class KzgSrs:
         if rc != 0:
             raise KinetError("kzg_commit", rc, CRYPTO_STATUS)
         return out.raw
+
+    def open(self, evals: bytes, z: int) -> tuple[bytes, int]:
+        """Opening proof at z (kzg_open): (compressed proof, y = f(z))."""
+        proof = ctypes.create_string_buffer(compressed_bytes(self.curve))
+        y = ctypes.create_string_buffer(32)
+        rc = library().kzg_open(self.handle, evals, len(evals) // 32, z.to_bytes(32, "little"), proof, y)
+        if rc != 0:
+            raise KinetError("kzg_open", rc, CRYPTO_STATUS)
+        return proof.raw, int.from_bytes(y.raw, "little")
 
     def msm(self, scalars: bytes) -> bytes:
         out = ctypes.create_string_buffer(point_bytes(self.curve))

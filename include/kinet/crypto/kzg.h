@@ -61,6 +61,15 @@ int kzg_srs_export(const kzg_srs* srs, size_t first, size_t count, uint8_t* poin
 
 /* Commitment to the polynomial with the given evaluations (host memory, synchronous). */
 int kzg_commit(const kzg_srs* srs, const uint8_t* evals, size_t n, uint8_t* commit_out);
+/* Opening proof of f at z, f given as for kzg_commit (n evaluations on the
+ * NTT domain, n a power of two >= 2): y = f(z) (32 bytes LE canonical) and
+ * proof = [q(tau)] G for q(X) = (f(X) - y) / (X - z), compressed like a
+ * commitment.  z: 32 bytes LE canonical, outside the domain (a domain point
+ * gives CRYPTO_ERR_INPUT).  The reference's kzg_commit_to_proof
+ * (c-abi/crypto.h:239) computes the same opening for an EIP-4844 blob over
+ * its trusted setup; this one works over a kzg_srs handle. */
+int kzg_open(const kzg_srs* srs, const uint8_t* evals, size_t n, const uint8_t z[32], uint8_t* proof_out,
+             uint8_t y_out[32]);
 /* Commitment from coefficients c_i (f(X) = sum c_i X^i): MSM + compression
  * only, the form documented for the reference's planned C API
  * (docs/content/docs/api/kzg.mdx:289-331, `kzg_commit(srs, coeffs, ...)`). */

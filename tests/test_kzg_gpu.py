@@ -124,3 +124,52 @@ def test_precomputed_srs_commits_unchanged(ca, curve):
     assert srs.commit(evs[4096]) == before[4096]
     with pytest.raises(ca.KinetError):
         srs.precompute(4)
+
+
+def _eval_at(curve, fs, x):
+    """f(x) for f given by its values on the NTT domain (natural order), by the
+    barycentric formula (x outside the domain)."""
+    r = M.CURVES[curve]["r"]
+    n = len(fs)
+    w = pow(7 if curve == 2 else 5, (r - 1) // n, r)
+    acc, wi = 0, 1
+    for f in fs:
+        acc = (acc + f * wi * pow((x - wi) % r, -1, r)) % r
+        wi = wi * w % r
+    return (pow(x, n, r) - 1) * pow(n, -1, r) % r * acc % r
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("n", [2, 16, 4096])
+def test_open_matches_closed_form(ca, curve, n):
+    """kzg_open: y = f(z) and the proof [q(tau)] G, q = (f - y) / (X - z), against
+    the closed form computed in Python from the evaluations (barycentric
+    formula), for random and special z."""
+    r = M.CURVES[curve]["r"]
+    tau = int.from_bytes(O.gen_scalars(curve, 21, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, max(n, 64))
+    ev = _evals(curve, n, 40 + n)
+    fs = [int.from_bytes(ev[32 * i:32 * i + 32], "little") for i in range(n)]
+    rnd = random.Random(n)
+    for z in (rnd.randrange(r), r - 2, 2):  # -1 = w^(n/2) is a domain point; -2 and 2 are not
+        print(f"kzg open curve={curve} n={n} z={z % 1000}", flush=True)
+        proof, y = srs.open(ev, z)
+        assert y == _eval_at(curve, fs, z)
+        q_tau = (_eval_at(curve, fs, tau) - y) * pow((tau - z) % r, -1, r) % r
+        assert proof == O.compress(curve, O.g1_mul_gen(curve, q_tau.to_bytes(32, "little")))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_open_rejects_domain_points_and_bad_inputs(ca, curve):
+    import crypto_amd
+
+    r = M.CURVES[curve]["r"]
+    n = 16
+    srs = ca.KzgSrs.generate(curve, 12345, 64)
+    ev = _evals(curve, n, 77)
+    w = pow(7 if curve == 2 else 5, (r - 1) // n, r)
+    for z in (1, w, pow(w, 5, r), r, r + 1):  # domain points, then non-canonical z
+        with pytest.raises(crypto_amd.KinetError):
+            srs.open(ev, z % (1 << 256))
+    with pytest.raises(crypto_amd.KinetError):
+        srs.open(ev[:32 * 12], 9)  # not a power of two
