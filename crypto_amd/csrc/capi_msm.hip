@@ -38,7 +38,11 @@ Engine& engine() {
     return *e;
 }
 
+// Curves of the device entries: BN254 and BLS12-381 G1 everywhere; secp256k1
+// only through the plain MSM (gpukit_multi_pippenger_hip, points_prepare,
+// msm_prepared), not through window tables or the G1 (GLV) form.
 static bool valid_curve(uint32_t curve) { return curve == 1 || curve == 2; }
+static bool valid_plain_curve(uint32_t curve) { return curve <= 2; }
 
 }  // namespace kz
 
@@ -65,8 +69,8 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
                                uint8_t* result) {
     if (!result) return GPUKIT_ERR_NULL_ARG;
     if (n > 0 && (!scalars || !points)) return GPUKIT_ERR_NULL_ARG;
-    if (curve == GPUKIT_CURVE_SECP256K1 || curve == GPUKIT_CURVE_BANDERWAGON) return GPUKIT_ERR_NOTIMPL;
-    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (curve == GPUKIT_CURVE_BANDERWAGON) return GPUKIT_ERR_NOTIMPL;
+    if (!valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     const size_t pb = point_wire_bytes((int)curve);
     if (n == 0) {
         std::memset(result, 0, pb);
@@ -113,7 +117,7 @@ int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t
 
 int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
                              void* stream) {
-    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (!valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (n > 0 && (!d_wire || !d_pts || !d_inf)) return GPUKIT_ERR_NULL_ARG;
     try {
         (void)engine();
@@ -129,7 +133,7 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 
 static int msm_prepared_impl(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                              size_t n, int window_bits, int chunk, void* d_out, void* stream, bool g1) {
-    if (!valid_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (g1 ? !valid_curve(curve) : !valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
     if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
     if (chunk < 0 || chunk > 4096) return GPUKIT_ERR_BAD_SIZE;

@@ -31,6 +31,16 @@ struct CurveBLS12381 {
     static constexpr int ID = 2;
 };
 
+// secp256k1 (y^2 = x^3 + 7, secp256k1/cpp/curve.hpp): only the plain MSM of
+// the multi-curve entry runs on it, so it has no scalar-field type here.
+struct CurveSecp256k1 {
+    using Fp = SecpFp;
+    using Fr = void;
+    static constexpr int FB = 32;
+    static constexpr uint32_t B = 7;
+    static constexpr int ID = 0;
+};
+
 template <class F>
 struct Affine {
     Fe<F> x, y;

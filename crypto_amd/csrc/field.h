@@ -13,7 +13,8 @@
 //   BLS12-381 Fp  N = 14  (381-bit p)   bls/cpp/bls12_381_fp.hpp:83-85
 //   BLS12-381 Fr  N = 9   (255-bit r)   kzg/cpp/kzg_oracle.hpp:66-71
 //   BN254 Fp, Fr  N = 9   (254-bit)     bn254/cpp/bn254_fp.hpp:75-81
-// All four satisfy 4p < R, so Montgomery products of values below p stay
+//   secp256k1 Fp  N = 10  (256-bit p)   secp256k1/cpp/field.hpp:67
+// All five satisfy 4p < R, so Montgomery products of values below p stay
 // below 2p before the final conditional subtraction.
 //
 // Invariant: every Fe produced by the arithmetic has limbs < 2^29 and value
@@ -90,6 +91,23 @@ struct BnFr {
                                        0x1CF855E3u, 0x1C15E103u, 0x07D09161u, 0x000A054Au};
     static constexpr uint32_t R3[9] = {0x001FDDB2u, 0x17D30B63u, 0x1A2600EEu, 0x09507C47u, 0x1496B29Bu,
                                        0x0B00A268u, 0x15B645EBu, 0x1F9FCB3Du, 0x001BAA96u};
+};
+
+// secp256k1 Fp, p = 2^256 - 2^32 - 977 (secp256k1/cpp/field.hpp:67).  Ten limbs
+// although nine hold p: with R = 2^290, R / p = 2^34 keeps the wide operand
+// bounds of the lazy point formulas (nine limbs would give R / p = 32 < 100).
+struct SecpFp {
+    static constexpr int N = 10;
+    static constexpr int BITS = 256;
+    static constexpr uint32_t PINV = 0x12253531u;
+    static constexpr uint32_t P[10] = {0x1FFFFC2Fu, 0x1FFFFFF7u, 0x1FFFFFFFu, 0x1FFFFFFFu, 0x1FFFFFFFu,
+                                       0x1FFFFFFFu, 0x1FFFFFFFu, 0x1FFFFFFFu, 0x00FFFFFFu, 0x00000000u};
+    static constexpr uint32_t ONE[10] = {0x00000000u, 0x00007A20u, 0x00000100u, 0x00000000u, 0x00000000u,
+                                         0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u};
+    static constexpr uint32_t R2[10] = {0x00000000u, 0x00000000u, 0x1A428400u, 0x00F44001u, 0x00010000u,
+                                        0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u};
+    static constexpr uint32_t R3[10] = {0x00000000u, 0x00000000u, 0x00000000u, 0x1B388000u, 0x078CDE57u,
+                                        0x0E600576u, 0x0100000Bu, 0x00000000u, 0x00000000u, 0x00000000u};
 };
 
 template <class F>

@@ -92,9 +92,9 @@ inline size_t msm_out_bytes(int curve) { return curve == 2 ? 96 + 48 : 64 + 32; 
 inline size_t point_wire_bytes(int curve) { return curve == 2 ? 96 : 64; }
 inline size_t compressed_bytes(int curve) { return curve == 2 ? 48 : 32; }
 // Device record sizes in 32-bit words (radix-2^29 limbs, padded to 16 bytes):
-// affine points 28 (BLS12-381) / 20 (BN254), XYZZ points 56 / 36.
+// affine points 28 (BLS12-381) / 20 (BN254, secp256k1), XYZZ points 56 / 36 / 40.
 inline int affine_words(int curve) { return curve == 2 ? 28 : 20; }
-inline int xyzz_words(int curve) { return curve == 2 ? 56 : 36; }
+inline int xyzz_words(int curve) { return curve == 2 ? 56 : curve == 1 ? 36 : 40; }
 
 // Wire affine points (BE, zero = identity) -> Montgomery affine (2N words each)
 // plus per-point identity flags.

@@ -225,7 +225,10 @@ const Stages kBls = {launch_accumulate_bls, launch_fold_bls,    launch_reduce_se
 const Stages kBn = {launch_accumulate_bn, launch_fold_bn,    launch_reduce_seg_bn,
                     launch_reduce_level_bn, launch_combine_bn, launch_table_rows_bn, 36};
 
-const Stages& stages_of(int curve) { return curve == 2 ? kBls : kBn; }
+const Stages kSecp = {launch_accumulate_secp, launch_fold_secp,    launch_reduce_seg_secp,
+                      launch_reduce_level_secp, launch_combine_secp, launch_table_rows_secp, 40};
+
+const Stages& stages_of(int curve) { return curve == 2 ? kBls : curve == 1 ? kBn : kSecp; }
 
 // Sizes of one pipeline run, single MSM or batch.
 struct Shape {
@@ -620,13 +623,15 @@ void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t
                       hipStream_t s) {
     if (n == 0) return;
     if (curve == 2) launch_points_from_wire_bls(d_wire, d_pts, d_inf, (uint32_t)n, s);
-    else launch_points_from_wire_bn(d_wire, d_pts, d_inf, (uint32_t)n, s);
+    else if (curve == 1) launch_points_from_wire_bn(d_wire, d_pts, d_inf, (uint32_t)n, s);
+    else launch_points_from_wire_secp(d_wire, d_pts, d_inf, (uint32_t)n, s);
     KZ_HIP(hipGetLastError());
 }
 
 void compress_on_device(int curve, const uint8_t* d_wire, uint8_t* d_out, hipStream_t s) {
     if (curve == 2) launch_compress_bls(d_wire, d_out, s);
-    else launch_compress_bn(d_wire, d_out, s);
+    else if (curve == 1) launch_compress_bn(d_wire, d_out, s);
+    else return;  // secp256k1 records leave the compressed field zero
     KZ_HIP(hipGetLastError());
 }
 

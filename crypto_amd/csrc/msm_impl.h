@@ -345,6 +345,10 @@ template <class C>
 __device__ void write_compressed(const Fe<typename C::Fp>& x_canon, const Fe<typename C::Fp>& y_canon,
                                  uint8_t* __restrict__ comp) {
     using F = typename C::Fp;
+    if constexpr (C::ID == 0) {  // secp256k1: no compressed form in the record
+        for (int k = 0; k < C::FB; ++k) comp[k] = 0;
+        return;
+    }
     const bool larger = fe_gt<F>(y_canon, fe_neg<F>(y_canon));  // y > p - y
     fe_to_be_bytes<F>(x_canon, comp, C::FB);
     if (C::ID == 2) comp[0] |= 0x80 | (larger ? 0x20 : 0);
@@ -354,7 +358,7 @@ __device__ void write_compressed(const Fe<typename C::Fp>& x_canon, const Fe<typ
 template <class C>
 __device__ void write_infinity(uint8_t* __restrict__ out) {
     for (int k = 0; k < 3 * C::FB; ++k) out[k] = 0;
-    out[2 * C::FB] = (C::ID == 2) ? 0xC0 : 0x40;
+    out[2 * C::FB] = (C::ID == 2) ? 0xC0 : (C::ID == 1) ? 0x40 : 0x00;
 }
 
 template <class C>

@@ -30,6 +30,7 @@ namespace kz {
 
 KZ_DECLARE_STAGES(bls)
 KZ_DECLARE_STAGES(bn)
+KZ_DECLARE_STAGES(secp)
 #undef KZ_DECLARE_STAGES
 
 }  // namespace kz

@@ -30,7 +30,7 @@ extern "C" {
 #endif
 
 typedef enum gpukit_curve {
-    GPUKIT_CURVE_SECP256K1 = 0,   /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
+    GPUKIT_CURVE_SECP256K1 = 0,   /* 64-byte BE x || y points, like BN254 */
     GPUKIT_CURVE_BN254_G1 = 1,
     GPUKIT_CURVE_BLS12_381_G1 = 2,
     GPUKIT_CURVE_BANDERWAGON = 3, /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
@@ -44,7 +44,7 @@ int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t
 
 /* The HIP backend itself, a sibling of gpukit_multi_pippenger_{cpu,cuda,metal,wgsl}
  * (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92).  n == 0 gives the
- * identity; n >= 2^26 gives GPUKIT_ERR_BAD_SIZE; secp256k1 and Banderwagon give
+ * identity; n >= 2^26 gives GPUKIT_ERR_BAD_SIZE; Banderwagon gives
  * GPUKIT_ERR_NOTIMPL; device failures give GPUKIT_ERR_BACKEND (message via
  * kinet_hip_last_error()). */
 int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,

@@ -12,6 +12,8 @@ own sources.  The fixtures are data only: inputs and the reference's outputs.
   field_golden.json    field products from the reference's CIOS mont_mul
   g1_golden.json       k*G, P+Q, 2P from the reference's Jacobian formulas
   msm_golden.json      gpukit_multi_pippenger_cpu outputs on seeded inputs
+  msm_golden_secp.json the same for secp256k1 (curve 0), points (k0 + i d) G
+                       from pymath, scalars below 2^255
 """
 from __future__ import annotations
 
@@ -25,6 +27,7 @@ from concurrent.futures import ProcessPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 import oracle_lib as O  # noqa: E402
+import pymath as M  # noqa: E402
 
 REF = os.environ.get("KINET_REFERENCE", "/root/reference")
 MSM_SIZES = [1, 2, 8, 64, 256, 1024, 4096]
@@ -135,6 +138,58 @@ def msm_cases():
     return cases
 
 
+def secp_points(n, k0, d):
+    """(k0 + i d) G for i < n on secp256k1, as 64-byte BE wire points."""
+    P, D = M.mul(0, M.CURVES[0]["g"], k0), M.mul(0, M.CURVES[0]["g"], d)
+    out = []
+    for _ in range(n):
+        out.append(M.to_wire(0, P))
+        P = M.add(0, P, D)
+    return b"".join(out)
+
+
+def secp_scalars(n, seed):
+    """n seeded 32-byte LE scalars below 2^255 (the reference drops a final
+    window carry, which scalars below 2^255 never produce)."""
+    import random
+
+    rnd = random.Random(seed)
+    return b"".join(rnd.getrandbits(255).to_bytes(32, "little") for _ in range(n))
+
+
+def secp_msm_cases():
+    cases = []
+    for n in MSM_SIZES:
+        seed, k0, d = 5000 + n, 0x5EC9 + n, 0xD1FF0000 + 7 * n
+        scalars, points = secp_scalars(n, seed), secp_points(n, k0, d)
+        out = O.ref_msm(0, scalars, points)
+        case = {"curve": 0, "n": n, "seed": seed, "k0": k0, "d": d, "kind": "seeded",
+                "sha256_scalars": hashlib.sha256(scalars).hexdigest(),
+                "sha256_points": hashlib.sha256(points).hexdigest(), "expected": out.hex()}
+        if n <= EXPLICIT_MAX:
+            case["scalars"], case["points"] = scalars.hex(), points.hex()
+        cases.append(case)
+        print("msm secp256k1", n, out.hex()[:16], flush=True)
+    pts = secp_points(8, 0xABCDEF, 0x1234567)
+    pt = [pts[64 * i: 64 * (i + 1)] for i in range(8)]
+    sc = [secp_scalars(8, 99)[32 * i: 32 * (i + 1)] for i in range(8)]
+    neg = lambda w: w[:32] + ((M.SECP_P - int.from_bytes(w[32:], "big")) % M.SECP_P).to_bytes(32, "big")  # noqa: E731
+    edge = [
+        ("identity_points", [bytes(64)] * 3 + pt[:5], sc),
+        ("zero_scalars", pt, [bytes(32)] * 4 + sc[:4]),
+        ("repeated_point", [pt[0]] * 8, sc),
+        ("point_and_negation", [pt[1], neg(pt[1]), pt[2], neg(pt[2])], [sc[3]] * 4),
+        ("max_digits_below_2_255", pt[:4], [((1 << 255) - 1).to_bytes(32, "little"),
+                                            int("7f" + "80" * 31, 16).to_bytes(32, "little"),
+                                            (1 << 254).to_bytes(32, "little"), (1).to_bytes(32, "little")]),
+    ]
+    for name, p, s in edge:
+        pb_all, sb_all = b"".join(p), b"".join(s)
+        cases.append({"curve": 0, "n": len(s), "kind": "explicit", "name": name, "scalars": sb_all.hex(),
+                      "points": pb_all.hex(), "expected": O.ref_msm(0, sb_all, pb_all).hex()})
+    return cases
+
+
 def field_cases():
     import random
     rnd = random.Random(20261203)
@@ -176,6 +231,7 @@ def main():
     dump("field_golden.json", field_cases())
     dump("g1_golden.json", g1_cases())
     dump("msm_golden.json", msm_cases())
+    dump("msm_golden_secp.json", secp_msm_cases())
 
 
 if __name__ == "__main__":

@@ -101,12 +101,13 @@ void point_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
 
 extern "C" {
 
-// field: 0 BLS Fp, 1 BLS Fr, 2 BN Fp, 3 BN Fr
+// field: 0 BLS Fp, 1 BLS Fr, 2 BN Fp, 3 BN Fr, 4 secp256k1 Fp
 void hc_field_op(int field, int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
     switch (field) {
         case 0: field_op<BlsFp>(op, a, b, out, 48); break;
         case 1: field_op<BlsFr>(op, a, b, out, 32); break;
         case 2: field_op<BnFp>(op, a, b, out, 32); break;
+        case 4: field_op<SecpFp>(op, a, b, out, 32); break;
         default: field_op<BnFr>(op, a, b, out, 32); break;
     }
 }
@@ -141,7 +142,8 @@ void hc_glv_split_bls(const uint8_t* s32, uint8_t* out) {
 
 void hc_point_op(int curve, int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
     if (curve == 2) point_op<CurveBLS12381>(op, a, b, out);
-    else point_op<CurveBN254>(op, a, b, out);
+    else if (curve == 1) point_op<CurveBN254>(op, a, b, out);
+    else point_op<CurveSecp256k1>(op, a, b, out);
 }
 
 }  // extern "C"

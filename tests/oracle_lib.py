@@ -19,7 +19,7 @@ CURVE_BN254 = 1
 CURVE_BLS12_381 = 2
 FIELD_BLS_FP, FIELD_BLS_FR, FIELD_BN_FP, FIELD_BN_FR = 0, 1, 2, 3
 
-POINT_BYTES = {CURVE_BN254: 64, CURVE_BLS12_381: 96}
+POINT_BYTES = {0: 64, CURVE_BN254: 64, CURVE_BLS12_381: 96}  # 0 = secp256k1
 COMP_BYTES = {CURVE_BN254: 32, CURVE_BLS12_381: 48}
 
 _u8p = ctypes.c_char_p

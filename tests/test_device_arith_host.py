@@ -21,10 +21,10 @@ def hc():
     return ctypes.CDLL(LIB)
 
 
-FIELDS = {0: (M.BLS_P, 48), 1: (M.BLS_R, 32), 2: (M.BN_P, 32), 3: (M.BN_R, 32)}
+FIELDS = {0: (M.BLS_P, 48), 1: (M.BLS_R, 32), 2: (M.BN_P, 32), 3: (M.BN_R, 32), 4: (M.SECP_P, 32)}
 
 
-@pytest.mark.parametrize("field", [0, 1, 2, 3])
+@pytest.mark.parametrize("field", [0, 1, 2, 3, 4])
 def test_field_ops(hc, field):
     p, nb = FIELDS[field]
     rnd = random.Random(100 + field)
@@ -42,7 +42,7 @@ def test_field_ops(hc, field):
             assert int.from_bytes(out.raw, "little") == pow(a, p - 2, p), (field, a)
 
 
-@pytest.mark.parametrize("field", [0, 1, 2, 3])
+@pytest.mark.parametrize("field", [0, 1, 2, 3, 4])
 def test_field_inverse_many(hc, field):
     """Divstep inversion against Fermat's a^(p-2) over random elements and
     inputs whose gcd runs are long or short: powers of two, 2^k - 1, values
@@ -62,7 +62,7 @@ def test_field_inverse_many(hc, field):
         assert int.from_bytes(out.raw, "little") == pow(a, p - 2, p), (field, a)
 
 
-@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("curve", [0, 1, 2])
 def test_lazy_mixed_addition_chain(hc, curve):
     """Mixed additions of the accumulation loop (coordinates kept below 8p /
     4p / 2p between them), then made canonical: P + 3Q for random points, for
@@ -93,7 +93,7 @@ def test_lazy_mixed_addition_chain(hc, curve):
         assert out.raw == (M.to_wire(curve, M.mul(curve, G, want)) if want else bytes(pb)), (kp, kq, 20)
 
 
-@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("curve", [0, 1, 2])
 def test_point_ops(hc, curve):
     pb = O.POINT_BYTES[curve]
     rnd = random.Random(curve)

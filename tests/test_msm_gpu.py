@@ -54,10 +54,9 @@ def test_msm_empty_and_backend_env(ca, curve, monkeypatch):
 
 
 def test_msm_unsupported_curves(ca):
-    for curve in (0, 3):
-        with pytest.raises(ca.KinetError) as e:
-            ca.multi_pippenger(curve, bytes(32), bytes(64))
-        assert e.value.code == -5
+    with pytest.raises(ca.KinetError) as e:  # Banderwagon
+        ca.multi_pippenger(3, bytes(32), bytes(64))
+    assert e.value.code == -5
 
 
 def _closed_form(curve, scalars, k0, d):

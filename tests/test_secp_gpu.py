@@ -1,0 +1,123 @@
+"""secp256k1 (curve 0) MSMs through gpukit_multi_pippenger and the prepared
+plain path: the reference's own outputs (golden fixtures, and the compiled
+reference CPU Pippenger live where oracle/_ref is present), closed forms
+sum s_i (k0 + i d) G for full-range scalars, every window size, edge cases
+and refusals."""
+import hashlib
+import json
+import os
+import random
+
+import pytest
+
+import oracle_lib as O
+import pymath as M
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = json.load(open(os.path.join(GOLDEN, "msm_golden_secp.json")))
+G, N = M.CURVES[0]["g"], M.CURVES[0]["r"]
+
+
+@pytest.fixture(scope="module")
+def ca():
+    import crypto_amd
+
+    crypto_amd.lib.require_device()
+    return crypto_amd
+
+
+def _points(n, k0, d):
+    P, D = M.mul(0, G, k0), M.mul(0, G, d)
+    out = []
+    for _ in range(n):
+        out.append(M.to_wire(0, P))
+        P = M.add(0, P, D)
+    return b"".join(out)
+
+
+def _closed_form(scalars, k0, d):
+    t = sum(int.from_bytes(scalars[32 * i:32 * i + 32], "little") * ((k0 + i * d) % N)
+            for i in range(len(scalars) // 32))
+    return M.to_wire(0, M.mul(0, G, t % N)) if t % N else bytes(64)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c['n']}-{c.get('name', 'seeded')}")
+def test_secp_msm_golden(ca, case):
+    if "scalars" in case:
+        scalars, points = bytes.fromhex(case["scalars"]), bytes.fromhex(case["points"])
+    else:
+        rnd = random.Random(case["seed"])
+        scalars = b"".join(rnd.getrandbits(255).to_bytes(32, "little") for _ in range(case["n"]))
+        points = _points(case["n"], case["k0"], case["d"])
+        assert hashlib.sha256(points).hexdigest() == case["sha256_points"]
+    print(f"secp256k1 golden n={case['n']} {case.get('name', '')}", flush=True)
+    assert ca.multi_pippenger(0, scalars, points).hex() == case["expected"]
+
+
+@pytest.mark.parametrize("n", [3, 777, 5000])
+def test_secp_msm_matches_reference_live(ca, n):
+    """Random points and scalars below 2^255 against the reference's compiled
+    CPU Pippenger (oracle/_ref), with identity points and repeats mixed in."""
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rnd = random.Random(n)
+    pts = _points(n, rnd.getrandbits(250), rnd.getrandbits(250))
+    pl = [pts[64 * i:64 * i + 64] for i in range(n)]
+    for i in range(0, n, 17):
+        pl[i] = bytes(64)
+    for i in range(5, n, 23):
+        pl[i] = pl[i - 1]
+    points = b"".join(pl)
+    scalars = b"".join(rnd.getrandbits(255).to_bytes(32, "little") for _ in range(n))
+    print(f"secp256k1 vs reference n={n}", flush=True)
+    assert ca.multi_pippenger(0, scalars, points) == O.ref_msm(0, scalars, points)
+
+
+@pytest.mark.parametrize("n", [1000, 1 << 16])
+def test_secp_msm_closed_form_full_range(ca, n):
+    """Scalars over all of [0, 2^256), with n - 1, n, 2^256 - 1 and zeros:
+    the result is sum s_i P_i (the final window carry is kept)."""
+    k0, d = 0xC0FFEE12345, 0xBADC0DE987654321
+    points = _points(n, k0, d)
+    rnd = random.Random(7 + n)
+    sc = [rnd.getrandbits(256) for _ in range(n)]
+    sc[:6] = [N - 1, N, N + 1, (1 << 256) - 1, 0, 1 << 255]
+    scalars = b"".join(s.to_bytes(32, "little") for s in sc)
+    print(f"secp256k1 closed form n={n}", flush=True)
+    assert ca.multi_pippenger(0, scalars, points) == _closed_form(scalars, k0, d)
+
+
+def test_secp_msm_every_window_size(ca):
+    """Every instantiated window size of the prepared plain path gives the
+    same bytes (c = 8, 16 with the carry window; 17, 18 with the global sort)."""
+    from crypto_amd import device as D
+
+    n, k0, d = 5000, 91, 0x7777777777777777
+    points = _points(n, k0, d)
+    rnd = random.Random(3)
+    sc = [rnd.getrandbits(256) for _ in range(n)]
+    sc[:40] = [(1 << 256) - 1 - i for i in range(40)]
+    sc[40:50] = [0] * 10
+    scalars = b"".join(s.to_bytes(32, "little") for s in sc)
+    want = _closed_form(scalars, k0, d)
+    pp = D.PreparedPoints(0, points)
+    d_sc = D.to_device(scalars)
+    for c in range(5, 19):
+        print(f"secp256k1 window bits c={c}", flush=True)
+        out = D.msm_prepared(pp, d_sc, window_bits=c).cpu().numpy().tobytes()
+        assert out[:64] == want, c
+        assert out[64:] == bytes(32), c  # no compressed form for secp256k1
+
+
+def test_secp_msm_empty_and_refusals(ca):
+    assert ca.multi_pippenger(0, b"", b"") == bytes(64)
+    assert ca.multi_pippenger(0, (3).to_bytes(32, "little"), M.to_wire(0, G)) == M.to_wire(0, M.mul(0, G, 3))
+    lib = ca.lib.require_device()
+    # window tables and the G1 (GLV) form are BN254 / BLS12-381 only
+    assert lib.kinet_hip_table_bytes(0, 16, 8) == 0
+    assert lib.kinet_hip_msm_prepared_g1(0, None, None, None, 0, 0, 0, None, None) == -3
+    with pytest.raises(ca.KinetError) as e:
+        ca.multi_pippenger(3, bytes(32), bytes(64))
+    assert e.value.code == -5
