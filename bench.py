@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "G1 MSM Mpoints/s at N=2^20; batched commits/s + p99 vs plain GPU path"
-CURVE_NAMES = {1: "bn254", 2: "bls12_381"}
+CURVE_NAMES = {0: "secp256k1", 1: "bn254", 2: "bls12_381"}
 
 
 def parse_args():
@@ -676,6 +676,11 @@ def main():
                                             g1=False)
         extras[f"{CURVE_NAMES[other]}_plain_path_mpoints_s"] = round(n / oms / 1e3, 3)
         extras[f"{CURVE_NAMES[other]}_plain_path_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
+        # secp256k1 (curve 0 of the multi-curve entry): plain path only
+        sms, _, sres, scheck, _ = timed_msm(lib, torch, D, 0, n, 4242, max(3, args.steps // 2), 2, 0, False,
+                                            g1=False)
+        extras["secp256k1_plain_path_mpoints_s"] = round(n / sms / 1e3, 3)
+        extras["secp256k1_plain_path_verified"] = sres[:64] == scheck
         # fixed point sets (an SRS): window table built once, then one window per MSM
         for cv, seed in ((curve, 20261203), (other, 777)):
             tms, tst, tres, tcheck, text = timed_msm(lib, torch, D, cv, n, seed, max(3, args.steps // 2), 2, 0,

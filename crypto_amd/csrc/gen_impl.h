@@ -22,7 +22,8 @@ template <class C>
 __device__ Affine<typename C::Fp> generator_affine() {
     using F = typename C::Fp;
     // BLS12-381 G1 generator: bls/cpp/bls12_381_g1.hpp:204-224; BN254: (1, 2)
-    // (gpukit/test/multi_pippenger_test.cpp:207-211).  Canonical, big-endian.
+    // (gpukit/test/multi_pippenger_test.cpp:207-211); secp256k1: SEC 2's G
+    // (secp256k1/cpp/field.hpp:88-91).  Canonical, big-endian.
     constexpr uint8_t bls_x[48] = {
         0x17, 0xF1, 0xD3, 0xA7, 0x31, 0x97, 0xD7, 0x94, 0x26, 0x95, 0x63, 0x8C, 0x4F, 0xA9, 0xAC, 0x0F,
         0xC3, 0x68, 0x8C, 0x4F, 0x97, 0x74, 0xB9, 0x05, 0xA1, 0x4E, 0x3A, 0x3F, 0x17, 0x1B, 0xAC, 0x58,
@@ -31,10 +32,19 @@ __device__ Affine<typename C::Fp> generator_affine() {
         0x08, 0xB3, 0xF4, 0x81, 0xE3, 0xAA, 0xA0, 0xF1, 0xA0, 0x9E, 0x30, 0xED, 0x74, 0x1D, 0x8A, 0xE4,
         0xFC, 0xF5, 0xE0, 0x95, 0xD5, 0xD0, 0x0A, 0xF6, 0x00, 0xDB, 0x18, 0xCB, 0x2C, 0x04, 0xB3, 0xED,
         0xD0, 0x3C, 0xC7, 0x44, 0xA2, 0x88, 0x8A, 0xE4, 0x0C, 0xAA, 0x23, 0x29, 0x46, 0xC5, 0xE7, 0xE1};
+    constexpr uint8_t secp_x[32] = {
+        0x79, 0xBE, 0x66, 0x7E, 0xF9, 0xDC, 0xBB, 0xAC, 0x55, 0xA0, 0x62, 0x95, 0xCE, 0x87, 0x0B, 0x07,
+        0x02, 0x9B, 0xFC, 0xDB, 0x2D, 0xCE, 0x28, 0xD9, 0x59, 0xF2, 0x81, 0x5B, 0x16, 0xF8, 0x17, 0x98};
+    constexpr uint8_t secp_y[32] = {
+        0x48, 0x3A, 0xDA, 0x77, 0x26, 0xA3, 0xC4, 0x65, 0x5D, 0xA4, 0xFB, 0xFC, 0x0E, 0x11, 0x08, 0xA8,
+        0xFD, 0x17, 0xB4, 0x48, 0xA6, 0x85, 0x54, 0x19, 0x9C, 0x47, 0xD0, 0x8F, 0xFB, 0x10, 0xD4, 0xB8};
     Affine<F> g;
     if constexpr (C::ID == 2) {
         g.x = fe_to_mont<F>(fe_from_be_bytes<F>(bls_x, 48));
         g.y = fe_to_mont<F>(fe_from_be_bytes<F>(bls_y, 48));
+    } else if constexpr (C::ID == 0) {
+        g.x = fe_to_mont<F>(fe_from_be_bytes<F>(secp_x, 32));
+        g.y = fe_to_mont<F>(fe_from_be_bytes<F>(secp_y, 32));
     } else {
         Fe<F> x = fe_zero<F>(), y = fe_zero<F>();
         x.v[0] = 1;

@@ -121,3 +121,21 @@ def test_secp_msm_empty_and_refusals(ca):
     with pytest.raises(ca.KinetError) as e:
         ca.multi_pippenger(3, bytes(32), bytes(64))
     assert e.value.code == -5
+
+
+def test_secp_fixed_base(ca):
+    """kinet_hip_g1_fixed_base_mul on curve 0 (the bench's point generator)
+    against double-and-add, for 0, 1, n - 1, n, 2^256 - 1 and random scalars."""
+    import torch
+
+    lib = ca.lib.require_device()
+    rnd = random.Random(11)
+    ks = [0, 1, 2, N - 1, N, (1 << 256) - 1] + [rnd.getrandbits(256) for _ in range(40)]
+    d_k = torch.tensor(list(b"".join(k.to_bytes(32, "little") for k in ks)), dtype=torch.uint8, device="cuda")
+    out = torch.empty(64 * len(ks), dtype=torch.uint8, device="cuda")
+    assert lib.kinet_hip_g1_fixed_base_mul(0, d_k.data_ptr(), len(ks), out.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().tobytes()
+    for i, k in enumerate(ks):
+        want = M.to_wire(0, M.mul(0, G, k % N)) if k % N else bytes(64)
+        assert got[64 * i:64 * i + 64] == want, k
