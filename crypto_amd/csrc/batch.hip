@@ -134,7 +134,7 @@ int kinet_hip_msm_set_window_table(uint32_t curve, int mode, const int* c_by_log
 }
 
 int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode) {
-    if ((curve != 1 && curve != 2) || mode < 0 || mode >= kMsmModes) return -1;
+    if (!msm_tunable((int)curve, mode)) return -1;
     return msm_window_bits((int)curve, n, mode);
 }
 

@@ -138,6 +138,12 @@ struct MsmBatchPlan {
 // Tuning modes: one MSM, a job of a batch, one MSM over G1 points through
 // the GLV split (2n points of 128-bit scalars).
 enum MsmMode { kMsmPlain = 0, kMsmBatched = 1, kMsmGlv = 2, kMsmModes = 3 };
+
+// (curve, mode) pairs with a tuning table: every mode of BN254 (1) and
+// BLS12-381 (2); secp256k1 (0) runs single plain MSMs only.
+inline bool msm_tunable(int curve, int mode) {
+    return mode >= 0 && mode < kMsmModes && (curve == 1 || curve == 2 || (curve == 0 && mode == kMsmPlain));
+}
 // A GLV cell holding kMsmGlvOff instead of window bits records that the
 // plain MSM beat the split at that size class: G1 MSMs of that class without
 // explicit window bits then run the plain path (same result).

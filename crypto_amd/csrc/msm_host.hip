@@ -40,7 +40,7 @@ int floor_log2(size_t n) {
 }
 
 void check_mode(int curve, int mode) {
-    if (curve < 1 || curve > 2 || mode < 0 || mode >= kMsmModes) throw std::invalid_argument("bad curve or mode");
+    if (!msm_tunable(curve, mode)) throw std::invalid_argument("bad curve or mode");
 }
 
 }  // namespace
@@ -48,7 +48,7 @@ void check_mode(int curve, int mode) {
 int msm_window_bits(int curve, size_t n, int mode) {
     const int lg = floor_log2(n);
     int c = 0;
-    if (curve >= 1 && curve <= 2 && mode >= 0 && mode < kMsmModes && lg < 32) {
+    if (msm_tunable(curve, mode) && lg < 32) {
         std::lock_guard<std::mutex> g(g_table_mu);
         c = g_table[curve][mode][lg].c;
     }
@@ -67,7 +67,7 @@ bool msm_glv_enabled(int curve, size_t n) {
 
 int msm_tuned_segment(int curve, size_t n, int mode) {
     const int lg = floor_log2(n);
-    if (curve < 1 || curve > 2 || mode < 0 || mode >= kMsmModes || lg >= 32) return 0;
+    if (!msm_tunable(curve, mode) || lg >= 32) return 0;
     std::lock_guard<std::mutex> g(g_table_mu);
     return g_table[curve][mode][lg].seg;
 }

@@ -28,7 +28,9 @@ struct Mismatch : std::runtime_error {
 };
 
 // Seeded 256-bit values below the scalar field modulus of `curve` (top byte
-// masked: BLS12-381 r < 2^255 starts 0x73, BN254 r starts 0x30), uploaded.
+// masked: BLS12-381 r < 2^255 starts 0x73, BN254 r starts 0x30; secp256k1's
+// group order is within 2^129 of 2^256, so its scalars take all 256 bits),
+// uploaded.
 void fill_scalars(int curve, uint64_t seed, size_t n, void* d, hipStream_t s) {
     std::vector<uint64_t> h(n * 4);
     uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
@@ -39,7 +41,7 @@ void fill_scalars(int curve, uint64_t seed, size_t n, void* d, hipStream_t s) {
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         w = z ^ (z >> 31);
     }
-    const uint64_t top = curve == 2 ? 0x3FFFFFFFFFFFFFFFull : 0x1FFFFFFFFFFFFFFFull;
+    const uint64_t top = curve == 0 ? ~0ull : curve == 2 ? 0x3FFFFFFFFFFFFFFFull : 0x1FFFFFFFFFFFFFFFull;
     for (size_t i = 0; i < n; ++i) h[4 * i + 3] &= top;
     KZ_HIP(hipMemcpyAsync(d, h.data(), n * 32, hipMemcpyHostToDevice, s));
     KZ_HIP(hipStreamSynchronize(s));  // h is freed on return
@@ -254,8 +256,7 @@ Choice tune_batched(Engine& e, int curve, int lg, const MsmInputs& in, size_t jo
 }
 
 void tune_msm(int curve, int mode, int log_min, int log_max) {
-    if (curve != 1 && curve != 2) throw std::invalid_argument("curve");
-    if (mode < 0 || mode >= kMsmModes) throw std::invalid_argument("mode");
+    if (!msm_tunable(curve, mode)) throw std::invalid_argument("curve or mode");
     if (log_min < 1 || log_max > 24 || log_min > log_max) throw std::invalid_argument("size classes out of range");
     if (mode == kMsmGlv && log_min < 11) throw std::invalid_argument("GLV size classes start at 2^11");
     Engine& e = engine();
@@ -354,7 +355,7 @@ std::vector<Cell> parse_table(const std::string& path) {
         if (kind == "msm") {
             c.msm = true;
             ok = (bool)(ls >> c.a >> c.b >> c.lg >> c.x >> c.y) && !(ls >> extra);
-            ok = ok && (c.a == 1 || c.a == 2) && c.b >= 0 && c.b < kMsmModes && c.lg >= 0 && c.lg < 32 &&
+            ok = ok && msm_tunable(c.a, c.b) && c.lg >= 0 && c.lg < 32 &&
                  (c.x == 0 || (c.x >= 5 && c.x <= 18) || (c.b == kMsmGlv && c.x == kMsmGlvOff)) &&
                  (c.y == 0 || (c.y > 0 && c.y <= 16 && (c.y & (c.y - 1)) == 0));
         } else if (kind == "ntt") {
@@ -379,12 +380,13 @@ void load_table(const std::string& path) {
 void save_table(const std::string& path) {
     std::ostringstream o;
     o << "# MSM and NTT schedules per size class k = floor(log2 n)\n"
-      << "# msm <curve 1=BN254 2=BLS12-381> <mode 0 single 1 batched 2 GLV> <k> <window bits> <segment>\n"
+      << "# msm <curve 0=secp256k1 1=BN254 2=BLS12-381> <mode 0 single 1 batched 2 GLV> <k> <window bits> "
+         "<segment>\n"
       << "# (a GLV cell of window bits 1: the plain MSM is faster at that size class)\n"
       << "# ntt <field 0=BLS12-381 Fr 1=BN254 Fr> <k> <log2 pass 1> <pass 2> <pass 3>\n";
-    for (int curve = 1; curve <= 2; ++curve)
+    for (int curve = 0; curve <= 2; ++curve)
         for (int mode = 0; mode < kMsmModes; ++mode)
-            for (int lg = 0; lg < 32; ++lg) {
+            for (int lg = 0; lg < 32 && msm_tunable(curve, mode); ++lg) {
                 int c = 0, seg = 0;
                 msm_get_tuned(curve, mode, lg, c, seg);
                 if (c || seg) o << "msm " << curve << ' ' << mode << ' ' << lg << ' ' << c << ' ' << seg << '\n';
@@ -401,9 +403,9 @@ void save_table(const std::string& path) {
 }
 
 void clear_tables() {
-    for (int curve = 1; curve <= 2; ++curve)
+    for (int curve = 0; curve <= 2; ++curve)
         for (int mode = 0; mode < kMsmModes; ++mode)
-            for (int lg = 0; lg < 32; ++lg) msm_set_tuned(curve, mode, lg, 0, 0);
+            for (int lg = 0; lg < 32 && msm_tunable(curve, mode); ++lg) msm_set_tuned(curve, mode, lg, 0, 0);
     for (int field = 0; field <= 1; ++field)
         for (int lg = 0; lg <= kNttMaxLog; ++lg) ntt_set_schedule(field, lg, 0, 0, 0);
 }

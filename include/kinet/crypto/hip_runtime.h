@@ -146,8 +146,8 @@ int kinet_hip_msm_window_bits(uint32_t curve, size_t n, int mode);
  * for mode 1; for mode 2 also the plain MSM under its mode-0 cell); NTT every
  * split of log2 n into one to three passes of at most 2^10.  Synchronous;
  * holds the library's device lock, so other calls wait.  Mode 2 needs
- * log_min >= 11 (smaller GLV MSMs run the plain path).  Sizes are limited to
- * log2 n <= 24. */
+ * log_min >= 11 (smaller GLV MSMs run the plain path).  Curve 0 (secp256k1)
+ * has mode 0 only.  Sizes are limited to log2 n <= 24. */
 int kinet_hip_tune_msm(uint32_t curve, int mode, int log_min, int log_max);
 int kinet_hip_tune_ntt(uint32_t field, int log_min, int log_max);
 

@@ -15,13 +15,14 @@ def T():
 def test_load_save_clear(T, tmp_path):
     p = tmp_path / "a.tune"
     p.write_text("# comment\n\nmsm 2 0 20 16 8\nmsm 1 1 12 9 0  # window only\nmsm 2 2 18 14 4\nntt 0 20 7 7 6\n"
-                 "ntt 1 11 1 10 0\n")
+                 "ntt 1 11 1 10 0\nmsm 0 0 16 11 4\n")
     T.load(str(p))
     assert T.msm_cell(2, 0, 20) == (16, 8)
     assert T.msm_cell(1, 1, 12) == (9, 0)
     assert T.window_bits(2, (1 << 18) + 5, T.MODE_GLV) == 14
     assert T.window_bits(1, 5000, T.MODE_BATCHED) == 9
     assert T.window_bits(1, 3000, T.MODE_BATCHED) == 11 - 4  # rule: floor(log2 n) - 4
+    assert T.msm_cell(0, 0, 16) == (11, 4) and T.window_bits(0, 70000) == 11  # secp256k1 single MSMs
     assert T.ntt_split(0, 20) == (7, 7, 6)
     assert T.ntt_split(1, 11) == (1, 10, 0)
     assert T.ntt_split(0, 21) == (7, 7, 7)  # built-in
@@ -31,7 +32,7 @@ def test_load_save_clear(T, tmp_path):
     T.save(str(q))
     lines = [ln for ln in q.read_text().splitlines() if ln and not ln.startswith("#")]
     assert sorted(lines) == sorted(["msm 2 0 20 16 8", "msm 1 1 12 9 0", "msm 2 2 18 14 4", "ntt 0 20 7 7 6",
-                                    "ntt 1 11 1 10 0"])
+                                    "ntt 1 11 1 10 0", "msm 0 0 16 11 4"])
     T.clear()
     assert T.msm_cell(2, 0, 20) == (0, 0) and T.ntt_split(0, 20) == (10, 10, 0)
     T.load(str(q))
@@ -56,7 +57,8 @@ def test_glv_off_cell(T, tmp_path):
                                   "msm 3 0 20 16 8", "msm 2 3 20 16 8", "msm 2 0 32 16 8", "msm 2 0 20 16",
                                   "msm 2 0 20 16 8 1", "ntt 0 20 10 11 0", "ntt 0 20 11 9 0", "ntt 0 20 0 20 0",
                                   "ntt 0 12 4 0 8", "ntt 2 12 6 6 0", "ntt 0 27 9 9 9", "ntx 0 12 6 6 0",
-                                  "msm two 0 20 16 8", "msm 2 0 20 1 0", "msm 2 1 20 1 0"])
+                                  "msm two 0 20 16 8", "msm 2 0 20 1 0", "msm 2 1 20 1 0",
+                                  "msm 0 1 16 10 0", "msm 0 2 16 10 0", "msm -1 0 16 10 0"])
 def test_malformed_lines_install_nothing(T, tmp_path, line):
     import crypto_amd
 
