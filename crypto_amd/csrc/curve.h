@@ -337,9 +337,13 @@ KZ_DEV void xyzz_to_affine(const Xyzz<F>& p, Affine<F>& out, bool& inf) {
 // 16-byte aligned and moves as whole dwordx4 transfers:
 //   affine  x[N] y[N]            -> kAffWords  (BLS12-381 28, BN254 20)
 //   XYZZ    X[N] Y[N] ZZ[N] ZZZ[N] -> kXyzzWords (BLS12-381 56, BN254 36)
+// Affine records sit kAffStride words apart: BLS12-381's 112-byte records
+// take a 128-byte slot, so a random gather touches one cache line instead of
+// two about half the time (the accumulation gathers every point once per window).
 template <class F>
 struct Rec {
     static constexpr int kAffWords = (2 * F::N + 3) / 4 * 4;
+    static constexpr int kAffStride = kAffWords == 28 ? 32 : kAffWords;
     static constexpr int kXyzzWords = (4 * F::N + 3) / 4 * 4;
 };
 
@@ -369,7 +373,7 @@ template <class F>
 KZ_DEV Affine<F> affine_load(const uint32_t* __restrict__ base, uint32_t idx) {
     constexpr int K = Rec<F>::kAffWords;
     uint32_t w[K];
-    load_words<K>(base + (size_t)idx * K, w);
+    load_words<K>(base + (size_t)idx * Rec<F>::kAffStride, w);
     Affine<F> a;
 #pragma unroll
     for (int i = 0; i < F::N; ++i) {
@@ -381,7 +385,7 @@ KZ_DEV Affine<F> affine_load(const uint32_t* __restrict__ base, uint32_t idx) {
 
 template <class F>
 KZ_DEV void affine_store(uint32_t* __restrict__ base, size_t idx, const Affine<F>& a) {
-    constexpr int K = Rec<F>::kAffWords;
+    constexpr int K = Rec<F>::kAffStride;  // padding written as zeros
     uint32_t w[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) w[i] = 0;

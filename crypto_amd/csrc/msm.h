@@ -93,7 +93,7 @@ inline size_t point_wire_bytes(int curve) { return curve == 2 ? 96 : 64; }
 inline size_t compressed_bytes(int curve) { return curve == 2 ? 48 : 32; }
 // Device record sizes in 32-bit words (radix-2^29 limbs, padded to 16 bytes):
 // affine points 28 (BLS12-381) / 20 (BN254, secp256k1), XYZZ points 56 / 36 / 40.
-inline int affine_words(int curve) { return curve == 2 ? 28 : 20; }
+inline int affine_words(int curve) { return curve == 2 ? 32 : 20; }  // record stride
 inline int xyzz_words(int curve) { return curve == 2 ? 56 : curve == 1 ? 36 : 40; }
 
 // Wire affine points (BE, zero = identity) -> Montgomery affine (2N words each)

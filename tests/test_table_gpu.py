@@ -105,7 +105,7 @@ def test_table_argument_errors(D):
                                    out.data_ptr(), None) == -2
     assert lib.kinet_hip_points_table_build(2, pp.pts.data_ptr(), pp.inf.data_ptr(), 64, 19,
                                             tab.table.data_ptr(), None) == -2
-    assert lib.kinet_hip_table_bytes(3, 64, 8) == 0 and lib.kinet_hip_table_bytes(2, 64, 8) == 33 * 64 * 28 * 4
+    assert lib.kinet_hip_table_bytes(3, 64, 8) == 0 and lib.kinet_hip_table_bytes(2, 64, 8) == 33 * 64 * 32 * 4
     assert lib.kinet_hip_msm_table(2, ctypes.c_void_p(0), tab.table.data_ptr(), None, 64, 0, 8, out.data_ptr(),
                                    None) == 0
     torch = pytest.importorskip("torch")
