@@ -171,6 +171,11 @@ int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void*
     return msm_prepared_impl(curve, d_scalars, d_pts, d_inf, n, window_bits, chunk, d_out, stream, true);
 }
 
+size_t kinet_hip_points_bytes(uint32_t curve, size_t n) {
+    if (!valid_plain_curve(curve)) return 0;
+    return (size_t)affine_words((int)curve) * sizeof(uint32_t) * n;
+}
+
 int kinet_hip_table_window_bits(size_t n) { return msm_table_window_bits(n); }
 
 size_t kinet_hip_table_bytes(uint32_t curve, size_t n, int window_bits) {

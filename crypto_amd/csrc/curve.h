@@ -335,7 +335,8 @@ KZ_DEV void xyzz_to_affine(const Xyzz<F>& p, Affine<F>& out, bool& inf) {
 // ------------------------------------------------------------ memory I/O
 // Point records in HBM are padded to a multiple of 4 words so every record is
 // 16-byte aligned and moves as whole dwordx4 transfers:
-//   affine  x[N] y[N]            -> kAffWords  (BLS12-381 28, BN254 20)
+//   affine  x[N] y[N]            -> kAffWords  (BLS12-381 28, BN254 20) of
+//                                   data in a kAffStride slot (32, 20)
 //   XYZZ    X[N] Y[N] ZZ[N] ZZZ[N] -> kXyzzWords (BLS12-381 56, BN254 36)
 // Affine records sit kAffStride words apart: BLS12-381's 112-byte records
 // take a 128-byte slot, so a random gather touches one cache line instead of

@@ -7,8 +7,19 @@
 
 namespace kz {
 
-// Device scratch reused across calls (grow-only).
+// Device scratch reused across calls (grow-only).  Successive MSMs over one
+// workspace are ordered on the device even when they run on different
+// streams: each records `last_use` on its stream, and the next one's stream
+// waits for it first (outside graph capture).
 struct MsmWorkspace {
+    MsmWorkspace() = default;
+    MsmWorkspace(const MsmWorkspace&) = delete;
+    MsmWorkspace& operator=(const MsmWorkspace&) = delete;
+    ~MsmWorkspace() {
+        if (last_use) (void)hipEventDestroy(last_use);
+    }
+    hipEvent_t last_use = nullptr;
+    hipStream_t last_stream = nullptr;
     DevBuf counts, offsets, cursor, block_sums, total;
     DevBuf entries;
     DevBuf bsum, carry, run_bucket, segA, segB, winres;
@@ -92,7 +103,8 @@ inline size_t msm_out_bytes(int curve) { return curve == 2 ? 96 + 48 : 64 + 32; 
 inline size_t point_wire_bytes(int curve) { return curve == 2 ? 96 : 64; }
 inline size_t compressed_bytes(int curve) { return curve == 2 ? 48 : 32; }
 // Device record sizes in 32-bit words (radix-2^29 limbs, padded to 16 bytes):
-// affine points 28 (BLS12-381) / 20 (BN254, secp256k1), XYZZ points 56 / 36 / 40.
+// affine points 28 words of data in a 32-word (128-byte) slot for BLS12-381,
+// 20 for BN254 and secp256k1; XYZZ points 56 / 36 / 40.
 inline int affine_words(int curve) { return curve == 2 ? 32 : 20; }  // record stride
 inline int xyzz_words(int curve) { return curve == 2 ? 56 : curve == 1 ? 36 : 40; }
 

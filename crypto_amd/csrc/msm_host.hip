@@ -301,6 +301,29 @@ void atomic_sort(const Shape& sh, Digits&& digits, MsmWorkspace& ws, hipStream_t
     digits(true, ws.cursor.as<uint32_t>(), ws.entries.as<uint32_t>());
 }
 
+// Scope of one MSM's use of a workspace: its stream first waits for the event
+// the previous user recorded (when that was another stream), and records the
+// event again at the end.  Skipped while the stream is being captured into a
+// graph: the order of graph replays is the caller's.
+struct WorkspaceUse {
+    MsmWorkspace& ws;
+    hipStream_t s;
+    bool on = true;
+    WorkspaceUse(MsmWorkspace& w, hipStream_t st) : ws(w), s(st) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) on = false;
+        if (on && ws.last_use && ws.last_stream != s) KZ_HIP(hipStreamWaitEvent(s, ws.last_use, 0));
+    }
+    ~WorkspaceUse() {
+        if (!on) return;
+        if (!ws.last_use && hipEventCreateWithFlags(&ws.last_use, hipEventDisableTiming) != hipSuccess) {
+            ws.last_use = nullptr;
+            return;
+        }
+        if (hipEventRecord(ws.last_use, s) == hipSuccess) ws.last_stream = s;
+    }
+};
+
 // `sort()` fills ws.offsets and ws.entries.
 template <class Sort>
 void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPlan* plan,
@@ -358,6 +381,7 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(curve);
+    WorkspaceUse use(ws, s);
     // GLV (points in G1): 2n points with scalar magnitudes below 2^127
     int bits = 256;
     size_t npts = n;
@@ -600,6 +624,7 @@ void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_
                    const uint32_t* d_scalars, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(plan.curve);
     const Shape sh = batch_shape(plan);
+    WorkspaceUse use(ws, s);
     msm_batch_reserve(plan, ws);
     const MsmJob* d_jobs = plan.d_jobs.as<MsmJob>();
     auto sort = [&] {

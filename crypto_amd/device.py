@@ -34,7 +34,7 @@ class PreparedPoints:
         pb = point_bytes(curve)
         d_wire = wire if isinstance(wire, torch.Tensor) else to_device(wire, device)
         self.n = d_wire.numel() // pb
-        self.pts = torch.empty(self.n * affine_words(curve) * 4 + 16, dtype=torch.uint8, device=device)
+        self.pts = torch.empty(lib.kinet_hip_points_bytes(curve, self.n) + 16, dtype=torch.uint8, device=device)
         self.inf = torch.empty(self.n + 16, dtype=torch.uint8, device=device)
         rc = lib.kinet_hip_points_prepare(curve, d_wire.data_ptr(), self.n, self.pts.data_ptr(),
                                           self.inf.data_ptr(), stream_handle())

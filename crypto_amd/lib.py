@@ -65,6 +65,8 @@ def library():
         lib.bls12_381_g1_msm.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
         lib.bls12_381_g1_msm.restype = i32
         lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
+        lib.kinet_hip_points_bytes.argtypes = [u32, sz]
+        lib.kinet_hip_points_bytes.restype = sz
         lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         lib.kinet_hip_msm_prepared_g1.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         lib.kinet_hip_points_check_g1.argtypes = [u32, vp, vp, sz, vp, vp]

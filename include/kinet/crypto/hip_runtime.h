@@ -6,8 +6,12 @@
  * serve callers whose inputs already live in device memory (benchmarks,
  * batching and serving layers).  Device pointers are HIP device addresses;
  * `stream` is a hipStream_t (NULL = the null stream).  Calls are asynchronous
- * on that stream; concurrent calls that share the library's MSM scratch are
- * serialised on the host.
+ * on that stream.  The MSM entries without a plan or context of their own
+ * (kinet_hip_msm_prepared, _prepared_g1, _msm_table) share one scratch
+ * workspace: the host enqueues them one at a time, and each call's stream
+ * waits (hipStreamWaitEvent) for the previous user of that workspace, so calls
+ * on different streams run one after the other on the device and never
+ * overwrite each other's scratch.
  */
 #pragma once
 #include <stddef.h>
@@ -24,10 +28,17 @@ int kinet_hip_available(void);
 const char* kinet_hip_last_error(void);
 
 /* Wire points (BE affine, zero = identity) -> device form: Montgomery affine
- * records of 28 (BLS12-381) or 20 (BN254) 32-bit words per point (radix-2^29
- * limbs, 16-byte aligned), and a byte per point flagging the identity. */
+ * records of 32 (BLS12-381: 28 words of radix-2^29 limbs in a 128-byte slot)
+ * or 20 (BN254, secp256k1) 32-bit words per point, 16-byte aligned, and a byte
+ * per point flagging the identity.  d_pts must hold kinet_hip_points_bytes(curve,
+ * n) bytes and d_inf n bytes. */
 int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
                              void* stream);
+
+/* Device bytes of n prepared points of `curve` (the d_pts size of
+ * kinet_hip_points_prepare; 0 for an unknown curve).  Authoritative: record
+ * sizes follow the device layout, which callers should not hard-code. */
+size_t kinet_hip_points_bytes(uint32_t curve, size_t n);
 
 /* MSM over prepared points.  d_scalars: n x 32 bytes LE.  window_bits: 0 picks
  * from n, else 5..18.  chunk: sorted bucket entries per accumulation thread,

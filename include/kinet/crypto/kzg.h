@@ -47,9 +47,10 @@ uint32_t kzg_srs_curve(const kzg_srs* srs);
  * nearest to the window width the library picks for their size; batched and
  * served commits take, per request, a table within 2 bits of the request's
  * batched window width (else ordinary windows).  A table MSM reduces one
- * window and combines none.  Costs R * size * 112 (BLS12-381) or 80 (BN254)
- * bytes of device memory per table: 7.5 GB for a 2^22-point BLS12-381 SRS at
- * c = 16.  window_bits 0 = the library's choice for the SRS size, else 5..18;
+ * window and combines none.  Costs R * size * 128 (BLS12-381) or 80 (BN254)
+ * bytes of device memory per table: 9.1 GB for a 2^22-point BLS12-381 SRS at
+ * c = 16 (kzg_srs_table_bytes gives the exact figure afterwards;
+ * kinet_hip_table_bytes the size of one table beforehand).  window_bits 0 = the library's choice for the SRS size, else 5..18;
  * all tables together must stay under 2^31 points.  Synchronous; call it
  * before the SRS is shared with other threads.  Results are unchanged byte
  * for byte.  No counterpart in the reference. */

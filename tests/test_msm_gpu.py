@@ -293,3 +293,79 @@ def test_msm_g1_glv_bn254(ca, n, c):
     plain = D.msm_prepared(pp, d_sc).cpu().numpy().tobytes()
     assert got[:pb] == want
     assert got == plain
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+def test_points_prepare_fills_exactly_the_queried_size(ca, curve):
+    """kinet_hip_points_prepare writes kinet_hip_points_bytes(curve, n) bytes of
+    records and n flag bytes, nothing beyond: guard bands after both buffers
+    keep their fill pattern, and the records give the closed-form MSM."""
+    import torch
+    from crypto_amd import device as D
+
+    lib = ca.library()
+    n = 777
+    want_rec = {0: 80, 1: 80, 2: 128}[curve]
+    nbytes = lib.kinet_hip_points_bytes(curve, n)
+    assert nbytes == want_rec * n
+    assert lib.kinet_hip_points_bytes(3, n) == 0
+    k0, d = 99, 0x10001
+    if curve == 0:  # secp256k1: the oracle has no secp256k1; pure-Python points
+        G, r = M.CURVES[0]["g"], M.CURVES[0]["r"]
+        P, Dd, pl = M.mul(0, G, k0), M.mul(0, G, d), []
+        for _ in range(n):
+            pl.append(M.to_wire(0, P))
+            P = M.add(0, P, Dd)
+        points = b"".join(pl)
+        rnd = random.Random(5150)
+        scalars = b"".join(rnd.getrandbits(255).to_bytes(32, "little") for _ in range(n))
+        t = sum(int.from_bytes(scalars[32 * i:32 * i + 32], "little") * (k0 + i * d) for i in range(n)) % r
+        want = M.to_wire(0, M.mul(0, G, t)) if t else bytes(64)
+    else:
+        points = O.gen_points_progression(curve, k0, d, n)
+        scalars = O.gen_scalars(curve, 5150, n)
+        want = _closed_form(curve, scalars, k0, d)
+    guard = 4096
+    pts = torch.full((nbytes + guard,), 0xA5, dtype=torch.uint8, device="cuda")
+    inf = torch.full((n + guard,), 0x5A, dtype=torch.uint8, device="cuda")
+    d_wire = D.to_device(points)
+    assert lib.kinet_hip_points_prepare(curve, d_wire.data_ptr(), n, pts.data_ptr(), inf.data_ptr(),
+                                        D.stream_handle()) == 0
+    torch.cuda.synchronize()
+    assert bool((pts[nbytes:] == 0xA5).all()), "points_prepare wrote past kinet_hip_points_bytes"
+    assert bool((inf[n:] == 0x5A).all()), "points_prepare wrote past n flag bytes"
+    out = torch.empty(ca.point_bytes(curve) * 2, dtype=torch.uint8, device="cuda")
+    assert lib.kinet_hip_msm_prepared(curve, D.to_device(scalars).data_ptr(), pts.data_ptr(), inf.data_ptr(), n, 0,
+                                      0, out.data_ptr(), D.stream_handle()) == 0
+    torch.cuda.synchronize()
+    pb = ca.point_bytes(curve)
+    assert out.cpu().numpy().tobytes()[:pb] == want
+
+
+@pytest.mark.parametrize("g1", [False, True])
+def test_shared_workspace_ordered_across_streams(ca, g1):
+    """Back-to-back kinet_hip_msm_prepared(_g1) calls on two different streams
+    share the library's MSM workspace: the second stream waits for the first
+    MSM, so both results are right (a large MSM first, then a smaller one)."""
+    import torch
+    from crypto_amd import device as D
+
+    curve = 2
+    n1, n2 = 1 << 17, 5000
+    k0, d = 0x1234, 0x777
+    points = O.gen_points_progression(curve, k0, d, n1)
+    pp = D.PreparedPoints(curve, points)
+    sc1 = O.gen_scalars(curve, 61, n1)
+    sc2 = O.gen_scalars(curve, 62, n2)
+    d1, d2 = D.to_device(sc1), D.to_device(sc2)
+    want1 = _closed_form(curve, sc1, k0, d)
+    want2 = _closed_form(curve, sc2, k0, d)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rep in range(2):
+        print(f"two streams g1={g1} rep={rep}", flush=True)
+        o1 = D.msm_prepared(pp, d1, stream=sa, sync=False, g1=g1)
+        o2 = D.msm_prepared(pp, d2, n=n2, stream=sb, sync=False, g1=g1)
+        torch.cuda.synchronize()
+        assert o1.cpu().numpy().tobytes()[:96] == want1
+        assert o2.cpu().numpy().tobytes()[:96] == want2
