@@ -78,8 +78,10 @@ class KzgServer:
 def poisson_load(server: KzgServer, make_request, rate: float, duration: float, seed: int = 1):
     """Open-loop load: arrivals with exponential gaps (mean 1/rate) for
     `duration` seconds; make_request(rng) -> (curve, evals array).  A waiter
-    thread collects completions.  Returns a list of (curve, n, latency_s) with
-    latency measured by the server from submit to completion."""
+    thread collects completions.  Returns ([(curve, n, latency_s, commitment,
+    evals)], wall seconds), latency measured by the server from submit to
+    completion, in submission order, so every result can be checked against
+    its request."""
     rng = np.random.default_rng(seed)
     tickets = []
     done = []
@@ -94,9 +96,9 @@ def poisson_load(server: KzgServer, make_request, rate: float, duration: float, 
                     cond.wait()
                 if k >= len(tickets):
                     return
-                tk, curve, n = tickets[k]
-            _, t0, t1 = server.wait(tk)
-            done.append((curve, n, (t1 - t0) * 1e-9))
+                tk, curve, ev = tickets[k]
+            commit, t0, t1 = server.wait(tk)
+            done.append((curve, ev.size // 32, (t1 - t0) * 1e-9, commit, ev))
             k += 1
 
     th = threading.Thread(target=waiter, daemon=True)
@@ -113,7 +115,7 @@ def poisson_load(server: KzgServer, make_request, rate: float, duration: float, 
         curve, ev = make_request(rng)
         tk = server.submit(curve, ev)
         with cond:
-            tickets.append((tk, curve, ev.size // 32))
+            tickets.append((tk, curve, ev))
             cond.notify()
     with cond:
         finished[0] = True

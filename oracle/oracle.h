@@ -95,6 +95,15 @@ int  orc_kzg_commit(int curve, const uint8_t* srs_wire, const uint8_t* evals, si
 /* closed form: compress([f(tau)] G) with f = iNTT(evals), Horner at tau. */
 int  orc_kzg_commit_closed_form(int curve, const uint8_t* tau_le, const uint8_t* evals, size_t n, uint8_t* out_compressed);
 
+/* ---- O(n) closed forms over Fr (fr: 0 = BLS12-381, 1 = BN254) ---- */
+/* sum a_i b_i mod r (a_i, b_i any 256-bit LE values). */
+int  orc_fr_dot(int fr, const uint8_t* a, const uint8_t* b, size_t n, uint8_t* out_le);
+/* sum c_i x^i (canonical inputs; -1 otherwise). */
+int  orc_fr_eval(int fr, const uint8_t* coeffs, size_t n, const uint8_t* x_le, uint8_t* out_le);
+/* f(tau) from f(w^j), j < n, by the barycentric formula; -1 if tau is in the
+ * domain, n is not a supported power of two or an input is not canonical. */
+int  orc_fr_lagrange(int fr, const uint8_t* evals, size_t n, const uint8_t* tau_le, uint8_t* out_le);
+
 /* ---- seeded inputs ---- */
 uint64_t orc_splitmix64(uint64_t* state);
 /* n scalars uniform in [0, r) by rejection over masked 4x64-bit draws. */

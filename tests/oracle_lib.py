@@ -54,8 +54,39 @@ def orc():
         lib.orc_gen_points.argtypes = [ctypes.c_int, _u8p, _sz, _u8p]
         lib.orc_gen_points_progression.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
         lib.orc_field_mont_consts.argtypes = [ctypes.c_int, _u8p, _u8p, ctypes.POINTER(ctypes.c_uint64)]
+        lib.orc_fr_dot.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
+        lib.orc_fr_eval.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
+        lib.orc_fr_lagrange.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
         _orc = lib
     return _orc
+
+
+def fr_of_curve(curve: int) -> int:
+    return 0 if curve == CURVE_BLS12_381 else 1
+
+
+def fr_dot(fr: int, a: bytes, b: bytes) -> int:
+    """sum a_i b_i mod r over two arrays of 32-byte LE values (O(n), C)."""
+    assert len(a) == len(b) and len(a) % 32 == 0
+    out = _buf(32)
+    orc().orc_fr_dot(fr, a, b, len(a) // 32, out)
+    return int.from_bytes(out.raw, "little")
+
+
+def fr_eval(fr: int, coeffs: bytes, x: int) -> int | None:
+    """sum c_i x^i (Horner, C)."""
+    out = _buf(32)
+    if orc().orc_fr_eval(fr, coeffs, len(coeffs) // 32, x.to_bytes(32, "little"), out):
+        return None
+    return int.from_bytes(out.raw, "little")
+
+
+def fr_lagrange(fr: int, evals: bytes, tau: int) -> int | None:
+    """f(tau) from the evaluations f(w^j) (barycentric formula, C)."""
+    out = _buf(32)
+    if orc().orc_fr_lagrange(fr, evals, len(evals) // 32, tau.to_bytes(32, "little"), out):
+        return None
+    return int.from_bytes(out.raw, "little")
 
 
 def ref_available() -> bool:

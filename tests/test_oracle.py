@@ -386,3 +386,58 @@ def test_secp256k1_golden_matches_closed_form():
                 want = M.add(0, want, M.mul(0, P, int.from_bytes(sc[32 * i:32 * i + 32], "little")))
         got = bytes.fromhex(case["expected"])
         assert got == (M.to_wire(0, want) if want is not M.INF else bytes(64)), (case["n"], case.get("name"))
+
+
+# ------------------------------------------------------- O(n) closed forms
+@pytest.mark.parametrize("fr", [0, 1])
+def test_fr_dot_against_python(fr):
+    """orc_fr_dot: sum a_i b_i mod r for values over the whole 256-bit range
+    (the MSM closed form takes unreduced scalars)."""
+    r = FRS[fr]
+    rnd = random.Random(90 + fr)
+    a = [rnd.getrandbits(256) for _ in range(300)] + [0, r - 1, r, r + 1, (1 << 256) - 1]
+    b = [rnd.getrandbits(256) for _ in range(300)] + [(1 << 256) - 1, r - 1, 5, r, 1]
+    assert O.fr_dot(fr, _enc(a), _enc(b)) == sum(x * y for x, y in zip(a, b)) % r
+    assert O.fr_dot(fr, b"", b"") == 0
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+def test_fr_eval_against_python_and_ntt(fr):
+    """orc_fr_eval is Horner's sum c_i x^i; at x = w^k it is output k of the
+    forward NTT (pinned against the Python DFT above)."""
+    r = FRS[fr]
+    c = _rand_fr(fr, 64, 91 + fr)
+    x = random.Random(92).randrange(r)
+    assert O.fr_eval(fr, _enc(c), x) == sum(ci * pow(x, i, r) for i, ci in enumerate(c)) % r
+    w = O.root_of_unity(fr, 64)
+    X = _dec(O.ntt(fr, _enc(c)))
+    for k in (0, 1, 17, 63):
+        assert O.fr_eval(fr, _enc(c), pow(w, k, r)) == X[k]
+    assert O.fr_eval(fr, _enc([r]), 1) is None
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 8, 64])
+def test_fr_lagrange_against_interpolation(fr, n):
+    """orc_fr_lagrange (barycentric, no NTT) equals f(tau) for f interpolated
+    through the evaluations by the naive inverse DFT on Python ints; tau in
+    the domain is refused."""
+    r = FRS[fr]
+    e = _rand_fr(fr, n, 93 + n + fr)
+    w = O.root_of_unity(fr, n) if n > 1 else 1
+    winv, ninv = pow(w, r - 2, r), pow(n, r - 2, r)
+    coeffs = [v * ninv % r for v in M.dft(e, winv, r)]
+    tau = random.Random(n).randrange(r)
+    want = sum(ci * pow(tau, i, r) for i, ci in enumerate(coeffs)) % r
+    assert O.fr_lagrange(fr, _enc(e), tau) == want
+    assert O.fr_lagrange(fr, _enc(e), pow(w, n - 1, r)) is None
+
+
+@pytest.mark.parametrize("fr", [0, 1])
+def test_fr_lagrange_matches_ntt_route_at_4096(fr):
+    """At 4096 points: barycentric f(tau) equals Horner on the oracle's iNTT."""
+    r = FRS[fr]
+    e = _rand_fr(fr, 4096, 94 + fr)
+    tau = random.Random(95).randrange(r)
+    coeffs = O.ntt(fr, _enc(e), inverse=True)
+    assert O.fr_lagrange(fr, _enc(e), tau) == O.fr_eval(fr, coeffs, tau)

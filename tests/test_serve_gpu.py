@@ -110,7 +110,9 @@ def test_serve_poisson_short(env):
     done, wall = serve.poisson_load(server, make, rate=400.0, duration=0.5, seed=3)
     server.close()
     assert len(done) > 50
-    assert all(lat > 0 for _, _, lat in done)
+    assert all(lat > 0 for _, _, lat, _, _ in done)
+    for curve, n, _, commit, ev in done[:40]:
+        assert commit == srs[curve].commit(ev.tobytes())
 
 
 @pytest.mark.parametrize("curve", [1, 2])
