@@ -49,17 +49,46 @@ inline void hip_check(hipError_t e, const char* expr, const char* file, int line
 // Records the last error message for kinet_hip_last_error().
 void set_last_error(const std::string& msg);
 
-// Grow-only device allocation.
+// Bump allocator over device memory the library already holds (a static
+// device array): hands out 256-byte aligned pieces and never calls hipMalloc.
+// Running past the end throws std::length_error before anything is launched.
+struct DevArena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || bytes > cap - off)
+            throw std::length_error("MSM scratch exhausted: the call needs more than the static scratch array holds");
+        used = off + bytes;
+        return base + off;
+    }
+};
+
+// Grow-only device allocation; with an arena bound, pieces of the arena
+// instead (no hipMalloc / hipFree).
 struct DevBuf {
     void* ptr = nullptr;
     size_t cap = 0;
+    DevArena* arena = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !arena) (void)hipFree(ptr);
+    }
+    // Points this buffer at an arena, forgetting any earlier piece.
+    void bind(DevArena* a) {
+        if (ptr && !arena) (void)hipFree(ptr);
+        arena = a;
+        ptr = nullptr;
+        cap = 0;
     }
     void* ensure(size_t bytes) {
+        if (bytes > cap && arena) {
+            ptr = arena->take(bytes < 256 ? 256 : bytes);
+            cap = bytes < 256 ? 256 : bytes;
+            return ptr;
+        }
         if (bytes > cap) {
             if (ptr) KZ_HIP(hipFree(ptr));
             ptr = nullptr;

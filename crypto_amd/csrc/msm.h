@@ -28,6 +28,20 @@ struct MsmWorkspace {
     DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 16-byte scalar magnitudes
     DevBuf sort_dig;                  // batch LDS sort: 16-bit digit codes
     DevBuf out;
+    // Every buffer becomes a piece of `a`, carved afresh on its next ensure.
+    void bind(DevArena* a) {
+        for (DevBuf* b : {&counts, &offsets, &cursor, &block_sums, &total, &entries, &bsum, &carry, &run_bucket,
+                          &segA, &segB, &winres, &lvA[0], &lvA[1], &lvB[0], &lvB[1], &sort_cnt, &sort_pre,
+                          &glv_pts, &glv_inf, &glv_sc, &sort_dig, &out})
+            b->bind(a);
+    }
+};
+
+// While an object of this type lives on a thread, MSMs on that thread record
+// no profiler events (so they create no HIP events).
+struct NoStageTimers {
+    NoStageTimers();
+    ~NoStageTimers();
 };
 
 struct MsmParams {

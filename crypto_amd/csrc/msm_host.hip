@@ -108,6 +108,7 @@ void msm_get_tuned(int curve, int mode, int lg, int& c, int& seg) {
 namespace {
 
 std::atomic<int> g_profiling{0};
+thread_local int g_timers_off = 0;  // NoStageTimers nesting on this thread
 // First-level reduction segment (buckets per running-sum chain): the chain is
 // 3 seg point additions long, so short segments cut latency; 0 = built-in rule.
 std::atomic<int> g_seg_override{0};
@@ -123,7 +124,7 @@ struct StageTimer {
     int stage;
     hipEvent_t a = nullptr;
     StageTimer(hipStream_t s_, int stage_) : s(s_), stage(stage_) {
-        if (!g_profiling.load()) return;
+        if (!g_profiling.load() || g_timers_off) return;
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
         KZ_HIP(hipEventCreate(&a));
@@ -142,6 +143,9 @@ struct StageTimer {
 }  // namespace
 
 void msm_set_profiling(int on) { g_profiling.store(on); }
+
+NoStageTimers::NoStageTimers() { ++g_timers_off; }
+NoStageTimers::~NoStageTimers() { --g_timers_off; }
 
 void msm_set_reduce_segment(int seg) {
     if (seg != 0 && (seg < 1 || seg > 16 || (seg & (seg - 1)) != 0))

@@ -69,6 +69,7 @@ def library():
         lib.kinet_hip_points_bytes.restype = sz
         lib.kinet_hip_msm_prepared.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
         lib.kinet_hip_msm_prepared_g1.argtypes = [u32, vp, vp, vp, sz, i32, i32, vp, vp]
+        lib.kinet_hip_msm_prepared_noalloc.argtypes = [u32, vp, vp, vp, sz, vp, vp]
         lib.kinet_hip_points_check_g1.argtypes = [u32, vp, vp, sz, vp, vp]
         lib.kinet_hip_points_decompress.argtypes = [u32, vp, sz, vp, vp, vp]
         lib.kinet_hip_g1_fixed_base_mul.argtypes = [u32, vp, sz, vp, vp]

@@ -47,6 +47,21 @@ size_t kinet_hip_points_bytes(uint32_t curve, size_t n);
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                            size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
+/* kinet_hip_msm_prepared for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and output
+ * record, with the built-in window rule (c = floor(log2 n) - 4 in [5, 16]) and
+ * the default chunking; the bytes equal kinet_hip_msm_prepared's.  Its
+ * scratch is a static 64 MiB device array that each device holds until the
+ * process ends, so n is limited to 2^16 points (GPUKIT_ERR_BAD_SIZE beyond);
+ * the call makes no hipMalloc, hipFree or other allocation (the first call on
+ * a device looks up the array's address and creates one ordering event).
+ * Calls on one device share the array: the host serialises them and each
+ * call's stream waits for the previous call, except under graph capture,
+ * where overlapping replays and calls on one device race. */
+int kinet_hip_msm_prepared_noalloc(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
+                                   size_t n, void* d_out, void* stream);
+
 /* The same MSM for points the caller guarantees to lie in G1 (for example
  * checked once by kinet_hip_points_check_g1, or generated as multiples of the
  * generator; for BN254, cofactor 1, every curve point).  It then uses the GLV
