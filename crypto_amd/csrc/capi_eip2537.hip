@@ -1,4 +1,4 @@
-// EIP-2537 G1 MSM entry (include/kinet/crypto/bls_g1_msm.h).
+// EIP-2537 G1 MSM entry (include/kinet/crypto/bls.h).
 //
 // Same checks and result as the reference (bls/cpp/bls.cpp:194-245): each
 // coordinate a 64-byte big-endian integer below p, each point on the curve and
@@ -11,7 +11,7 @@
 
 #include "engine.h"
 #include "glv.h"
-#include "kinet/crypto/bls_g1_msm.h"
+#include "kinet/crypto/bls.h"
 #include "scan.h"
 
 namespace kz {

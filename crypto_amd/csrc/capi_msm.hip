@@ -1,12 +1,13 @@
 // C ABI of the MSM path.
 //
-//   gpukit_multi_pippenger      drop-in for gpukit/c-abi/c_multi_pippenger.cpp:9-42
 //   gpukit_multi_pippenger_hip  new HIP backend next to the reference's
 //                               _cpu/_cuda/_metal/_wgsl entries
-//                               (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92)
+//                               (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92),
+//                               called by the reference's own dispatcher
+//                               (gpukit/c-abi/c_multi_pippenger.cpp:9-42) once
+//                               it has a GPUKIT_BACKEND_HIP case
 //   kinet_hip_*                 device-pointer variants for callers whose data
 //                               already lives in HBM (bench, batching layers)
-#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -98,21 +99,6 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
         set_last_error(ex.what());
         return GPUKIT_ERR_BACKEND;
     }
-}
-
-// Dispatcher: this library carries only the HIP backend.  GPUKIT_BACKEND unset,
-// empty or "hip" selects it; any other value names a backend this library
-// does not have and is reported as GPUKIT_ERR_BACKEND (no silent CPU route).
-int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
-                           uint8_t* result) {
-    if (!result) return GPUKIT_ERR_NULL_ARG;
-    if (n > 0 && (!scalars || !points)) return GPUKIT_ERR_NULL_ARG;
-    const char* env = std::getenv("GPUKIT_BACKEND");
-    if (env && *env && std::strcmp(env, "hip") != 0) {
-        set_last_error(std::string("GPUKIT_BACKEND=") + env + " is not provided by the HIP library");
-        return GPUKIT_ERR_BACKEND;
-    }
-    return gpukit_multi_pippenger_hip(curve, scalars, points, n, result);
 }
 
 int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,

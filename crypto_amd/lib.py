@@ -59,9 +59,8 @@ def library():
         vp, sz, u32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int
         lib.kinet_hip_last_error.restype = ctypes.c_char_p
         lib.kinet_hip_available.restype = i32
-        for name in ("gpukit_multi_pippenger", "gpukit_multi_pippenger_hip"):
-            getattr(lib, name).argtypes = [u32, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p]
-            getattr(lib, name).restype = i32
+        lib.gpukit_multi_pippenger_hip.argtypes = [u32, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p]
+        lib.gpukit_multi_pippenger_hip.restype = i32
         lib.bls12_381_g1_msm.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
         lib.bls12_381_g1_msm.restype = i32
         lib.kinet_hip_points_prepare.argtypes = [u32, vp, sz, vp, vp, vp]
@@ -265,7 +264,7 @@ def kzg_commit_batch(srs_list, evals_list) -> list[bytes]:
 
 
 def bls12_381_g1_msm(xycs: bytes) -> tuple[bool, bytes, bytes]:
-    """EIP-2537 G1 MSM (include/kinet/crypto/bls_g1_msm.h; the reference's
+    """EIP-2537 G1 MSM (include/kinet/crypto/bls.h; the reference's
     bls12_381_g1_msm, bls/c-abi/c_bls_pairing.h:153-156): xycs is N tuples of
     x[64] || y[64] || c[32] (big-endian); returns (ok, rx[64], ry[64]) with the
     reference's convention of ok = False on any encoding, curve or subgroup
@@ -276,14 +275,15 @@ def bls12_381_g1_msm(xycs: bytes) -> tuple[bool, bytes, bytes]:
     return ok, rx.raw, ry.raw
 
 
-def multi_pippenger(curve: int, scalars: bytes, points: bytes, backend: str = "dispatch") -> bytes:
-    """sum s_i P_i through gpukit_multi_pippenger (or _hip with backend="hip")."""
+def multi_pippenger(curve: int, scalars: bytes, points: bytes) -> bytes:
+    """sum s_i P_i through gpukit_multi_pippenger_hip (the reference's wire
+    format: 32-byte LE scalars, BE affine points, all-zero = identity)."""
     lib = require_device()
     n = len(scalars) // 32
     if len(scalars) != 32 * n or len(points) != point_bytes(curve) * n:
         raise ValueError("scalars must be n*32 bytes and points n*point_bytes(curve) bytes")
     out = ctypes.create_string_buffer(point_bytes(curve))
-    fn = lib.gpukit_multi_pippenger_hip if backend == "hip" else lib.gpukit_multi_pippenger
+    fn = lib.gpukit_multi_pippenger_hip
     rc = fn(curve, scalars, points, n, out)
     if rc != GPUKIT_OK:
         raise KinetError(fn.__name__, rc, GPUKIT_STATUS)

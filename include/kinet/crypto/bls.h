@@ -5,8 +5,8 @@
  * (bls/c-abi/c_bls_pairing.h:153-156, body bls/cpp/bls.cpp:194-245) with the
  * same name, arguments and return convention.  Host buffers, synchronous.
  */
-#ifndef KINET_CRYPTO_BLS_G1_MSM_H
-#define KINET_CRYPTO_BLS_G1_MSM_H
+#ifndef KINET_CRYPTO_BLS_H
+#define KINET_CRYPTO_BLS_H
 
 #include <stddef.h>
 #include <stdint.h>

@@ -36,17 +36,20 @@ typedef enum gpukit_curve {
     GPUKIT_CURVE_BANDERWAGON = 3, /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
 } gpukit_curve;
 
-/* Replaces the dispatcher of gpukit/c-abi/c_multi_pippenger.cpp:9-42.
- * GPUKIT_BACKEND unset, empty or "hip" runs the HIP backend; any other value
- * returns GPUKIT_ERR_BACKEND, since this library holds no other backend. */
-int gpukit_multi_pippenger(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
-                           uint8_t* result);
-
-/* The HIP backend itself, a sibling of gpukit_multi_pippenger_{cpu,cuda,metal,wgsl}
- * (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92).  n == 0 gives the
- * identity; n >= 2^26 gives GPUKIT_ERR_BAD_SIZE; Banderwagon gives
- * GPUKIT_ERR_NOTIMPL; device failures give GPUKIT_ERR_BACKEND (message via
- * kinet_hip_last_error()). */
+/* The HIP backend, a sibling of gpukit_multi_pippenger_{cpu,cuda,metal,wgsl}
+ * (gpukit/include/kinet/gpukit/multi_pippenger.h:69-92).  The multi-backend
+ * entry gpukit_multi_pippenger stays the reference's dispatcher
+ * (gpukit/c-abi/c_multi_pippenger.cpp:9-42): it calls this function for
+ * GPUKIT_BACKEND=hip and keeps its CPU fallback on GPUKIT_ERR_NOTIMPL and
+ * GPUKIT_ERR_BACKEND.  This library defines no gpukit_multi_pippenger, so it
+ * links beside the reference's gpukit without a duplicate symbol.
+ *
+ * n == 0 gives the identity.  n >= 2^26 gives GPUKIT_ERR_BAD_SIZE: sorted
+ * bucket entries carry 31-bit indices and an MSM has up to 22 windows per
+ * point (2^26 BLS12-381 points are 6.4 GB of wire input; the reference's CPU
+ * path has no cap).  Banderwagon gives GPUKIT_ERR_NOTIMPL (the dispatcher's
+ * CPU fallback then serves it); device failures give GPUKIT_ERR_BACKEND
+ * (message via kinet_hip_last_error()). */
 int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
                                uint8_t* result);
 

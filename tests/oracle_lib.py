@@ -14,6 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIBORC = os.path.join(ORACLE_DIR, "liborc.so")
 LIBREF = os.path.join(ORACLE_DIR, "_ref", "libref.so")
+LIBBNREF = os.path.join(ORACLE_DIR, "_ref", "libbnref.so")
 
 CURVE_BN254 = 1
 CURVE_BLS12_381 = 2
@@ -59,6 +60,54 @@ def orc():
         lib.orc_fr_lagrange.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
         _orc = lib
     return _orc
+
+
+_bnref = None
+
+
+def bnref_available() -> bool:
+    return os.path.exists(LIBBNREF)
+
+
+def bnref():
+    """The reference's BN254 CUDA driver built without CUDA: its CPU path
+    behind kinet_bn254_cuda_{g1_add, g1_mul, fp_mul} (same layout as the HIP
+    kinet_bn254_hip_* entries)."""
+    global _bnref
+    if _bnref is None:
+        lib = ctypes.CDLL(LIBBNREF)
+        for name in ("kinet_bn254_cuda_g1_add", "kinet_bn254_cuda_g1_mul", "kinet_bn254_cuda_fp_mul"):
+            getattr(lib, name).argtypes = [_u8p, _u8p, _u8p, ctypes.c_uint]
+            getattr(lib, name).restype = ctypes.c_int
+        _bnref = lib
+    return _bnref
+
+
+# BN254 batch layout of the reference driver (bn254/gpu/cuda/bn254_driver_cuda.h):
+# Fp elements as 4 x u64 LE in Montgomery form x 2^256 mod p; affine points as
+# x[4] || y[4] || inf (9 x u64).
+BN_P = 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47
+
+
+def bn_fp_ref(x: int) -> bytes:
+    return (x * (1 << 256) % BN_P).to_bytes(32, "little")
+
+
+def bn_fp_from_ref(b: bytes) -> int:
+    return int.from_bytes(b, "little") * pow(1 << 256, -1, BN_P) % BN_P
+
+
+def bn_point_ref(P) -> bytes:
+    """(x, y) plain ints or None (identity) -> 72-byte record."""
+    if P is None:
+        return bytes(64) + (1).to_bytes(8, "little")
+    return bn_fp_ref(P[0]) + bn_fp_ref(P[1]) + bytes(8)
+
+
+def bn_point_from_ref(b: bytes):
+    if int.from_bytes(b[64:72], "little"):
+        return None
+    return bn_fp_from_ref(b[:32]), bn_fp_from_ref(b[32:64])
 
 
 def fr_of_curve(curve: int) -> int:

@@ -24,8 +24,12 @@ def declared_functions():
 
 def test_headers_declare_the_boundary():
     names = declared_functions()
-    for required in ("gpukit_multi_pippenger", "gpukit_multi_pippenger_hip", "kinet_hip_msm_prepared"):
+    for required in ("gpukit_multi_pippenger_hip", "kinet_hip_msm_prepared", "bls12_381_g1_msm", "ntt_fr_forward",
+                     "kzg_commit", "kzg_commit_batch", "kinet_bn254_hip_g1_add", "kinet_hip_msm_prepared_noalloc"):
         assert required in names
+    # the multi-backend dispatcher stays the reference's: this library must not
+    # define a second gpukit_multi_pippenger
+    assert "gpukit_multi_pippenger" not in names
 
 
 def test_library_exports_every_declared_symbol():
@@ -37,6 +41,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(path)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
+    assert not hasattr(lib, "gpukit_multi_pippenger")
 
 
 def test_no_silent_cpu_route_without_device():
@@ -49,7 +54,7 @@ def test_no_silent_cpu_route_without_device():
     if lib.kinet_hip_available():
         pytest.skip("a HIP device is present; covered by the gpu tests")
     out = ctypes.create_string_buffer(64)
-    rc = lib.gpukit_multi_pippenger(1, bytes(32), bytes(64), 1, out)
+    rc = lib.gpukit_multi_pippenger_hip(1, bytes(32), bytes(64), 1, out)
     assert rc == -3  # GPUKIT_ERR_BACKEND
     with pytest.raises(crypto_amd.HipUnavailable):
         crypto_amd.multi_pippenger(1, bytes(32), bytes(64))
@@ -72,3 +77,50 @@ def test_eip2537_msm_rejects_without_device_work():
     assert lib.bls12_381_g1_msm(rx, ry, b"", 0) == 1 and rx.raw == bytes(64) and ry.raw == bytes(64)
     if not lib.kinet_hip_available():
         assert lib.bls12_381_g1_msm(rx, ry, bytes(160), 160) == 0
+
+
+def test_hip_backend_argument_checks_without_device():
+    """gpukit_multi_pippenger_hip refuses bad arguments before touching the
+    device, with the reference's status codes: NULL result or inputs, an
+    unknown curve, Banderwagon (NOTIMPL, so the reference dispatcher falls back
+    to its CPU path) and n >= 2^26 (31-bit bucket entries); n = 0 is the
+    identity."""
+    import crypto_amd
+
+    if not os.path.exists(crypto_amd.library_path()):
+        pytest.skip("HIP library not built")
+    lib = crypto_amd.library()
+    out = ctypes.create_string_buffer(96)
+    one = bytes(96)
+    assert lib.gpukit_multi_pippenger_hip(2, one, one, 1, None) == -1
+    assert lib.gpukit_multi_pippenger_hip(2, None, one, 1, out) == -1
+    assert lib.gpukit_multi_pippenger_hip(3, one, one, 1, out) == -5
+    assert lib.gpukit_multi_pippenger_hip(7, one, one, 1, out) == -3
+    assert lib.gpukit_multi_pippenger_hip(2, one, one, 1 << 26, out) == -2
+    out = ctypes.create_string_buffer(b"\xee" * 96, 96)
+    assert lib.gpukit_multi_pippenger_hip(2, None, None, 0, out) == 0 and out.raw == bytes(96)
+
+
+def test_c_caller_compiles_links_and_runs(tmp_path):
+    """A C99 program including every public header compiles with -Wall
+    -Werror, links against the library and gets the documented status codes
+    for the calls that need no device."""
+    import shutil
+    import subprocess
+
+    import crypto_amd
+
+    path = crypto_amd.library_path()
+    if not os.path.exists(path):
+        pytest.skip("HIP library not built")
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    exe = str(tmp_path / "caller")
+    libdir = os.path.dirname(path)
+    subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c_caller", "caller.c"), "-L", libdir, "-lkinet_crypto_hip",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "C caller: ok" in res.stdout

@@ -1,4 +1,4 @@
-"""EIP-2537 G1 MSM (bls12_381_g1_msm, include/kinet/crypto/bls_g1_msm.h) against
+"""EIP-2537 G1 MSM (bls12_381_g1_msm, include/kinet/crypto/bls.h) against
 the reference's semantics (bls/cpp/bls.cpp:194-245): results against the
 closed form sum c_k k_k G for points P_k = k_k G, full 256-bit big-endian
 scalars, infinity entries skipped, and every rejection the reference makes

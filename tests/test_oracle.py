@@ -441,3 +441,30 @@ def test_fr_lagrange_matches_ntt_route_at_4096(fr):
     tau = random.Random(95).randrange(r)
     coeffs = O.ntt(fr, _enc(e), inverse=True)
     assert O.fr_lagrange(fr, _enc(e), tau) == O.fr_eval(fr, coeffs, tau)
+
+
+# ------------------------------------------ BN254 batch layout of the reference
+@pytest.mark.skipif(not O.bnref_available(), reason="oracle/_ref not built")
+def test_bn254_ref_layout_helpers_against_reference_driver():
+    """The Python encoders of the reference driver's layout (Montgomery
+    x 2^256, 9-word points) agree with the reference's own CPU path of
+    kinet_bn254_cuda_{fp_mul, g1_add, g1_mul}: products, sums and multiples
+    decoded from its outputs equal the textbook values."""
+    import ctypes
+
+    lib = O.bnref()
+    p, r, G = M.BN_P, M.BN_R, M.CURVES[1]["g"]
+    rnd = random.Random(254)
+    a, b = [rnd.randrange(p) for _ in range(20)], [rnd.randrange(p) for _ in range(20)]
+    out = ctypes.create_string_buffer(32 * 20)
+    assert lib.kinet_bn254_cuda_fp_mul(b"".join(map(O.bn_fp_ref, a)), b"".join(map(O.bn_fp_ref, b)), out, 20) == 0
+    assert [O.bn_fp_from_ref(out.raw[32 * i:32 * i + 32]) for i in range(20)] == [x * y % p for x, y in zip(a, b)]
+    ks = [rnd.randrange(1, r) for _ in range(6)]
+    P = [M.mul(1, G, k) for k in ks]
+    Q = [M.mul(1, G, k + 1) for k in ks]
+    out = ctypes.create_string_buffer(72 * 6)
+    assert lib.kinet_bn254_cuda_g1_add(b"".join(map(O.bn_point_ref, P)), b"".join(map(O.bn_point_ref, Q)), out, 6) == 0
+    assert [O.bn_point_from_ref(out.raw[72 * i:72 * i + 72]) for i in range(6)] == [M.add(1, x, y) for x, y in zip(P, Q)]
+    sc = b"".join(k.to_bytes(32, "little") for k in ks)
+    assert lib.kinet_bn254_cuda_g1_mul(b"".join(map(O.bn_point_ref, [G] * 6)), sc, out, 6) == 0
+    assert [O.bn_point_from_ref(out.raw[72 * i:72 * i + 72]) for i in range(6)] == P

@@ -1,4 +1,4 @@
-"""secp256k1 (curve 0) MSMs through gpukit_multi_pippenger and the prepared
+"""secp256k1 (curve 0) MSMs through gpukit_multi_pippenger_hip and the prepared
 plain path: the reference's own outputs (golden fixtures, and the compiled
 reference CPU Pippenger live where oracle/_ref is present), closed forms
 sum s_i (k0 + i d) G for full-range scalars, every window size, edge cases
