@@ -87,6 +87,36 @@ int kzg_commit_batch(const kzg_srs* const* srs, const uint8_t* const* evals, con
  * result in wire format (64 or 96 bytes). */
 int kzg_msm(const kzg_srs* srs, const uint8_t* scalars, size_t n, uint8_t* result);
 
+/* ---------------------------------------------------------- EIP-4844 blobs
+ * The reference's blob entries (c-abi/crypto.h:238-239, c-kzg-4844 v2.1.7
+ * behind them, kzg/cpp/kzg_blob.cpp:69-125), same names, arguments and
+ * return codes.  A blob is 4096 BLS12-381 Fr elements, 32-byte big-endian,
+ * each < r (else CRYPTO_ERR_INPUT); they are the evaluations of f on the
+ * domain w^brp(i), w = 7^((r - 1) / 4096), brp the 12-bit bit reversal.
+ *   kzg_blob_to_commit:  commit = [f(tau)] G1, 48-byte compressed
+ *   kzg_commit_to_proof: y = f(z) (32-byte big-endian) and the proof
+ *                        [(f(tau) - y) / (tau - z)] G1, for any canonical z
+ *                        (z >= r: CRYPTO_ERR_INPUT), z in the domain included
+ * Both need a trusted setup: the reference embeds the mainnet file at
+ * configure time; here it is loaded by kzg_load_trusted_setup_file, or on
+ * first use from the file named by the environment variable
+ * KINET_KZG_TRUSTED_SETUP.  Without one they return CRYPTO_ERR_BACKEND. */
+int kzg_blob_to_commit(const uint8_t blob[131072], uint8_t commit[48]);
+int kzg_commit_to_proof(const uint8_t blob[131072], const uint8_t z[32], uint8_t proof[48], uint8_t y[32]);
+
+/* Loads a trusted setup in c-kzg-4844's text format (the file the reference
+ * embeds, trusted_setup.txt): "4096", the G2 point count, 4096 compressed G1
+ * points in Lagrange form (natural order, hex), the compressed G2 points,
+ * then -- in files of c-kzg 2.x -- 4096 compressed G1 points in monomial
+ * form.  G1 points are decompressed and checked to lie in G1 on the device;
+ * G2 points are checked for count and encoding only (verification is not
+ * provided).  Replaces a loaded setup.  CRYPTO_ERR_INPUT for a malformed or
+ * invalid file (the old setup is kept), CRYPTO_ERR_BACKEND for device
+ * failures.  Names after c-kzg's load_trusted_setup_file / free_trusted_setup
+ * (called at kzg/cpp/kzg_blob.cpp:36-47). */
+int kzg_load_trusted_setup_file(const char* path);
+void kzg_free_trusted_setup(void);
+
 /* Contexts for device-resident, asynchronous use (batching, serving): each
  * owns scratch and runs on `stream` (hipStream_t; NULL = null stream).  Calls
  * on one context are ordered; different contexts may run concurrently. */

@@ -36,6 +36,8 @@ int main(void) {
     CHECK(kinet_hip_points_bytes(2, 10) == 1280 && kinet_hip_points_bytes(1, 10) == 800);
     CHECK(kinet_hip_msm_prepared_noalloc(2, buf, buf, buf, ((size_t)1 << 16) + 1, out, NULL) == GPUKIT_ERR_BAD_SIZE);
     CHECK(kinet_bn254_hip_fp_mul(buf, buf, out, 0) == 0);
+    CHECK(kzg_blob_to_commit(NULL, out) == -1);
+    CHECK(kzg_load_trusted_setup_file(NULL) == -1);
     printf("device available: %d\n", kinet_hip_available());
     printf("%s\n", fails ? "C caller: FAIL" : "C caller: ok");
     return fails != 0;

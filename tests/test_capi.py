@@ -124,3 +124,41 @@ def test_c_caller_compiles_links_and_runs(tmp_path):
     res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "C caller: ok" in res.stdout
+
+
+def test_blob_entries_and_setup_parser_without_device(tmp_path):
+    """The EIP-4844 entries refuse malformed input before any device work:
+    non-canonical blob elements and z (CRYPTO_ERR_INPUT), a missing setup
+    (CRYPTO_ERR_BACKEND), and every malformed setup file (CRYPTO_ERR_INPUT)."""
+    import crypto_amd
+
+    if not os.path.exists(crypto_amd.library_path()):
+        pytest.skip("HIP library not built")
+    lib = crypto_amd.library()
+    lib.kzg_load_trusted_setup_file.argtypes = [ctypes.c_char_p]
+    r = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+    out, y = ctypes.create_string_buffer(48), ctypes.create_string_buffer(32)
+    bad = bytes(32 * 5) + r.to_bytes(32, "big") + bytes(131072 - 32 * 6)
+    assert lib.kzg_blob_to_commit(bad, out) == -1
+    assert lib.kzg_commit_to_proof(bytes(131072), r.to_bytes(32, "big"), out, y) == -1
+    assert lib.kzg_blob_to_commit(None, out) == -1
+    if not lib.kinet_hip_available():
+        os.environ.pop("KINET_KZG_TRUSTED_SETUP", None)
+        assert lib.kzg_blob_to_commit(bytes(131072), out) == -4
+    g1 = "a" * 96
+    g2 = "b" * 192
+    cases = {
+        "empty": "",
+        "count": "4095\n65\n" + (g1 + "\n") * 4095 + (g2 + "\n") * 65,
+        "g2count": "4096\n1\n" + (g1 + "\n") * 4096 + g2 + "\n",
+        "short": "4096\n65\n" + (g1 + "\n") * 4000,
+        "hex": "4096\n65\n" + (g1 + "\n") * 4095 + "zz" * 48 + "\n" + (g2 + "\n") * 65,
+        "g2flag": "4096\n2\n" + (g1 + "\n") * 4096 + ("3" + "b" * 191 + "\n") * 2,
+        "trailing": "4096\n2\n" + (g1 + "\n") * 4096 + (g2 + "\n") * 2 + (g1 + "\n") * 4096 + "00\n",
+        "monomial": "4096\n2\n" + (g1 + "\n") * 4096 + (g2 + "\n") * 2 + (g1 + "\n") * 10,
+    }
+    for name, text in cases.items():
+        p = tmp_path / (name + ".txt")
+        p.write_text(text)
+        assert lib.kzg_load_trusted_setup_file(str(p).encode()) == -1, name
+    assert lib.kzg_load_trusted_setup_file(str(tmp_path / "missing.txt").encode()) == -1
