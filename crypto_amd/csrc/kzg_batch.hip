@@ -36,23 +36,17 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
     size_t total = 0;
     for (uint32_t v : n) total += v;
     hipStream_t s = lane.stream;
+    // everything that allocates happens before any launch (a capture may follow)
     uint32_t* ev = static_cast<uint32_t*>(lane.evals.ensure(32 * total));
     uint32_t* co = static_cast<uint32_t*>(lane.coeffs.ensure(32 * total));
     uint32_t* tmp = static_cast<uint32_t*>(lane.tmp.ensure(32 * total));
     const size_t rec = msm_out_bytes(curve);
     uint8_t* out = static_cast<uint8_t*>(lane.out.ensure(commit_batch_out_bytes(curve, count)));
+    void* pin_out = lane.pin_out.ensure(commit_batch_out_bytes(curve, count));
     int* err = reinterpret_cast<int*>(out + count * rec);
-    device_fill_u32(reinterpret_cast<uint32_t*>(err), 0, 1, s);
-    KZ_HIP(hipMemcpyAsync(ev, lane.pin_in.ptr, 32 * total, hipMemcpyHostToDevice, s));
     const int field = curve == 2 ? 0 : 1;
-    size_t off = 0;
-    for (size_t i = 0; i < count;) {
-        size_t j = i;
-        while (j < count && n[j] == n[i]) ++j;
-        ntt_device_batch(field, log2_exact(n[i]), true, ev + off * 8, co + off * 8, tmp + off * 8, j - i, err, s);
-        off += (size_t)n[i] * (j - i);
-        i = j;
-    }
+    for (size_t i = 0; i < count; ++i)
+        if (i == 0 || n[i] != n[i - 1]) ntt_prepare(field, log2_exact(n[i]), true, s);
     // Plans are cached per (curve, SRS size and window tables, job sizes).
     std::vector<uint32_t> kv;
     kv.push_back((uint32_t)srs->n);
@@ -86,14 +80,59 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
                 }
             }
         }
-        auto plan = std::make_unique<MsmBatchPlan>();
-        msm_batch_plan(curve, specs.data(), count, 0, *plan);
-        it = lane.plans.emplace(key, std::move(plan)).first;
+        auto lp = std::make_unique<LanePlan>();
+        msm_batch_plan(curve, specs.data(), count, 0, lp->plan);
+        it = lane.plans.emplace(key, std::move(lp)).first;
     }
+    LanePlan& lp = *it->second;
+    msm_batch_reserve(lp.plan, lane.ws);
     const uint32_t* pts = srs->table_c.empty() ? srs->pts.as<uint32_t>() : srs->tables.as<uint32_t>();
-    msm_batch_run(*it->second, pts, srs->inf.as<uint8_t>(), co, lane.ws, s, out);
-    KZ_HIP(hipMemcpyAsync(lane.pin_out.ensure(commit_batch_out_bytes(curve, count)), out, count * rec + sizeof(int),
-                          hipMemcpyDeviceToHost, s));
+    const uint8_t* inf = srs->inf.as<uint8_t>();
+    auto launch = [&] {
+        device_fill_u32(reinterpret_cast<uint32_t*>(err), 0, 1, s);
+        KZ_HIP(hipMemcpyAsync(ev, lane.pin_in.ptr, 32 * total, hipMemcpyHostToDevice, s));
+        size_t off = 0;
+        for (size_t i = 0; i < count;) {
+            size_t j = i;
+            while (j < count && n[j] == n[i]) ++j;
+            ntt_device_batch(field, log2_exact(n[i]), true, ev + off * 8, co + off * 8, tmp + off * 8, j - i, err, s);
+            off += (size_t)n[i] * (j - i);
+            i = j;
+        }
+        msm_batch_run(lp.plan, pts, inf, co, lane.ws, s, out);
+        KZ_HIP(hipMemcpyAsync(pin_out, out, count * rec + sizeof(int), hipMemcpyDeviceToHost, s));
+    };
+    ++lp.uses;
+    const void* bufs[LanePlan::kAddrs] = {ev, co, tmp, out, lane.pin_in.ptr, pin_out, pts, inf};
+    lane.ws.addresses(bufs + 8);
+    bool use_graph = lp.uses >= 2 && !lp.no_graph;
+    if (use_graph && !(lp.exec && std::equal(bufs, bufs + LanePlan::kAddrs, lp.bufs))) {
+        lp.drop_graph();
+        KZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            launch();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        KZ_HIP(hipStreamEndCapture(s, &lp.graph));
+        if (hipGraphInstantiate(&lp.exec, lp.graph, nullptr, nullptr, 0) != hipSuccess) {
+            lp.drop_graph();
+            lp.no_graph = true;
+            use_graph = false;
+        } else {
+            std::copy(bufs, bufs + LanePlan::kAddrs, lp.bufs);
+        }
+    }
+    if (use_graph) {
+        KZ_HIP(hipGraphLaunch(lp.exec, s));
+        ++lane.graph_launches;
+    } else {
+        launch();
+        ++lane.direct_launches;
+    }
     KZ_HIP(hipEventRecord(lane.done, s));
 }
 

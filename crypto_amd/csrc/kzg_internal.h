@@ -76,9 +76,36 @@ struct PinnedBuf {
     }
 };
 
+// A lane's cached batch plan for one set of job sizes and, once those sizes
+// come back, the captured launch sequence of the whole commit pipeline (H2D,
+// inverse NTTs, batched MSM, D2H) over the lane's buffers, replayed as one
+// graph launch while the buffers stay where they are.
+struct LanePlan {
+    MsmBatchPlan plan;
+    uint32_t uses = 0;
+    bool no_graph = false;  // capture failed once: launch directly
+    // every device and host address the captured graph uses: the lane's
+    // buffers, the SRS arrays and the lane's MSM workspace (which a larger
+    // plan of the same lane may reallocate)
+    static constexpr int kAddrs = 8 + MsmWorkspace::kBuffers;
+    const void* bufs[kAddrs] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    LanePlan() = default;
+    LanePlan(const LanePlan&) = delete;
+    LanePlan& operator=(const LanePlan&) = delete;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~LanePlan() { drop_graph(); }
+};
+
 // One stream with everything a batch of commitments needs: device buffers
 // for evaluations / coefficients / scratch, the MSM workspace, page-locked
-// staging both ways, cached MSM batch plans (keyed by the job sizes) and a
+// staging both ways, cached plans and graphs (keyed by the job sizes) and a
 // completion event.  A lane runs one batch at a time.
 struct CommitLane {
     hipStream_t stream = nullptr;
@@ -86,7 +113,8 @@ struct CommitLane {
     MsmWorkspace ws;
     DevBuf evals, coeffs, tmp, out, err;
     PinnedBuf pin_in, pin_out;
-    std::map<std::pair<int, std::vector<uint32_t>>, std::unique_ptr<MsmBatchPlan>> plans;
+    std::map<std::pair<int, std::vector<uint32_t>>, std::unique_ptr<LanePlan>> plans;
+    uint64_t graph_launches = 0, direct_launches = 0;
 
     explicit CommitLane(bool own_stream);
     ~CommitLane();
@@ -101,7 +129,10 @@ struct CommitLane {
 // one inverse-NTT launch sequence per distinct size, one batched MSM over the
 // SRS prefix, D2H of the records (wire + compressed) and the error flag into
 // lane.pin_out: record i at pin_out + i * msm_out_bytes, the flag (int) after
-// the last record.  Records the lane's done event.  Nothing synchronises.
+// the last record.  The first batch of a set of job sizes launches directly;
+// from the second on, the sequence is captured into a HIP graph (once per
+// buffer layout) and replayed.  Records the lane's done event.  Nothing
+// synchronises.
 void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector<uint32_t>& n);
 // Bytes of lane.pin_in / lane.pin_out a batch of these sizes needs.
 size_t commit_batch_in_bytes(const std::vector<uint32_t>& n);

@@ -1,6 +1,7 @@
 // Host interface of the HIP Pippenger MSM pipeline.
 #pragma once
 
+#include <array>
 #include <vector>
 
 #include "common.h"
@@ -28,12 +29,24 @@ struct MsmWorkspace {
     DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 16-byte scalar magnitudes
     DevBuf sort_dig;                  // batch LDS sort: 16-bit digit codes
     DevBuf out;
+    static constexpr int kBuffers = 23;
     // Every buffer becomes a piece of `a`, carved afresh on its next ensure.
     void bind(DevArena* a) {
-        for (DevBuf* b : {&counts, &offsets, &cursor, &block_sums, &total, &entries, &bsum, &carry, &run_bucket,
-                          &segA, &segB, &winres, &lvA[0], &lvA[1], &lvB[0], &lvB[1], &sort_cnt, &sort_pre,
-                          &glv_pts, &glv_inf, &glv_sc, &sort_dig, &out})
-            b->bind(a);
+        for (DevBuf* b : buffers()) b->bind(a);
+    }
+    // Current device addresses of every buffer: a captured graph that used
+    // this workspace is valid only while they are unchanged (a growing
+    // buffer is freed and allocated anew).
+    void addresses(const void** out) const {
+        int k = 0;
+        for (const DevBuf* b : const_cast<MsmWorkspace*>(this)->buffers()) out[k++] = b->ptr;
+    }
+
+private:
+    std::array<DevBuf*, kBuffers> buffers() {
+        return {&counts, &offsets, &cursor, &block_sums, &total, &entries, &bsum, &carry, &run_bucket,
+                &segA,   &segB,    &winres, &lvA[0],     &lvA[1], &lvB[0],  &lvB[1], &sort_cnt, &sort_pre,
+                &glv_pts, &glv_inf, &glv_sc, &sort_dig, &out};
     }
 };
 

@@ -177,6 +177,58 @@ class MsmBatch:
             pass
 
 
+class JobSet:
+    """A fixed set of MSMs and Fr NTT groups run as one HIP graph launch
+    (kinet_hip_job_set_*).  groups: [(log2 n, count, first element offset)]."""
+
+    def __init__(self, curve: int, msm_sizes, field: int, groups, inverse: bool = False, sc_off=None, pt_off=None):
+        import ctypes
+
+        from .lib import KinetNttGroup
+
+        lib = require_device()
+        self.curve = curve
+        self.count = len(msm_sizes)
+        n_arr = (ctypes.c_size_t * max(1, self.count))(*msm_sizes)
+        sc_arr = (ctypes.c_uint64 * self.count)(*sc_off) if sc_off is not None else None
+        pt_arr = (ctypes.c_uint32 * self.count)(*pt_off) if pt_off is not None else None
+        g_arr = (KinetNttGroup * max(1, len(groups)))(*[KinetNttGroup(lg, c, o) for lg, c, o in groups])
+        h = ctypes.c_void_p()
+        rc = lib.kinet_hip_job_set_create(curve, n_arr, sc_arr, pt_arr, self.count, field, 1 if inverse else 0,
+                                          g_arr, len(groups), ctypes.byref(h))
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_job_set_create", rc, GPUKIT_STATUS)
+        self.handle = h.value
+        self.record = point_bytes(curve) + point_bytes(curve) // 2
+
+    def run(self, pp: PreparedPoints | None, d_scalars, out, d_ntt_in, d_ntt_out, d_ntt_tmp, d_err, stream=None,
+            sync: bool = True):
+        lib = require_device()
+
+        def ptr(t):
+            return t.data_ptr() if t is not None else None
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = lib.kinet_hip_job_set_run(self.handle, ptr(d_scalars), pp.pts.data_ptr() if pp else None,
+                                       pp.inf.data_ptr() if pp else None, ptr(out), ptr(d_ntt_in), ptr(d_ntt_out),
+                                       ptr(d_ntt_tmp), ptr(d_err), s.cuda_stream)
+        if rc != GPUKIT_OK:
+            raise KinetError("kinet_hip_job_set_run", rc, GPUKIT_STATUS)
+        if sync:
+            s.synchronize()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            require_device().kinet_hip_job_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def ntt_batch(field: int, d_in: torch.Tensor, n: int, count: int, inverse: bool = False,
               d_out: torch.Tensor | None = None, d_tmp: torch.Tensor | None = None, d_err: torch.Tensor | None = None,
               stream=None, sync: bool = True) -> torch.Tensor:

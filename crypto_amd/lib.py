@@ -32,6 +32,11 @@ CRYPTO_STATUS = {0: "CRYPTO_OK", -1: "CRYPTO_ERR_INPUT", -2: "CRYPTO_ERR_LENGTH"
                  -4: "CRYPTO_ERR_BACKEND", -5: "CRYPTO_ERR_NOTIMPL", -6: "CRYPTO_ERR_INTERNAL"}
 
 
+class KinetNttGroup(ctypes.Structure):
+    """kinet_ntt_group of include/kinet/crypto/hip_runtime.h."""
+    _fields_ = [("log_n", ctypes.c_uint32), ("count", ctypes.c_uint32), ("offset", ctypes.c_uint64)]
+
+
 class HipUnavailable(RuntimeError):
     """The HIP library is missing or no HIP device is usable."""
 
@@ -112,6 +117,13 @@ def library():
         lib.kinet_hip_msm_batch_destroy.restype = None
         lib.kinet_hip_msm_batch_window_bits.argtypes = [vp, sz]
         lib.kinet_hip_msm_batch_run.argtypes = [vp, vp, vp, vp, vp, i32, vp]
+        lib.kinet_hip_job_set_create.argtypes = [u32, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_uint64),
+                                                 ctypes.POINTER(u32), sz, u32, i32, ctypes.POINTER(KinetNttGroup), sz,
+                                                 pp]
+        lib.kinet_hip_job_set_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.kinet_hip_job_set_destroy.argtypes = [vp]
+        lib.kinet_hip_job_set_destroy.restype = None
+        lib.kinet_hip_job_set_window_bits.argtypes = [vp, sz]
         lib.kinet_hip_msm_set_window_table.argtypes = [u32, i32, ctypes.POINTER(i32), i32]
         lib.kinet_hip_msm_window_bits.argtypes = [u32, sz, i32]
         lib.kinet_hip_table_window_bits.argtypes = [sz]

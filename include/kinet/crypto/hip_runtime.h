@@ -144,6 +144,41 @@ int kinet_hip_msm_batch_window_bits(const kinet_msm_batch* batch, size_t job);
 int kinet_hip_msm_batch_run(kinet_msm_batch* batch, const void* d_scalars, const void* d_pts, const void* d_inf,
                             void* d_out, int use_graph, void* stream);
 
+/* ------------------------------------------------------------- job sets
+ * Many MSMs and Fr NTTs of uneven sizes as ONE HIP graph launch.  The MSMs
+ * form one batch plan (as kinet_hip_msm_batch_*); the NTTs come in groups of
+ * same-size transforms laid back to back.  The first run with a set of
+ * pointers and a stream captures the whole launch sequence -- the NTT groups
+ * on a second stream forked inside the capture, so they overlap the MSM
+ * pipeline -- and later runs replay it with one hipGraphLaunch.  Outputs are
+ * byte-identical to running the same MSMs (kinet_hip_msm_prepared) and NTTs
+ * (kinet_hip_ntt_fr) one by one.  No counterpart in the reference. */
+typedef struct kinet_job_set kinet_job_set;
+typedef struct kinet_ntt_group {
+    uint32_t log_n;   /* transform size 2^log_n */
+    uint32_t count;   /* transforms in the group, 1..65535 */
+    uint64_t offset;  /* first element of the group in the NTT in/out/tmp arrays */
+} kinet_ntt_group;
+/* MSM jobs as for kinet_hip_msm_batch_create (msm_count may be 0; curve 1
+ * or 2; window bits from the tuning table); NTT groups over `field` (0
+ * BLS12-381 Fr, 1 BN254 Fr; ngroups may be 0), all forward (ntt_inverse 0)
+ * or all inverse.  Builds the MSM plan, its scratch and the twiddle tables
+ * now.  gpukit codes. */
+int kinet_hip_job_set_create(uint32_t curve, const size_t* msm_n, const uint64_t* sc_off, const uint32_t* pt_off,
+                             size_t msm_count, uint32_t field, int ntt_inverse, const kinet_ntt_group* groups,
+                             size_t ngroups, kinet_job_set** out);
+void kinet_hip_job_set_destroy(kinet_job_set* set);
+/* Window bits of MSM job `job` (-1 if out of range). */
+int kinet_hip_job_set_window_bits(const kinet_job_set* set, size_t job);
+/* Runs the set on `stream` (not NULL): MSM job j's record (wire point, then
+ * compressed) at d_msm_out + j * record; NTT group g from d_ntt_in + 32 *
+ * offset into d_ntt_out (d_ntt_tmp: scratch of the same layout).  *d_err is
+ * cleared by the run itself and becomes nonzero if an NTT input is >= r.
+ * Unused pointers may be NULL.  Runs of one set must not overlap. */
+int kinet_hip_job_set_run(kinet_job_set* set, const void* d_scalars, const void* d_pts, const void* d_inf,
+                          void* d_msm_out, const void* d_ntt_in, void* d_ntt_out, void* d_ntt_tmp, int* d_err,
+                          void* stream);
+
 /* ------------------------------------------------------------- tuning
  * Schedules are chosen per size class k = floor(log2 n) from in-process
  * tables; an empty cell means the built-in rule.  MSM cells are per curve

@@ -213,3 +213,67 @@ def test_ntt_batch_rejects_noncanonical(D):
     with pytest.raises(KinetError):
         D.ntt_batch(0, D.to_device(bytes(data)), n, count)
     del torch
+
+
+@pytest.mark.parametrize("curve,field", [(1, 1), (2, 0)])
+def test_job_set_one_graph_matches_per_job(D, curve, field):
+    """kinet_hip_job_set_*: uneven MSMs and NTT groups in one graph launch give
+    each job's bytes from the per-job entries (kinet_hip_msm_prepared,
+    kinet_hip_ntt_fr), on the first run (capture) and on replays with eager
+    work and new inputs in between; *d_err is cleared by every run and set by
+    a non-canonical NTT input."""
+    import torch
+
+    rnd = random.Random(curve * 7 + field)
+    sizes = [1, 3, 100, 4097, 20000, 777]
+    npts = max(sizes)
+    k0, d = 1234567, 0x1000001
+    pp = D.PreparedPoints(curve, _progression(curve, k0, d, npts))
+    logs = [(4, 3), (10, 2), (13, 1)]  # (log2 n, count)
+    off, groups = 0, []
+    for lg, cnt in logs:
+        groups.append((lg, cnt, off))
+        off += cnt << lg
+    d_tmp = torch.empty(off * 32, dtype=torch.uint8, device="cuda")
+    d_nout = torch.empty_like(d_tmp)
+    d_err = torch.zeros(1, dtype=torch.int32, device="cuda")
+    out = torch.empty(len(sizes) * (D.point_bytes(curve) * 3 // 2), dtype=torch.uint8, device="cuda")
+    rec = D.point_bytes(curve) * 3 // 2
+    stream = torch.cuda.Stream()
+    jset = D.JobSet(curve, sizes, field, groups)
+    try:
+        for rep in range(3):
+            sc = b"".join(rnd.getrandbits(254).to_bytes(32, "little") for _ in range(sum(sizes)))
+            vals = b"".join(rnd.randrange(M.BLS_R if field == 0 else M.BN_R).to_bytes(32, "little")
+                            for _ in range(off))
+            d_sc = D.to_device(sc)
+            d_nin = D.to_device(vals)
+            if rep == 0:
+                keep_sc, keep_nin = d_sc, d_nin
+            else:  # the same buffers, refilled: replays read them afresh
+                keep_sc.copy_(d_sc)
+                keep_nin.copy_(d_nin)
+            torch.cuda.synchronize()
+            print(f"job set curve={curve} field={field} run={rep}", flush=True)
+            jset.run(pp, keep_sc, out, keep_nin, d_nout, d_tmp, d_err, stream=stream)
+            assert int(d_err.item()) == 0
+            host = out.cpu().numpy().tobytes()
+            o = 0
+            for j, n in enumerate(sizes):
+                one = D.msm_prepared(pp, keep_sc[32 * o:32 * (o + n)], n=n).cpu().numpy().tobytes()
+                assert host[rec * j:rec * (j + 1)] == one, (rep, j)
+                o += n
+            for lg, cnt, go in groups:
+                for t in range(cnt):
+                    a = 32 * (go + (t << lg))
+                    b = a + (32 << lg)
+                    want = O.ntt(field, vals[a:b])
+                    assert d_nout[a:b].cpu().numpy().tobytes() == want, (rep, lg, t)
+        bad = bytearray(vals)
+        bad[32 * 5:32 * 6] = bytes([0xFF]) * 32
+        keep_nin.copy_(D.to_device(bytes(bad)))
+        torch.cuda.synchronize()
+        jset.run(pp, keep_sc, out, keep_nin, d_nout, d_tmp, d_err, stream=stream)
+        assert int(d_err.item()) != 0
+    finally:
+        jset.close()

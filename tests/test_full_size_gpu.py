@@ -99,10 +99,11 @@ def test_kzg_commit_2p22_closed_form(ca, curve):
 def test_batched_set_every_job_closed_form(ca):
     """The full batched job set -- 256 BN254 MSMs of 2^10 .. 2^18 points over a
     shared 2^18-point SRS prefix and 256 BN254-Fr NTTs of 2^12 .. 2^20
-    elements -- as the bench runs it (one batch pipeline replayed from a HIP
-    graph, one batched launch sequence per NTT size).  Every MSM job against
-    its closed form; every NTT job at one random output against Horner, plus
-    the device round trip iNTT(NTT(x)) = x; graph replays equal the first run."""
+    elements -- as ONE HIP graph launch (kinet_hip_job_set_run: the MSM batch
+    pipeline and the NTT groups on a forked branch), replayed with eager work
+    between replays.  Every MSM job against its closed form; every NTT job at
+    one random output against Horner, plus the round trip iNTT(NTT(x)) = x;
+    every replay equal to the first run."""
     import torch
     from crypto_amd import device as D
     from crypto_amd import inputs
@@ -112,39 +113,49 @@ def test_batched_set_every_job_closed_form(ca):
     msm_n, ntt_log = W.batched_set_sizes()
     ks = inputs.scalars_bytes(curve, 4242, W.BATCH_SRS_POINTS)
     sc = inputs.scalars_bytes(curve, 4243, sum(msm_n))
-    print(f"batched set: {len(msm_n)} MSMs ({sum(msm_n)} points), {len(ntt_log)} NTTs", flush=True)
+    print(f"batched set: {len(msm_n)} MSMs ({sum(msm_n)} points), {len(ntt_log)} NTTs, one graph", flush=True)
     pp = D.PreparedPoints(curve, _fixed_base_points(ca, curve, ks))
     d_sc = D.to_device(sc)
-    stream = torch.cuda.Stream()
-    batch = D.MsmBatch(curve, msm_n)
-    try:
-        first = batch.run(pp, d_sc, use_graph=True, stream=stream).clone()
-        for _ in range(2):
-            again = batch.run(pp, d_sc, use_graph=True, stream=stream)
-            assert torch.equal(again, first)
-    finally:
-        batch.close()
-    host = first.cpu().numpy().tobytes()
-    off = 0
-    for j, n in enumerate(msm_n):
-        t = O.fr_dot(field, sc[32 * off:32 * (off + n)], ks[:32 * n])
-        _check_point(curve, host[96 * j:96 * (j + 1)], O.g1_mul_gen(curve, t.to_bytes(32, "little")))
-        off += n
-
     off_of, total, groups = W.ntt_layout(ntt_log)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(4244)
     d_in = torch.randint(0, 256, (total * 32,), dtype=torch.uint8, device="cuda", generator=gen)
     d_in.view(-1, 32)[:, 31] &= 0x1F  # < 2^253 < r
     d_out = torch.empty_like(d_in)
-    d_back = torch.empty_like(d_in)
     d_tmp = torch.empty_like(d_in)
-    d_err = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_err = torch.ones(1, dtype=torch.int32, device="cuda")  # the run clears it
+    out = torch.empty(len(msm_n) * 96, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    jset = D.JobSet(curve, msm_n, field, [(lg, len(jobs), off_of[jobs[0]]) for lg, jobs in groups.items()])
+    try:
+        jset.run(pp, d_sc, out, d_in, d_out, d_tmp, d_err, stream=stream)
+        first_msm, first_ntt = out.clone(), d_out.clone()
+        assert int(d_err.item()) == 0
+        for rep in range(2):
+            out.zero_()
+            d_out.zero_()
+            torch.cuda.synchronize()
+            jset.run(pp, d_sc, out, d_in, d_out, d_tmp, d_err, stream=stream)
+            assert torch.equal(out, first_msm) and torch.equal(d_out, first_ntt), rep
+    finally:
+        jset.close()
+    # the MSM batch alone (kinet_hip_msm_batch_*) gives the same records
+    batch = D.MsmBatch(curve, msm_n)
+    try:
+        assert torch.equal(batch.run(pp, d_sc), first_msm)
+    finally:
+        batch.close()
+    host = first_msm.cpu().numpy().tobytes()
+    off = 0
+    for j, n in enumerate(msm_n):
+        t = O.fr_dot(field, sc[32 * off:32 * (off + n)], ks[:32 * n])
+        _check_point(curve, host[96 * j:96 * (j + 1)], O.g1_mul_gen(curve, t.to_bytes(32, "little")))
+        off += n
+
+    d_back = torch.empty_like(d_in)
     for lg, jobs in groups.items():
         o = 32 * off_of[jobs[0]]
         cnt = len(jobs) * (32 << lg)
-        D.ntt_batch(field, d_in[o:o + cnt], 1 << lg, len(jobs), d_out=d_out[o:o + cnt], d_tmp=d_tmp[o:o + cnt],
-                    d_err=d_err, sync=False)
         D.ntt_batch(field, d_out[o:o + cnt], 1 << lg, len(jobs), inverse=True, d_out=d_back[o:o + cnt],
                     d_tmp=d_tmp[o:o + cnt], d_err=d_err, sync=False)
     torch.cuda.synchronize()

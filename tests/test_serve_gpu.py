@@ -171,3 +171,34 @@ def test_serve_and_batch_with_srs_window_tables():
     assert got == want
     for s in list(plain.values()) + list(tabled.values()):
         s.close()
+
+
+def test_commit_batch_graph_replays_read_new_inputs(env):
+    """kzg_commit_batch with the same job sizes again and again: from the
+    second call on the commit pipeline (H2D, inverse NTTs, batched MSM, D2H)
+    replays as a captured graph; every call's commitments equal kzg_commit's
+    for that call's evaluations."""
+    ca, serve, srs = env
+    sizes = [16, 16, 256, 1024, 4096]
+    for rep in range(4):
+        evs = [_evals(2, 9000 + 10 * rep + i, n).tobytes() for i, n in enumerate(sizes)]
+        print(f"commit batch replay {rep}", flush=True)
+        got = ca.kzg_commit_batch([srs[2]] * len(sizes), evs)
+        assert got == [srs[2].commit(e) for e in evs], rep
+        if rep == 2:  # eager work on the device between replays
+            srs[1].commit(_evals(1, 77, 2048).tobytes())
+
+
+def test_commit_batch_graph_after_workspace_growth(env):
+    """A captured commit graph is replayed only while the lane's buffers and
+    MSM workspace keep their addresses: sizes A twice (the second call
+    captures), then larger sizes B (which grow the lane's workspace), then A
+    again -- the third A must be recaptured, not replayed over freed scratch,
+    and every call's commitments equal kzg_commit's."""
+    ca, serve, srs = env
+    plan_a = [1024, 1024]
+    plan_b = [4096] * 6
+    for rep, sizes in enumerate([plan_a, plan_a, plan_b, plan_a, plan_a]):
+        evs = [_evals(2, 9500 + 10 * rep + i, n).tobytes() for i, n in enumerate(sizes)]
+        print(f"commit batch after workspace growth: call {rep}, sizes {sizes}", flush=True)
+        assert ca.kzg_commit_batch([srs[2]] * len(sizes), evs) == [srs[2].commit(e) for e in evs], rep
