@@ -2,6 +2,7 @@
 // (one inverse-NTT launch per distinct size, one batched MSM over the SRS
 // prefix).  Used by kzg_commit_batch and by the serving loop (serve.hip).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.h"
@@ -30,6 +31,21 @@ size_t commit_batch_in_bytes(const std::vector<uint32_t>& n) {
 
 size_t commit_batch_out_bytes(int curve, size_t count) { return count * msm_out_bytes(curve) + 16; }
 
+// Batches of one shape are captured into a graph from this use on (the
+// capture and instantiation cost about as much as several direct launch
+// sequences, so only recurring shapes pay it back).  The environment
+// variable KINET_HIP_COMMIT_GRAPH_MIN_USES overrides it; 0 never captures.
+static uint32_t graph_min_uses() {
+    static const uint32_t v = [] {
+        const char* env = std::getenv("KINET_HIP_COMMIT_GRAPH_MIN_USES");
+        if (!env || !*env) return 4u;
+        char* end = nullptr;
+        const unsigned long x = std::strtoul(env, &end, 10);
+        return (end && *end == 0 && x < 1000000ul) ? (uint32_t)x : 4u;
+    }();
+    return v;
+}
+
 void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector<uint32_t>& n) {
     const int curve = srs->curve;
     const size_t count = n.size();
@@ -56,7 +72,7 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
     auto key = std::make_pair(curve, kv);
     auto it = lane.plans.find(key);
     if (it == lane.plans.end()) {
-        if (lane.plans.size() >= 256) lane.plans.clear();
+        if (lane.plans.size() >= 1024) lane.plans.clear();
         std::vector<MsmJob> specs(count);
         uint64_t sc = 0;
         for (size_t i = 0; i < count; ++i) {
@@ -105,7 +121,8 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
     ++lp.uses;
     const void* bufs[LanePlan::kAddrs] = {ev, co, tmp, out, lane.pin_in.ptr, pin_out, pts, inf};
     lane.ws.addresses(bufs + 8);
-    bool use_graph = lp.uses >= 2 && !lp.no_graph;
+    const uint32_t min_uses = graph_min_uses();
+    bool use_graph = min_uses > 0 && lp.uses >= min_uses && !lp.no_graph;
     if (use_graph && !(lp.exec && std::equal(bufs, bufs + LanePlan::kAddrs, lp.bufs))) {
         lp.drop_graph();
         KZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -129,9 +129,9 @@ struct CommitLane {
 // one inverse-NTT launch sequence per distinct size, one batched MSM over the
 // SRS prefix, D2H of the records (wire + compressed) and the error flag into
 // lane.pin_out: record i at pin_out + i * msm_out_bytes, the flag (int) after
-// the last record.  The first batch of a set of job sizes launches directly;
-// from the second on, the sequence is captured into a HIP graph (once per
-// buffer layout) and replayed.  Records the lane's done event.  Nothing
+// the last record.  A set of job sizes launches directly at first; once it
+// recurs (4 uses by default), the sequence is captured into a HIP graph (once
+// per buffer layout) and replayed.  Records the lane's done event.  Nothing
 // synchronises.
 void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector<uint32_t>& n);
 // Bytes of lane.pin_in / lane.pin_out a batch of these sizes needs.

@@ -174,13 +174,13 @@ def test_serve_and_batch_with_srs_window_tables():
 
 
 def test_commit_batch_graph_replays_read_new_inputs(env):
-    """kzg_commit_batch with the same job sizes again and again: from the
-    second call on the commit pipeline (H2D, inverse NTTs, batched MSM, D2H)
-    replays as a captured graph; every call's commitments equal kzg_commit's
-    for that call's evaluations."""
+    """kzg_commit_batch with the same job sizes again and again: once the shape
+    recurs (the fourth call by default) the commit pipeline (H2D, inverse
+    NTTs, batched MSM, D2H) is captured and later calls replay the graph;
+    every call's commitments equal kzg_commit's for that call's evaluations."""
     ca, serve, srs = env
     sizes = [16, 16, 256, 1024, 4096]
-    for rep in range(4):
+    for rep in range(8):
         evs = [_evals(2, 9000 + 10 * rep + i, n).tobytes() for i, n in enumerate(sizes)]
         print(f"commit batch replay {rep}", flush=True)
         got = ca.kzg_commit_batch([srs[2]] * len(sizes), evs)
@@ -191,14 +191,14 @@ def test_commit_batch_graph_replays_read_new_inputs(env):
 
 def test_commit_batch_graph_after_workspace_growth(env):
     """A captured commit graph is replayed only while the lane's buffers and
-    MSM workspace keep their addresses: sizes A twice (the second call
-    captures), then larger sizes B (which grow the lane's workspace), then A
-    again -- the third A must be recaptured, not replayed over freed scratch,
-    and every call's commitments equal kzg_commit's."""
+    MSM workspace keep their addresses: sizes A five times (captured at the
+    fourth use, replayed at the fifth), then larger sizes B (which grow the
+    lane's workspace), then A again -- A must be recaptured, not replayed over
+    freed scratch -- and every call's commitments equal kzg_commit's."""
     ca, serve, srs = env
-    plan_a = [1024, 1024]
-    plan_b = [4096] * 6
-    for rep, sizes in enumerate([plan_a, plan_a, plan_b, plan_a, plan_a]):
+    plan_a = [1024, 1024, 2048]
+    plan_b = [4096] * 48  # more than any batch of the earlier tests: the workspace grows
+    for rep, sizes in enumerate([plan_a] * 5 + [plan_b] + [plan_a] * 2):
         evs = [_evals(2, 9500 + 10 * rep + i, n).tobytes() for i, n in enumerate(sizes)]
         print(f"commit batch after workspace growth: call {rep}, sizes {sizes}", flush=True)
         assert ca.kzg_commit_batch([srs[2]] * len(sizes), evs) == [srs[2].commit(e) for e in evs], rep
