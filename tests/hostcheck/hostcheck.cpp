@@ -1,9 +1,13 @@
 // Host build of the library's device arithmetic (crypto_amd/csrc/field.h,
 // curve.h), exported for the CPU tests, which compare it with the oracle.
 // Test infrastructure only.
+#include <atomic>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "curve.h"
+#include "curve_quad.h"
 #include "glv.h"
 
 using namespace kz;
@@ -97,9 +101,181 @@ void point_op(int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {
     write_wire<C>(r, out);
 }
 
+// ---------------------------------------------------------------- quads
+// Four host threads in lock step stand in for the four lanes of a quad: the
+// device's quad-cooperative point operations (curve_quad.h) run unchanged
+// with this lane exchange in place of ds_swizzle.
+struct SpinBarrier {
+    std::atomic<int> count{0}, gen{0};
+    void wait() {
+        const int g = gen.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) == 3) {
+            count.store(0, std::memory_order_relaxed);
+            gen.fetch_add(1, std::memory_order_acq_rel);
+        } else {
+            for (int spins = 0; gen.load(std::memory_order_acquire) == g; ++spins)
+                if (spins > 4096) std::this_thread::yield();
+        }
+    }
+};
+
+SpinBarrier g_quad_bar;
+uint32_t g_quad_slot[4][16];
+
+struct HostQuad {
+    static thread_local int id;
+    static int lane() { return id; }
+    template <int K, class F>
+    static Fe<F> read(const Fe<F>& a) {
+        std::memcpy(g_quad_slot[id], a.v, sizeof a.v);
+        g_quad_bar.wait();
+        Fe<F> r;
+        std::memcpy(r.v, g_quad_slot[K], sizeof r.v);
+        g_quad_bar.wait();
+        return r;
+    }
+};
+thread_local int HostQuad::id = 0;
+
+// Runs fn(lane) on four lanes; true if their results (returned as Xyzz)
+// agree, with the result in out.
+template <class F, class Fn>
+bool run_quad(Fn fn, Xyzz<F>& out) {
+    Xyzz<F> res[4];
+    std::thread t[3];
+    for (int k = 1; k < 4; ++k)
+        t[k - 1] = std::thread([&, k] {
+            HostQuad::id = k;
+            res[k] = fn();
+        });
+    HostQuad::id = 0;
+    res[0] = fn();
+    for (auto& th : t) th.join();
+    for (int k = 1; k < 4; ++k)
+        if (std::memcmp(&res[k], &res[0], sizeof res[0]) != 0) return false;
+    out = res[0];
+    return true;
+}
+
+// value + k p without reduction (value < p, k small: stays below 2^(29 N))
+template <class F>
+Fe<F> add_kp(const Fe<F>& a, int k) {
+    uint32_t w[16] = {0}, pw[16] = {0};
+    const int nw = (kLimbBits * F::N + 31) / 32;
+    fe_to_words<F>(a, w, nw);
+    fe_to_words<F>(fe_const<F>(F::P), pw, nw);
+    for (int t = 0; t < k; ++t) {
+        uint64_t c = 0;
+        for (int i = 0; i < nw; ++i) {
+            c += (uint64_t)w[i] + pw[i];
+            w[i] = (uint32_t)c;
+            c >>= 32;
+        }
+    }
+    return fe_from_words<F>(w, nw);
+}
+
+// XYZZ form of an affine wire point with ZZ = z^2, ZZZ = z^3 (z LE bytes),
+// each coordinate raised by k[i] p (the lazy forms the device passes along).
+template <class C>
+Xyzz<typename C::Fp> lazy_point(const uint8_t* wire, const uint8_t* z_le, const int* k) {
+    using F = typename C::Fp;
+    Xyzz<F> p = read_wire<C>(wire);
+    if (xyzz_is_inf<F>(p)) return p;
+    const Fe<F> z = fe_to_mont<F>(from_le<F>(z_le, C::FB));
+    const Fe<F> z2 = fe_mul<F>(z, z), z3 = fe_mul<F>(z2, z);
+    Xyzz<F> r;
+    r.X = add_kp<F>(fe_mul<F>(p.X, z2), k[0]);
+    r.Y = add_kp<F>(fe_mul<F>(p.Y, z3), k[1]);
+    r.ZZ = add_kp<F>(z2, k[2]);
+    r.ZZZ = add_kp<F>(z3, k[3]);
+    return r;
+}
+
+template <class C>
+int quad_op(int op, int lazy, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, const int* ka,
+            const int* kb, uint8_t* out) {
+    using F = typename C::Fp;
+    const Xyzz<F> P = lazy_point<C>(a, za, ka), Q = lazy_point<C>(b, zb, kb);
+    Xyzz<F> r;
+    bool same;
+    if (lazy)
+        same = run_quad<F>([&] { return op == 0 ? xyzz_add_quad<F, MulCallLazy, HostQuad>(P, Q)
+                                                : xyzz_dbl_quad<F, MulCallLazy, HostQuad>(P); }, r);
+    else
+        same = run_quad<F>([&] { return op == 0 ? xyzz_add_quad<F, MulCall, HostQuad>(P, Q)
+                                                : xyzz_dbl_quad<F, MulCall, HostQuad>(P); }, r);
+    write_wire<C>(xyzz_canon_acc<F>(r), out);
+    return same ? 0 : -1;
+}
+
+// Sum of n points (n a power of two) by levels of lazy quad additions with no
+// canonical step in between: at each level element i takes element i + half,
+// the shape of a fold tree (three levels for fan-in 8); then canonical.
+template <class C>
+int quad_tree(int lazy, const uint8_t* pts, const uint8_t* zs, int n, uint8_t* out) {
+    using F = typename C::Fp;
+    std::vector<Xyzz<F>> v(n);
+    const int k0[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) v[i] = lazy_point<C>(pts + 2 * C::FB * i, zs + C::FB * i, k0);
+    bool same = true;
+    for (int half = n / 2; half >= 1; half /= 2)
+        for (int i = 0; i < half; ++i) {
+            Xyzz<F> r;
+            const Xyzz<F> A = v[i], B = v[i + half];
+            if (lazy)
+                same &= run_quad<F>([&] { return xyzz_add_quad<F, MulCallLazy, HostQuad>(A, B); }, r);
+            else
+                same &= run_quad<F>([&] { return xyzz_add_quad<F, MulCall, HostQuad>(A, B); }, r);
+            v[i] = r;
+        }
+    write_wire<C>(xyzz_canon_acc<F>(v[0]), out);
+    return same ? 0 : -1;
+}
+
+// count lazy quad doublings of a point (the window combine's chain), then canonical
+template <class C>
+int quad_dbl_chain(const uint8_t* a, const uint8_t* za, const int* ka, int count, uint8_t* out) {
+    using F = typename C::Fp;
+    const Xyzz<F> P = lazy_point<C>(a, za, ka);
+    Xyzz<F> r;
+    const bool same = run_quad<F>([&] {
+        Xyzz<F> x = P;
+        for (int i = 0; i < count; ++i) x = xyzz_dbl_quad<F, MulCallLazy, HostQuad>(x);
+        return x;
+    }, r);
+    write_wire<C>(xyzz_canon_acc<F>(r), out);
+    return same ? 0 : -1;
+}
+
 }  // namespace
 
 extern "C" {
+
+// The device's quad-cooperative point operations run on the host (four
+// lock-stepped threads as the quad's lanes).  curve 0 secp256k1, 1 BN254,
+// 2 BLS12-381; op 0 a + b, 1 2a; lazy 0 canonical products, 1 lazy; z: one
+// LE coordinate-size value per point (nonzero), k: four coordinate offsets
+// per point (multiples of p added to X, Y, ZZ, ZZZ).  Return 0 when the four
+// lanes agreed.
+int hc_quad_op(int curve, int op, int lazy, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb,
+               const int* ka, const int* kb, uint8_t* out) {
+    if (curve == 2) return quad_op<CurveBLS12381>(op, lazy, a, b, za, zb, ka, kb, out);
+    if (curve == 1) return quad_op<CurveBN254>(op, lazy, a, b, za, zb, ka, kb, out);
+    return quad_op<CurveSecp256k1>(op, lazy, a, b, za, zb, ka, kb, out);
+}
+
+int hc_quad_tree(int curve, int lazy, const uint8_t* pts, const uint8_t* zs, int n, uint8_t* out) {
+    if (curve == 2) return quad_tree<CurveBLS12381>(lazy, pts, zs, n, out);
+    if (curve == 1) return quad_tree<CurveBN254>(lazy, pts, zs, n, out);
+    return quad_tree<CurveSecp256k1>(lazy, pts, zs, n, out);
+}
+
+int hc_quad_dbl_chain(int curve, const uint8_t* a, const uint8_t* za, const int* ka, int count, uint8_t* out) {
+    if (curve == 2) return quad_dbl_chain<CurveBLS12381>(a, za, ka, count, out);
+    if (curve == 1) return quad_dbl_chain<CurveBN254>(a, za, ka, count, out);
+    return quad_dbl_chain<CurveSecp256k1>(a, za, ka, count, out);
+}
 
 // field: 0 BLS Fp, 1 BLS Fr, 2 BN Fp, 3 BN Fr, 4 secp256k1 Fp
 void hc_field_op(int field, int op, const uint8_t* a, const uint8_t* b, uint8_t* out) {

@@ -178,3 +178,112 @@ def test_glv_split_bls(hc):
         assert (k1 + lam * k2 - s) % r == 0, hex(s)
         max_bits = max(max_bits, abs(k1).bit_length(), abs(k2).bit_length())
     assert max_bits <= 127
+
+# ------------------------------------------------ quad-cooperative operations
+# xyzz_add_quad / xyzz_dbl_quad (curve_quad.h) are the fold, reduction and
+# combine's point operations.  Here they run on the host, unchanged, with four
+# lock-stepped threads as the quad's lanes.  Inputs are points in the lazy
+# forms the device passes between operations: ZZ = z^2, ZZZ = z^3 for a
+# random z, and each coordinate raised by a multiple of p up to the lazy
+# bounds (X < 8p, Y < 4p, ZZ, ZZZ < 2p).  BN254's headroom R / p is about 169
+# against 2^25 for BLS12-381, so bound slips show on BN254 first.
+LAZY_MAX = (7, 3, 1, 1)
+
+
+def _rand_points(curve, count, rnd):
+    """count random points of the group as (x, y): k G by the C oracle (BN254,
+    BLS12-381), or a progression P + i D for secp256k1 (which it lacks)."""
+    G, r = M.CURVES[curve]["g"], M.CURVES[curve]["r"]
+    if curve == 0:
+        P, D = M.mul(0, G, rnd.randrange(1, r)), M.mul(0, G, rnd.randrange(1, r))
+        out = []
+        for _ in range(count):
+            out.append(P)
+            P = M.add(0, P, D)
+        return out
+    return [M.from_wire(curve, O.g1_mul_gen(curve, rnd.randrange(1, r).to_bytes(32, "little")))
+            for _ in range(count)]
+
+
+def _quad_args(curve, P, rnd, lazy, extreme=False):
+    p, fb = M.CURVES[curve]["p"], M.CURVES[curve]["fb"]
+    z = rnd.randrange(1, p)
+    if extreme:
+        k = LAZY_MAX if lazy else (0, 0, 0, 0)
+    else:
+        k = tuple(rnd.randint(0, m) for m in LAZY_MAX) if lazy else (0, 0, 0, 0)
+    w = M.to_wire(curve, P) if P is not None else bytes(2 * fb)
+    return w, z.to_bytes(fb, "little"), (ctypes.c_int * 4)(*k)
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+@pytest.mark.parametrize("lazy", [0, 1])
+def test_quad_ops_match_textbook(hc, curve, lazy):
+    """a + b and 2a by the quad operations for random points, a = b (the
+    doubling case reached through the addition, with different z, so the
+    inputs differ as XYZZ and agree only as points), a = -b, identities on
+    either side; coordinates at random and at the extreme lazy offsets."""
+    G, r = M.CURVES[curve]["g"], M.CURVES[curve]["r"]
+    pb = O.POINT_BYTES[curve]
+    rnd = random.Random(1000 + 10 * curve + lazy)
+    out = ctypes.create_string_buffer(pb)
+    pts = _rand_points(curve, 24, rnd)
+    cases = [(pts[i], pts[i + 1]) for i in range(0, 16, 2)]
+    cases += [(pts[16], pts[16]), (pts[17], M.neg(curve, pts[17])), (None, pts[18]), (pts[19], None), (None, None)]
+    cases += [(pts[20], M.add(curve, pts[20], pts[20]))]
+    for i, (A, B) in enumerate(cases):
+        for extreme in (False, True):
+            wa, za, ka = _quad_args(curve, A, rnd, lazy, extreme)
+            wb, zb, kb = _quad_args(curve, B, rnd, lazy, extreme)
+            assert hc.hc_quad_op(curve, 0, lazy, wa, wb, za, zb, ka, kb, out) == 0, "lanes disagree"
+            want = M.add(curve, A, B)
+            assert out.raw == (M.to_wire(curve, want) if want is not None else bytes(pb)), (curve, lazy, i, extreme)
+            assert hc.hc_quad_op(curve, 1, lazy, wa, wa, za, za, ka, ka, out) == 0
+            want = M.add(curve, A, A)
+            assert out.raw == (M.to_wire(curve, want) if want is not None else bytes(pb)), (curve, lazy, i, "dbl")
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+@pytest.mark.parametrize("n", [8, 64])
+def test_quad_lazy_trees(hc, curve, n):
+    """Trees of lazy quad additions with no canonical step between levels
+    (fan-in 8 has three levels, fan-in 64 six) over random points with
+    identities, repeated points and cancelling pairs mixed in, so doublings
+    and identities occur mid-tree on lazy inputs; against the textbook sum."""
+    G, r, p, fb = M.CURVES[curve]["g"], M.CURVES[curve]["r"], M.CURVES[curve]["p"], M.CURVES[curve]["fb"]
+    pb = O.POINT_BYTES[curve]
+    rnd = random.Random(2000 + curve * 100 + n)
+    out = ctypes.create_string_buffer(pb)
+    for trial in range(6):
+        pts = _rand_points(curve, n, rnd)
+        if trial >= 1:
+            pts[1] = None
+            pts[n // 2] = pts[0]                      # doubling at the first level
+            pts[n // 2 + 1] = M.neg(curve, pts[1] or pts[2])
+        if trial >= 3:
+            pts = [pts[0]] * n                         # every level doubles
+        if trial == 5:
+            pts = [pts[0], M.neg(curve, pts[0])] * (n // 2)  # sums through the identity
+        wire = b"".join(M.to_wire(curve, P) if P is not None else bytes(2 * fb) for P in pts)
+        zs = b"".join(rnd.randrange(1, p).to_bytes(fb, "little") for _ in range(n))
+        want = None
+        for P in pts:
+            want = M.add(curve, want, P)
+        for lazy in (1, 0):
+            assert hc.hc_quad_tree(curve, lazy, wire, zs, n, out) == 0, "lanes disagree"
+            assert out.raw == (M.to_wire(curve, want) if want is not None else bytes(pb)), (curve, n, trial, lazy)
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+def test_quad_doubling_chains(hc, curve):
+    """Chains of lazy quad doublings as in the window combine (16 per window,
+    112 over a 127-bit GLV scalar), from points at the extreme lazy offsets."""
+    G, r = M.CURVES[curve]["g"], M.CURVES[curve]["r"]
+    pb = O.POINT_BYTES[curve]
+    rnd = random.Random(3000 + curve)
+    out = ctypes.create_string_buffer(pb)
+    for count in (1, 16, 112):
+        P = _rand_points(curve, 1, rnd)[0]
+        w, z, k = _quad_args(curve, P, rnd, 1, extreme=True)
+        assert hc.hc_quad_dbl_chain(curve, w, z, k, count, out) == 0
+        assert out.raw == M.to_wire(curve, M.mul(curve, P, 1 << count)), (curve, count)
