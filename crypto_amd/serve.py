@@ -19,7 +19,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "batches", "max_batch_requests",
-                                               "peak_inflight_bytes", "budget_waits")]
+                                               "peak_inflight_bytes", "budget_waits", "rerun_batches",
+                                               "launch_failures")]
 
 
 class KzgServer:

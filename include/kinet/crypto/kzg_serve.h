@@ -36,6 +36,8 @@ typedef struct kzg_server_stats {
     uint64_t max_batch_requests;  /* largest batch seen */
     uint64_t peak_inflight_bytes; /* largest admitted footprint */
     uint64_t budget_waits;        /* times a ready batch waited for the budget */
+    uint64_t rerun_batches;       /* batches re-run one request at a time after an error flag */
+    uint64_t launch_failures;     /* batches whose staging or launch failed (their requests fail) */
 } kzg_server_stats;
 
 /* srs_bn254 / srs_bls12_381: resident SRS handles (either may be NULL if that

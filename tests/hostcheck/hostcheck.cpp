@@ -1,7 +1,11 @@
 // Host build of the library's device arithmetic (crypto_amd/csrc/field.h,
 // curve.h), exported for the CPU tests, which compare it with the oracle.
 // Test infrastructure only.
+#include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <map>
+#include <stdexcept>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -9,6 +13,7 @@
 #include "curve.h"
 #include "curve_quad.h"
 #include "glv.h"
+#include "serve_core.h"
 
 using namespace kz;
 
@@ -248,9 +253,119 @@ int quad_dbl_chain(const uint8_t* a, const uint8_t* za, const int* ka, int count
     return same ? 0 : -1;
 }
 
+// ---------------------------------------------------------------- serving
+// A host executor for the serving scheduler (serve_core.h) with injected
+// failures: it "commits" a request as the first 48 bytes of its evaluations
+// (so the tests can tell results apart), flags a batch whose evaluations hold
+// an element >= 2^255 (above r), throws from launch() for chosen batches
+// (a staging allocation or launch failure) and reports a device failure from
+// poll() for others.  Batches take `latency` to finish.
+struct FakeExec {
+    struct L {
+        int curve = 0;
+        int batch = -1;
+        bool err = false, dev_fail = false;
+        std::vector<std::vector<uint8_t>> results;
+        std::chrono::steady_clock::time_point ready;
+    };
+    std::vector<L> lane;
+    std::vector<int> launch_fail, device_fail;  // batch indices
+    std::chrono::microseconds latency{200};
+    std::mutex mu;
+    int next_batch = 0;
+    std::map<const uint8_t*, int> batch_of;  // request evaluations -> last batch that ran them
+
+    explicit FakeExec(size_t lanes) : lane(lanes) {}
+    size_t lanes() const { return lane.size(); }
+    uint64_t footprint(int, const std::vector<uint32_t>& sizes) {
+        uint64_t b = 0;
+        for (uint32_t n : sizes) b += 32ull * n;
+        return b;
+    }
+    void launch(size_t li, int curve, const std::vector<std::pair<const uint8_t*, uint32_t>>& src,
+                const std::vector<uint32_t>&) {
+        std::lock_guard<std::mutex> g(mu);
+        const int b = next_batch++;
+        for (auto& p : src) batch_of[p.first] = b;
+        if (std::find(launch_fail.begin(), launch_fail.end(), b) != launch_fail.end())
+            throw std::runtime_error("injected launch failure");
+        L& l = lane[li];
+        l.curve = curve;
+        l.batch = b;
+        l.dev_fail = std::find(device_fail.begin(), device_fail.end(), b) != device_fail.end();
+        l.err = false;
+        l.results.clear();
+        for (auto& p : src) {
+            for (uint32_t e = 0; e < p.second; ++e) l.err |= p.first[32 * e + 31] >= 0x80;
+            l.results.emplace_back(p.first, p.first + 48);
+        }
+        l.ready = std::chrono::steady_clock::now() + latency;
+    }
+    int poll(size_t li) {
+        std::lock_guard<std::mutex> g(mu);
+        const L& l = lane[li];
+        if (std::chrono::steady_clock::now() < l.ready) return 0;
+        return l.dev_fail ? -1 : 1;
+    }
+    bool error_flag(size_t li, size_t) {
+        std::lock_guard<std::mutex> g(mu);
+        return lane[li].err;
+    }
+    const uint8_t* result(size_t li, size_t k) {
+        std::lock_guard<std::mutex> g(mu);
+        return lane[li].results[k].data();
+    }
+};
+
 }  // namespace
 
 extern "C" {
+
+// One serving scenario on the host scheduler with the fake executor: nreq
+// requests (curve[i], n[i] elements, bad[i] != 0 puts an element >= r into
+// it) are submitted at once, then waited for.  launch_fail / device_fail:
+// batch indices (in launch order) whose launch throws / whose device work
+// fails, each list ended by -1.  Outputs: status[i] (0 or the error code),
+// the first 8 bytes of each commitment (commit8[i]), the batch that ran each
+// request last (batch[i]) and the 9 statistics in kzg_server_stats order.
+int hc_serve_scenario(uint32_t lanes, uint32_t max_batch, uint64_t max_batch_elems, uint64_t budget, size_t nreq,
+                      const int* curve, const uint32_t* n, const int* bad, const int* launch_fail,
+                      const int* device_fail, int latency_us, int* status, uint64_t* commit8, int* batch,
+                      uint64_t* stats_out) {
+    FakeExec ex(lanes);
+    for (const int* p = launch_fail; p && *p >= 0; ++p) ex.launch_fail.push_back(*p);
+    for (const int* p = device_fail; p && *p >= 0; ++p) ex.device_fail.push_back(*p);
+    ex.latency = std::chrono::microseconds(latency_us);
+    std::vector<std::vector<uint8_t>> ev(nreq);
+    uint32_t maxn = 0;
+    for (size_t i = 0; i < nreq; ++i) {
+        ev[i].assign(32 * (size_t)std::max<uint32_t>(n[i], 2), 0);
+        for (int k = 0; k < 8; ++k) ev[i][k] = (uint8_t)((i + 1) >> (8 * k));  // the request's id
+        if (bad[i]) ev[i][32 * (n[i] - 1) + 31] = 0xFF;
+        maxn = std::max(maxn, n[i]);
+    }
+    kzg_server_config cfg{lanes, max_batch, max_batch_elems, budget};
+    const size_t max_n[3] = {0, maxn, maxn};
+    std::vector<int64_t> tk(nreq);
+    kzg_server_stats st;
+    {
+        ServeCore<FakeExec> core(ex, cfg, max_n);
+        for (size_t i = 0; i < nreq; ++i) tk[i] = core.submit((uint32_t)curve[i], ev[i].data(), n[i]);
+        for (size_t i = 0; i < nreq; ++i) {
+            uint8_t c[48] = {0};
+            status[i] = tk[i] < 0 ? (int)tk[i] : core.wait(tk[i], c, nullptr, nullptr);
+            uint64_t v = 0;
+            std::memcpy(&v, c, 8);
+            commit8[i] = v;
+        }
+        st = core.stats();
+    }
+    for (size_t i = 0; i < nreq; ++i) batch[i] = ex.batch_of.count(ev[i].data()) ? ex.batch_of[ev[i].data()] : -1;
+    const uint64_t sv[9] = {st.submitted, st.completed, st.failed, st.batches, st.max_batch_requests,
+                            st.peak_inflight_bytes, st.budget_waits, st.rerun_batches, st.launch_failures};
+    std::memcpy(stats_out, sv, sizeof sv);
+    return 0;
+}
 
 // The device's quad-cooperative point operations run on the host (four
 // lock-stepped threads as the quad's lanes).  curve 0 secp256k1, 1 BN254,
