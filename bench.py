@@ -562,6 +562,7 @@ def ntt_and_kzg(lib, torch, p_mad: float, reps: int = 10):
     rooflines, and KZG commitments at n = 2^22 on both curves (device-resident
     evaluations and SRS; iNTT -> MSM -> compressed point, per commit)."""
     import crypto_amd
+    import crypto_amd.tune
 
     stream = torch.cuda.Stream()
     out = {"ntt": {}, "kzg_2p22": {}}
@@ -591,8 +592,10 @@ def ntt_and_kzg(lib, torch, p_mad: float, reps: int = 10):
                     stream.synchronize()
                     ms = e0.elapsed_time(e1) / reps
                     mads = (n // 2) * logn * 136 + (n if inverse else 0) * 136
-                    passes = 1 if logn <= 10 else (2 if logn <= 20 else 3)
+                    split = [v for v in crypto_amd.tune.ntt_split(field, logn) if v]
+                    passes = len(split)
                     out["ntt"][f"{fname}_2p{logn}_{'inverse' if inverse else 'forward'}"] = {
+                        "passes": split,
                         "ms": round(ms, 4),
                         "melems_per_s": round(n / ms / 1e3, 1),
                         "hbm_gbs_compulsory": round(64 * n / (ms * 1e-3) / 1e9, 1),

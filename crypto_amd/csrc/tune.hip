@@ -305,15 +305,22 @@ void tune_ntt(int field, int log_min, int log_max) {
             ntt_device_batch(field, lg, false, in.as<uint32_t>(), dst, tmp.as<uint32_t>(), count, d_err, s);
         };
         run(ref.as<uint32_t>());
+        // each trial's four-step tables are freed after it (up to n x 36 bytes
+        // per split); the chosen split's are rebuilt below
+        const size_t mark = ntt_table_mark();
         int bp[3] = {0, 0, 0};
         float best = 0;
-        for (int p1 = 1; p1 <= std::min(10, lg); ++p1)
-            for (int p2 = 0; p2 <= std::min(10, lg - p1); ++p2)
-                for (int p3 = 0; p3 <= std::min(10, lg - p1 - p2); ++p3) {
+        const int pm = kNttPassMaxLog;
+        for (int p1 = 1; p1 <= std::min(pm, lg); ++p1)
+            for (int p2 = 0; p2 <= std::min(pm, lg - p1); ++p2)
+                for (int p3 = 0; p3 <= std::min(pm, lg - p1 - p2); ++p3) {
                     if (!ntt_split_valid(lg, p1, p2, p3)) continue;
                     ntt_set_schedule(field, lg, p1, p2, p3);
                     const float ms = time_ms([&] { run(out.as<uint32_t>()); }, s);
-                    if (!device_equal(ref.ptr, out.ptr, count * n * 32, d_err, s)) {
+                    const bool same = device_equal(ref.ptr, out.ptr, count * n * 32, d_err, s);
+                    KZ_HIP(hipStreamSynchronize(s));
+                    ntt_drop_tables_since(mark);
+                    if (!same) {
                         ntt_set_schedule(field, lg, o[0], o[1], o[2]);
                         throw Mismatch("NTT split " + std::to_string(p1) + "/" + std::to_string(p2) + "/" +
                                        std::to_string(p3) + " differs from the built-in split");
@@ -326,6 +333,7 @@ void tune_ntt(int field, int log_min, int log_max) {
                     }
                 }
         ntt_set_schedule(field, lg, bp[0], bp[1], bp[2]);
+        ntt_prepare(field, lg, false, s);
     }
     KZ_HIP(hipStreamSynchronize(s));
 }

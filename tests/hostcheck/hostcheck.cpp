@@ -13,6 +13,7 @@
 #include "curve.h"
 #include "curve_quad.h"
 #include "glv.h"
+#include "ntt_math.h"
 #include "serve_core.h"
 
 using namespace kz;
@@ -319,6 +320,71 @@ struct FakeExec {
 
 }  // namespace
 
+// One NTT pass of L = 2^logL points with the device kernel's lazy butterflies
+// (ntt_math.h) in its order: bit-reversed load, a lone twiddle-free stage 0
+// for odd logL, then radix-4 stage pairs; the end as in a last pass
+// (canonical; n^-1 folded in for the inverse).  in: L 32-byte LE integers
+// below 2^256 (a pass input may be up to 2r); tw: w_L^e for e < L/2 and ninv,
+// canonical 32-byte LE.  *max_top gets the largest top limb held between stages.
+template <class F>
+void ntt_lazy_pass(int logL, int inverse, const uint8_t* in, const uint8_t* tw, const uint8_t* ninv, uint8_t* out,
+                   uint32_t* max_top) {
+    const uint32_t L = 1u << logL;
+    auto load = [](const uint8_t* b) {
+        uint32_t w[8];
+        for (int k = 0; k < 8; ++k) w[k] = (uint32_t)b[4 * k] | ((uint32_t)b[4 * k + 1] << 8) |
+                                           ((uint32_t)b[4 * k + 2] << 16) | ((uint32_t)b[4 * k + 3] << 24);
+        return fe_from_words<F>(w, 8);
+    };
+    std::vector<Fe<F>> x(L), w(std::max<uint32_t>(1, L / 2));
+    for (uint32_t j = 0; j < L; ++j) {
+        uint32_t r = 0;
+        for (int b = 0; b < logL; ++b) r |= ((j >> b) & 1u) << (logL - 1 - b);
+        x[r] = load(in + 32 * j);
+    }
+    for (size_t e = 0; e < w.size(); ++e) w[e] = fe_to_mont<F>(load(tw + 32 * e));
+    uint32_t top = 0;
+    auto note = [&] {
+        for (auto& v : x) top = std::max(top, v.v[F::N - 1]);
+    };
+    uint32_t s = 0;
+    if (logL & 1) {
+        for (uint32_t p = 0; p < L / 2; ++p) ntt_bfly0<F>(x[2 * p], x[2 * p + 1]);
+        note();
+        s = 1;
+    }
+    for (; s < (uint32_t)logL; s += 2) {
+        const uint32_t h = 1u << s, sh1 = logL - 1 - s, sh2 = logL - 2 - s;
+        for (uint32_t q = 0; q < L / 4; ++q) {
+            const uint32_t i = q & (h - 1), e = ((q >> s) << (s + 2)) + i;
+            if (s == 0) ntt_radix4<F, true>(x[e], x[e + h], x[e + 2 * h], x[e + 3 * h], w[i << sh2], w[i << sh2],
+                                            w[(i + h) << sh2]);
+            else ntt_radix4<F, false>(x[e], x[e + h], x[e + 2 * h], x[e + 3 * h], w[i << sh1], w[i << sh2],
+                                      w[(i + h) << sh2]);
+        }
+        note();
+    }
+    const Fe<F> nv = fe_to_mont<F>(load(ninv));
+    for (uint32_t k = 0; k < L; ++k) {
+        const Fe<F> y = inverse ? fe_mul<F>(x[k], nv) : fe_reduce_wide<F>(x[k]);
+        uint32_t wd[8];
+        fe_to_words<F>(y, wd, 8);
+        for (int b = 0; b < 32; ++b) out[32 * k + b] = (uint8_t)(wd[b / 4] >> (8 * (b % 4)));
+    }
+    *max_top = top;
+}
+
+template <class F>
+void reduce_wide_bytes(const uint8_t* in36, uint8_t* out32) {
+    uint32_t w[9];
+    for (int k = 0; k < 9; ++k) w[k] = (uint32_t)in36[4 * k] | ((uint32_t)in36[4 * k + 1] << 8) |
+                                       ((uint32_t)in36[4 * k + 2] << 16) | ((uint32_t)in36[4 * k + 3] << 24);
+    const Fe<F> y = fe_reduce_wide<F>(fe_from_words<F>(w, 9));
+    uint32_t wd[8];
+    fe_to_words<F>(y, wd, 8);
+    for (int b = 0; b < 32; ++b) out32[b] = (uint8_t)(wd[b / 4] >> (8 * (b % 4)));
+}
+
 extern "C" {
 
 // One serving scenario on the host scheduler with the fake executor: nreq
@@ -390,6 +456,19 @@ int hc_quad_dbl_chain(int curve, const uint8_t* a, const uint8_t* za, const int*
     if (curve == 2) return quad_dbl_chain<CurveBLS12381>(a, za, ka, count, out);
     if (curve == 1) return quad_dbl_chain<CurveBN254>(a, za, ka, count, out);
     return quad_dbl_chain<CurveSecp256k1>(a, za, ka, count, out);
+}
+
+// field: 0 BLS12-381 Fr, 1 BN254 Fr
+void hc_ntt_lazy_pass(int field, int logL, int inverse, const uint8_t* in, const uint8_t* tw, const uint8_t* ninv,
+                      uint8_t* out, uint32_t* max_top) {
+    if (field == 0) ntt_lazy_pass<BlsFr>(logL, inverse, in, tw, ninv, out, max_top);
+    else ntt_lazy_pass<BnFr>(logL, inverse, in, tw, ninv, out, max_top);
+}
+
+// x mod r for x < 2^261 given as 36 LE bytes (fe_reduce_wide: x below 64r)
+void hc_reduce_wide(int field, const uint8_t* in36, uint8_t* out32) {
+    if (field == 0) reduce_wide_bytes<BlsFr>(in36, out32);
+    else reduce_wide_bytes<BnFr>(in36, out32);
 }
 
 // field: 0 BLS Fp, 1 BLS Fr, 2 BN Fp, 3 BN Fr, 4 secp256k1 Fp

@@ -25,9 +25,12 @@ def test_load_save_clear(T, tmp_path):
     assert T.msm_cell(0, 0, 16) == (11, 4) and T.window_bits(0, 70000) == 11  # secp256k1 single MSMs
     assert T.ntt_split(0, 20) == (7, 7, 6)
     assert T.ntt_split(1, 11) == (1, 10, 0)
-    assert T.ntt_split(0, 21) == (7, 7, 7)  # built-in
+    # built-in: one pass up to 2^11, then two balanced passes up to 2^20, then three
+    assert T.ntt_split(0, 21) == (7, 7, 7)
     assert T.ntt_split(0, 16) == (8, 8, 0)
-    assert T.ntt_split(1, 9) == (9, 0, 0)
+    assert T.ntt_split(1, 9) == (9, 0, 0) and T.ntt_split(0, 11) == (11, 0, 0)
+    assert T.ntt_split(1, 11) == (1, 10, 0)  # the loaded cell
+    assert T.ntt_split(1, 23) == (8, 8, 7) and T.ntt_split(0, 26) == (9, 9, 8)
     q = tmp_path / "b.tune"
     T.save(str(q))
     lines = [ln for ln in q.read_text().splitlines() if ln and not ln.startswith("#")]
@@ -55,7 +58,7 @@ def test_glv_off_cell(T, tmp_path):
 
 @pytest.mark.parametrize("line", ["msm 2 0 20 19 8", "msm 2 0 20 4 8", "msm 2 0 20 16 3", "msm 2 0 20 16 32",
                                   "msm 3 0 20 16 8", "msm 2 3 20 16 8", "msm 2 0 32 16 8", "msm 2 0 20 16",
-                                  "msm 2 0 20 16 8 1", "ntt 0 20 10 11 0", "ntt 0 20 11 9 0", "ntt 0 20 0 20 0",
+                                  "msm 2 0 20 16 8 1", "ntt 0 20 10 11 0", "ntt 0 20 12 8 0", "ntt 0 20 0 20 0",
                                   "ntt 0 12 4 0 8", "ntt 2 12 6 6 0", "ntt 0 27 9 9 9", "ntx 0 12 6 6 0",
                                   "msm two 0 20 16 8", "msm 2 0 20 1 0", "msm 2 1 20 1 0",
                                   "msm 0 1 16 10 0", "msm 0 2 16 10 0", "msm -1 0 16 10 0"])
