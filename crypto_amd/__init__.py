@@ -13,6 +13,7 @@ from .lib import (  # noqa: F401
     HipUnavailable,
     KinetError,
     KzgSrs,
+    batch_inverse,
     bls12_381_g1_msm,
     kzg_commit_batch,
     library,

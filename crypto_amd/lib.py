@@ -78,6 +78,12 @@ def library():
         lib.kinet_hip_points_decompress.argtypes = [u32, vp, sz, vp, vp, vp]
         lib.kinet_hip_g1_fixed_base_mul.argtypes = [u32, vp, sz, vp, vp]
         lib.kinet_hip_valu_peak.argtypes = [i32, ctypes.POINTER(ctypes.c_double)]
+        for name in ("gpukit_batch_inv_secp256k1_fp_hip", "gpukit_batch_inv_bn254_fp_hip",
+                     "gpukit_batch_inv_bls12_381_fp_hip"):
+            getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, sz]
+            getattr(lib, name).restype = i32
+        lib.kinet_hip_batch_inv_fp.argtypes = [u32, vp, vp, sz, vp, vp]
+        lib.kinet_hip_batch_inv_fp.restype = i32
         for name in ("ntt_fr_forward", "ntt_fr_inverse"):
             getattr(lib, name).argtypes = [u32, ctypes.c_void_p, sz]
             getattr(lib, name).restype = i32
@@ -285,6 +291,27 @@ def bls12_381_g1_msm(xycs: bytes) -> tuple[bool, bytes, bytes]:
     rx, ry = ctypes.create_string_buffer(64), ctypes.create_string_buffer(64)
     ok = lib.bls12_381_g1_msm(rx, ry, bytes(xycs), len(xycs)) == 1
     return ok, rx.raw, ry.raw
+
+
+BATCH_INV_BYTES = {0: 32, 1: 32, 2: 48}  # base-field element bytes: secp256k1, BN254, BLS12-381
+_BATCH_INV = {0: "gpukit_batch_inv_secp256k1_fp_hip", 1: "gpukit_batch_inv_bn254_fp_hip",
+              2: "gpukit_batch_inv_bls12_381_fp_hip"}
+
+
+def batch_inverse(curve: int, elems: bytes) -> bytes:
+    """Inverses of base-field elements (LE records) through
+    gpukit_batch_inv_<field>_fp_hip; KinetError (GPUKIT_ERR_BAD_SIZE) when an
+    element is zero."""
+    lib = require_device()
+    nb = BATCH_INV_BYTES[curve]
+    if len(elems) % nb:
+        raise ValueError(f"elements are {nb}-byte records")
+    out = ctypes.create_string_buffer(max(1, len(elems)))
+    fn = getattr(lib, _BATCH_INV[curve])
+    rc = fn(elems, out, len(elems) // nb)
+    if rc != GPUKIT_OK:
+        raise KinetError(fn.__name__, rc, GPUKIT_STATUS)
+    return out.raw[: len(elems)]
 
 
 def multi_pippenger(curve: int, scalars: bytes, points: bytes) -> bytes:

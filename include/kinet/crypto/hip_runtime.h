@@ -248,6 +248,18 @@ int kinet_hip_profile_collect(double* stage_ms, int* stage_calls, int nstages);
  * (its reduction is a CPU loop). */
 int kinet_hip_msm_set_reduce_segment(int seg);
 
+/* Montgomery batch inversion of device-resident base-field elements (curve
+ * 0 secp256k1, 1 BN254: 32-byte LE records; 2 BLS12-381: 48-byte LE records),
+ * asynchronous on `stream`: the device form of gpukit_batch_inv_<field>_fp_hip
+ * (kinet/gpukit/batch_inversion.h), whose results it gives.  *d_status is
+ * written when the work completes: 0, or 1 when an element is zero mod p
+ * (outputs undefined; the host form returns GPUKIT_ERR_BAD_SIZE).  d_in and d_out
+ * must not overlap and must be 4-byte aligned (GPUKIT_ERR_BAD_SIZE).  The
+ * launches allocate nothing and may be captured in a graph.  No counterpart
+ * in the reference (its GPU forms take host buffers). */
+int kinet_hip_batch_inv_fp(uint32_t curve, const void* d_in, void* d_out, size_t n, uint32_t* d_status,
+                           void* stream);
+
 /* Measured VALU throughput of the current device, in instructions per second
  * summed over all lanes: op 0 v_mad_u64_u32 (the roofline denominator of the
  * MSM and NTT kernels), 1 v_mul_lo_u32 + add, 2 v_mul_hi_u32 + xor,

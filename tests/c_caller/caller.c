@@ -10,6 +10,7 @@
 #include "kinet/crypto/kzg.h"
 #include "kinet/crypto/kzg_serve.h"
 #include "kinet/crypto/ntt.h"
+#include "kinet/gpukit/batch_inversion.h"
 #include "kinet/gpukit/gpukit.h"
 #include "kinet/gpukit/multi_pippenger.h"
 
@@ -38,6 +39,10 @@ int main(void) {
     CHECK(kinet_bn254_hip_fp_mul(buf, buf, out, 0) == 0);
     CHECK(kzg_blob_to_commit(NULL, out) == -1);
     CHECK(kzg_load_trusted_setup_file(NULL) == -1);
+    CHECK(gpukit_batch_inv_bls12_381_fp_hip(buf, out, 0) == GPUKIT_OK);
+    CHECK(gpukit_batch_inv_bn254_fp_hip(NULL, out, 1) == GPUKIT_ERR_NULL_ARG);
+    CHECK(kinet_hip_batch_inv_fp(1, buf, buf + 16, 1, NULL, NULL) == GPUKIT_ERR_NULL_ARG);
+    CHECK(kinet_hip_batch_inv_fp(1, buf, buf + 16, 1, (uint32_t*)out, buf) == GPUKIT_ERR_BAD_SIZE); /* overlap */
     printf("device available: %d\n", kinet_hip_available());
     printf("%s\n", fails ? "C caller: FAIL" : "C caller: ok");
     return fails != 0;

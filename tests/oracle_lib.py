@@ -309,6 +309,21 @@ def ref_msm(curve: int, scalars: bytes, points: bytes) -> bytes:
     return out.raw
 
 
+BATCH_INV_BYTES = {0: 32, 1: 32, 2: 48}  # secp256k1, BN254, BLS12-381 base fields
+_BATCH_INV_CPU = {0: "gpukit_batch_inv_secp256k1_fp_cpu", 1: "gpukit_batch_inv_bn254_fp_cpu",
+                  2: "gpukit_batch_inv_bls12_381_fp_cpu"}
+
+
+def ref_batch_inv(curve: int, data: bytes) -> tuple[int, bytes]:
+    """The reference's CPU batch inversion (batch_inversion.cpp): (status, out)."""
+    n = len(data) // BATCH_INV_BYTES[curve]
+    out = ctypes.create_string_buffer(b"\xa5" * max(1, len(data)), max(1, len(data)))
+    fn = getattr(ref(), _BATCH_INV_CPU[curve])
+    fn.argtypes = [_u8p, _u8p, _sz]
+    rc = fn(data, out, n)
+    return rc, out.raw[: len(data)]
+
+
 def ref_field_mul(field: int, a: bytes, b: bytes) -> bytes:
     nb = 48 if field == FIELD_BLS_FP else 32
     out = _buf(nb)

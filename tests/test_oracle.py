@@ -468,3 +468,21 @@ def test_bn254_ref_layout_helpers_against_reference_driver():
     sc = b"".join(k.to_bytes(32, "little") for k in ks)
     assert lib.kinet_bn254_cuda_g1_mul(b"".join(map(O.bn_point_ref, [G] * 6)), sc, out, 6) == 0
     assert [O.bn_point_from_ref(out.raw[72 * i:72 * i + 72]) for i in range(6)] == P
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+def test_reference_batch_inversion_does_not_invert(curve):
+    """Why the GPU batch inversion is pinned on Fermat inversion and not on
+    the reference's CPU form (oracle/_ref): that form returns GPUKIT_OK and
+    the right size, but not inverses -- 2 times its "inverse of 2" is not 1
+    mod p (its generic reduction subtracts p 2^k with no comparison,
+    batch_inversion.cpp:110-141, 263-289).  Its zero-element refusal, the part
+    of the contract it does keep, matches."""
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    p = {0: (1 << 256) - (1 << 32) - 977, 1: M.BN_P, 2: M.BLS_P}[curve]
+    nb = O.BATCH_INV_BYTES[curve]
+    rc, out = O.ref_batch_inv(curve, (2).to_bytes(nb, "little"))
+    assert rc == 0 and 2 * int.from_bytes(out, "little") % p != 1
+    assert O.ref_batch_inv(curve, (1).to_bytes(nb, "little")) == (0, (1).to_bytes(nb, "little"))
+    assert O.ref_batch_inv(curve, (7).to_bytes(nb, "little") + bytes(nb))[0] == -2
