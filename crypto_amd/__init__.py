@@ -6,6 +6,7 @@ crypto_amd/lib/libkinet_crypto_hip.so).  There is no CPU fallback: if the HIP
 library is not built or no HIP device is usable, calls raise.
 """
 from .lib import (  # noqa: F401
+    CURVE_BANDERWAGON,
     CURVE_BN254_G1,
     CURVE_BLS12_381_G1,
     FIELD_BLS12_381_FR,

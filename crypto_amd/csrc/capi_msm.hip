@@ -12,6 +12,8 @@
 #include <string>
 
 #include "engine.h"
+#include "msm_stages.h"
+#include "scan.h"
 #include "kinet/gpukit/multi_pippenger.h"
 #include "kinet/crypto/hip_runtime.h"
 
@@ -40,10 +42,26 @@ Engine& engine() {
 }
 
 // Curves of the device entries: BN254 and BLS12-381 G1 everywhere; secp256k1
-// only through the plain MSM (gpukit_multi_pippenger_hip, points_prepare,
-// msm_prepared), not through window tables or the G1 (GLV) form.
+// and Banderwagon only through the plain MSM (gpukit_multi_pippenger_hip,
+// points_prepare, msm_prepared), not through window tables or the G1 (GLV)
+// form.
 static bool valid_curve(uint32_t curve) { return curve == 1 || curve == 2; }
-static bool valid_plain_curve(uint32_t curve) { return curve <= 2; }
+static bool valid_plain_curve(uint32_t curve) { return curve <= 3; }
+
+// Banderwagon's validating reader on stream s; returns the error flags
+// (bit 0: a point the reference's deserialize_uncompressed rejects, bit 1: a
+// scalar not below the group order), synchronising s.  d_sc may be null.
+static uint32_t bw_read(Engine& e, const uint8_t* d_wire, const uint32_t* d_sc, size_t n, uint32_t* d_pts,
+                        uint8_t* d_inf, hipStream_t s) {
+    uint32_t* d_err = static_cast<uint32_t*>(e.err.ensure(sizeof(uint32_t)));
+    device_fill_u32(d_err, 0, 1, s);
+    bw_points_from_wire(d_wire, d_pts, d_inf, (uint32_t)n, d_err, s);
+    if (d_sc) bw_check_scalars(d_sc, (uint32_t)n, d_err, s);
+    uint32_t err = 0;
+    KZ_HIP(hipMemcpyAsync(&err, d_err, sizeof err, hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    return err;
+}
 
 }  // namespace kz
 
@@ -70,11 +88,11 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
                                uint8_t* result) {
     if (!result) return GPUKIT_ERR_NULL_ARG;
     if (n > 0 && (!scalars || !points)) return GPUKIT_ERR_NULL_ARG;
-    if (curve == GPUKIT_CURVE_BANDERWAGON) return GPUKIT_ERR_NOTIMPL;
     if (!valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     const size_t pb = point_wire_bytes((int)curve);
     if (n == 0) {
         std::memset(result, 0, pb);
+        if (curve == GPUKIT_CURVE_BANDERWAGON) result[63] = 1;  // the identity (0, 1)
         return GPUKIT_OK;
     }
     if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
@@ -89,7 +107,14 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
         uint8_t* d_out = static_cast<uint8_t*>(e.out.ensure(msm_out_bytes((int)curve)));
         KZ_HIP(hipMemcpyAsync(d_sc, scalars, 32 * n, hipMemcpyHostToDevice, s));
         KZ_HIP(hipMemcpyAsync(d_wire, points, pb * n, hipMemcpyHostToDevice, s));
-        points_from_wire((int)curve, d_wire, d_pts, d_inf, n, s);
+        if (curve == GPUKIT_CURVE_BANDERWAGON) {
+            // the reference rejects the whole call for any bad point or scalar
+            // (multi_pippenger.cpp:324-339)
+            if (bw_read(e, d_wire, reinterpret_cast<const uint32_t*>(d_sc), n, d_pts, d_inf, s))
+                return GPUKIT_ERR_BAD_SIZE;
+        } else {
+            points_from_wire((int)curve, d_wire, d_pts, d_inf, n, s);
+        }
         MsmParams prm;
         msm_device((int)curve, d_pts, d_inf, reinterpret_cast<const uint32_t*>(d_sc), n, prm, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
@@ -106,8 +131,15 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
     if (!valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (n > 0 && (!d_wire || !d_pts || !d_inf)) return GPUKIT_ERR_NULL_ARG;
     try {
-        (void)engine();
+        Engine& e = engine();
         hipStream_t s = static_cast<hipStream_t>(stream);
+        if (curve == GPUKIT_CURVE_BANDERWAGON) {
+            std::lock_guard<std::mutex> g(e.mu);
+            return bw_read(e, static_cast<const uint8_t*>(d_wire), nullptr, n, static_cast<uint32_t*>(d_pts),
+                           static_cast<uint8_t*>(d_inf), s)
+                       ? GPUKIT_ERR_BAD_SIZE
+                       : GPUKIT_OK;
+        }
         points_from_wire((int)curve, static_cast<const uint8_t*>(d_wire), static_cast<uint32_t*>(d_pts),
                          static_cast<uint8_t*>(d_inf), n, s);
         return GPUKIT_OK;
@@ -129,6 +161,10 @@ static int msm_prepared_impl(uint32_t curve, const void* d_scalars, const void* 
         std::lock_guard<std::mutex> g(e.mu);
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (n == 0) {
+            if (curve == GPUKIT_CURVE_BANDERWAGON) {
+                bw_identity_record(static_cast<uint8_t*>(d_out), s);
+                return GPUKIT_OK;
+            }
             KZ_HIP(hipMemsetAsync(d_out, 0, msm_out_bytes((int)curve), s));
             uint8_t* d_zero = static_cast<uint8_t*>(d_out);
             compress_on_device((int)curve, d_zero, d_zero + point_wire_bytes((int)curve), s);

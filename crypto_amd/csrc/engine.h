@@ -17,8 +17,9 @@ struct Engine {
     std::mutex mu;
     MsmWorkspace ws;
     DevBuf h2d_a, h2d_b, pts, inf, out;
+    DevBuf err;  // validation flags of the Banderwagon reader
     DevBuf ntt_a, ntt_b, ntt_tw;
-    DevBuf gen_table[3];  // fixed-base tables, indexed by curve id
+    DevBuf gen_table[4];  // fixed-base tables, indexed by curve id
 };
 
 // Returns the engine, initialising HIP on first use; throws HipError when no

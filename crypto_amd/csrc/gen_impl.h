@@ -8,10 +8,11 @@
 //   k_gen_norm     one thread per entry: affine normalisation
 // then k_gen_mul adds T[j][byte_j(k)] over the 32 bytes of each scalar (any
 // 256-bit k: the bytes reconstruct k exactly) and writes the affine point in
-// the wire format (BE x || y, identity = zero bytes).
+// the wire format (BE x || y, identity = zero bytes).  The point operations
+// are the curve's (point_ops.h): XYZZ, or extended Edwards for Banderwagon.
 #pragma once
 
-#include "curve.h"
+#include "point_ops.h"
 
 namespace kz {
 
@@ -19,7 +20,7 @@ constexpr int kGenWindows = 32;
 constexpr int kGenDigits = 256;
 
 template <class C>
-__device__ Affine<typename C::Fp> generator_affine() {
+__device__ typename PointOps<typename C::Fp>::Aff generator_affine() {
     using F = typename C::Fp;
     // BLS12-381 G1 generator: bls/cpp/bls12_381_g1.hpp:204-224; BN254: (1, 2)
     // (gpukit/test/multi_pippenger_test.cpp:207-211); secp256k1: SEC 2's G
@@ -38,61 +39,76 @@ __device__ Affine<typename C::Fp> generator_affine() {
     constexpr uint8_t secp_y[32] = {
         0x48, 0x3A, 0xDA, 0x77, 0x26, 0xA3, 0xC4, 0x65, 0x5D, 0xA4, 0xFB, 0xFC, 0x0E, 0x11, 0x08, 0xA8,
         0xFD, 0x17, 0xB4, 0x48, 0xA6, 0x85, 0x54, 0x19, 0x9C, 0x47, 0xD0, 0x8F, 0xFB, 0x10, 0xD4, 0xB8};
-    Affine<F> g;
-    if constexpr (C::ID == 2) {
-        g.x = fe_to_mont<F>(fe_from_be_bytes<F>(bls_x, 48));
-        g.y = fe_to_mont<F>(fe_from_be_bytes<F>(bls_y, 48));
-    } else if constexpr (C::ID == 0) {
-        g.x = fe_to_mont<F>(fe_from_be_bytes<F>(secp_x, 32));
-        g.y = fe_to_mont<F>(fe_from_be_bytes<F>(secp_y, 32));
+    // Banderwagon: the Bandersnatch base point of banderwagon/cpp/element.cpp:48-75
+    constexpr uint8_t bw_x[32] = {
+        0x29, 0xC1, 0x32, 0xCC, 0x2C, 0x0B, 0x34, 0xC5, 0x74, 0x37, 0x11, 0x77, 0x7B, 0xBE, 0x42, 0xF3,
+        0x2B, 0x79, 0xC0, 0x22, 0xAD, 0x99, 0x84, 0x65, 0xE1, 0xE7, 0x18, 0x66, 0xA2, 0x52, 0xAE, 0x18};
+    constexpr uint8_t bw_y[32] = {
+        0x2A, 0x6C, 0x66, 0x9E, 0xDA, 0x12, 0x3E, 0x0F, 0x15, 0x7D, 0x8B, 0x50, 0xBA, 0xDC, 0xD5, 0x86,
+        0x35, 0x8C, 0xAD, 0x81, 0xEE, 0xE4, 0x64, 0x60, 0x5E, 0x31, 0x67, 0xB6, 0xCC, 0x97, 0x41, 0x66};
+    if constexpr (C::ID == 3) {
+        EdAff<F> g;
+        g.x = fe_to_mont<F>(fe_from_be_bytes<F>(bw_x, 32));
+        g.y = fe_to_mont<F>(fe_from_be_bytes<F>(bw_y, 32));
+        g.t = fe_mul<F>(fe_mul<F>(g.x, g.y), fe_const<F>(EdConst::D));
+        return g;
     } else {
-        Fe<F> x = fe_zero<F>(), y = fe_zero<F>();
-        x.v[0] = 1;
-        y.v[0] = 2;
-        g.x = fe_to_mont<F>(x);
-        g.y = fe_to_mont<F>(y);
+        Affine<F> g;
+        if constexpr (C::ID == 2) {
+            g.x = fe_to_mont<F>(fe_from_be_bytes<F>(bls_x, 48));
+            g.y = fe_to_mont<F>(fe_from_be_bytes<F>(bls_y, 48));
+        } else if constexpr (C::ID == 0) {
+            g.x = fe_to_mont<F>(fe_from_be_bytes<F>(secp_x, 32));
+            g.y = fe_to_mont<F>(fe_from_be_bytes<F>(secp_y, 32));
+        } else {
+            Fe<F> x = fe_zero<F>(), y = fe_zero<F>();
+            x.v[0] = 1;
+            y.v[0] = 2;
+            g.x = fe_to_mont<F>(x);
+            g.y = fe_to_mont<F>(y);
+        }
+        return g;
     }
-    return g;
 }
 
 template <class C>
 __global__ void k_gen_bases(uint32_t* __restrict__ bases) {
-    using F = typename C::Fp;
+    using P = PointOps<typename C::Fp>;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    Xyzz<F> b = xyzz_from_affine<F>(generator_affine<C>());
+    typename P::Pt b = P::from_aff(generator_affine<C>());
 #pragma unroll 1
     for (int j = 0; j < kGenWindows; ++j) {
-        xyzz_store<F>(bases, j, b);
+        P::store(bases, j, b);
 #pragma unroll 1
-        for (int i = 0; i < 8; ++i) b = xyzz_dbl<F, MulCall>(b);
+        for (int i = 0; i < 8; ++i) b = P::template dbl<MulCall>(b);
     }
 }
 
 template <class C>
 __global__ void __launch_bounds__(64) k_gen_table(const uint32_t* __restrict__ bases, uint32_t* __restrict__ table) {
-    using F = typename C::Fp;
+    using P = PointOps<typename C::Fp>;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // t = j * 256 + d
     if (t >= kGenWindows * kGenDigits) return;
     const uint32_t j = t >> 8, d = t & 255;
-    const Xyzz<F> b = xyzz_load<F>(bases, j);
-    Xyzz<F> acc = xyzz_inf<F>();
+    const typename P::Pt b = P::load(bases, j);
+    typename P::Pt acc = P::inf();
 #pragma unroll 1
     for (int bit = 7; bit >= 0; --bit) {
-        acc = xyzz_dbl<F, MulCall>(acc);
-        if ((d >> bit) & 1u) acc = xyzz_add<F, MulCall>(acc, b);
+        acc = P::template dbl<MulCall>(acc);
+        if ((d >> bit) & 1u) acc = P::template add<MulCall>(acc, b);
     }
-    xyzz_store<F>(table, t, acc);
+    P::store(table, t, acc);
 }
 
 template <class C>
 __global__ void __launch_bounds__(64) k_gen_norm(const uint32_t* __restrict__ table, uint32_t* __restrict__ aff) {
-    using F = typename C::Fp;
+    using P = PointOps<typename C::Fp>;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= kGenWindows * kGenDigits) return;
-    Affine<F> a;
+    typename P::Aff a;
     bool inf;
-    xyzz_to_affine<F>(xyzz_load<F>(table, t), a, inf);  // d = 0 entries are infinity, never read
-    affine_store<F>(aff, t, a);
+    P::to_affine(P::load(table, t), a, inf);  // d = 0 entries are the identity, never read
+    P::store_aff(aff, t, a);
 }
 
 template <class C>
@@ -100,18 +116,19 @@ __global__ void __launch_bounds__(64)
 k_gen_mul(const uint32_t* __restrict__ aff_table, const uint32_t* __restrict__ scalars, uint32_t n,
           uint8_t* __restrict__ out_wire) {
     using F = typename C::Fp;
+    using P = PointOps<F>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t* s = scalars + (size_t)i * 8;
-    Xyzz<F> acc = xyzz_inf<F>();
+    typename P::Pt acc = P::inf();
 #pragma unroll 1
     for (int j = 0; j < kGenWindows; ++j) {
         const uint32_t d = (s[j >> 2] >> (8 * (j & 3))) & 255u;  // cached global read, no register indexing
-        if (d) xyzz_madd<F, MulCall>(acc, affine_load<F>(aff_table, (uint32_t)j * kGenDigits + d));
+        if (d) P::template madd<MulCall>(acc, P::load_aff(aff_table, (uint32_t)j * kGenDigits + d));
     }
-    Affine<F> a;
+    typename P::Aff a;
     bool inf;
-    xyzz_to_affine<F>(acc, a, inf);
+    P::to_affine(acc, a, inf);
     uint8_t* w = out_wire + (size_t)i * 2 * C::FB;
     if (inf) {
         for (int k = 0; k < 2 * C::FB; ++k) w[k] = 0;
@@ -125,9 +142,8 @@ k_gen_mul(const uint32_t* __restrict__ aff_table, const uint32_t* __restrict__ s
 // using `scratch` (kGenWindows + kGenWindows * kGenDigits XYZZ records).
 template <class C>
 void gen_build_table(uint32_t* scratch, uint32_t* aff_table, hipStream_t s) {
-    using F = typename C::Fp;
     uint32_t* bases = scratch;
-    uint32_t* table = scratch + (size_t)kGenWindows * Rec<F>::kXyzzWords;
+    uint32_t* table = scratch + (size_t)kGenWindows * PointOps<typename C::Fp>::kPtWords;
     const unsigned nt = kGenWindows * kGenDigits;
     hipLaunchKernelGGL(k_gen_bases<C>, dim3(1), dim3(64), 0, s, bases);
     hipLaunchKernelGGL(k_gen_table<C>, dim3(nt / 64), dim3(64), 0, s, bases, table);

@@ -130,10 +130,11 @@ inline size_t msm_out_bytes(int curve) { return curve == 2 ? 96 + 48 : 64 + 32; 
 inline size_t point_wire_bytes(int curve) { return curve == 2 ? 96 : 64; }
 inline size_t compressed_bytes(int curve) { return curve == 2 ? 48 : 32; }
 // Device record sizes in 32-bit words (radix-2^29 limbs, padded to 16 bytes):
-// affine points 28 words of data in a 32-word (128-byte) slot for BLS12-381,
-// 20 for BN254 and secp256k1; XYZZ points 56 / 36 / 40.
-inline int affine_words(int curve) { return curve == 2 ? 32 : 20; }  // record stride
-inline int xyzz_words(int curve) { return curve == 2 ? 56 : curve == 1 ? 36 : 40; }
+// affine points 28 words of data in a 32-word (128-byte) slot for BLS12-381
+// and for Banderwagon (x, y, d x y), 20 for BN254 and secp256k1; projective
+// points: XYZZ 56 / 36 / 40, Banderwagon's extended (X, Y, T, Z) 36.
+inline int affine_words(int curve) { return curve == 2 || curve == 3 ? 32 : 20; }  // record stride
+inline int xyzz_words(int curve) { return curve == 2 ? 56 : curve == 1 || curve == 3 ? 36 : 40; }
 
 // Wire affine points (BE, zero = identity) -> Montgomery affine (2N words each)
 // plus per-point identity flags.

@@ -232,7 +232,13 @@ const Stages kBn = {launch_accumulate_bn, launch_fold_bn,    launch_reduce_seg_b
 const Stages kSecp = {launch_accumulate_secp, launch_fold_secp,    launch_reduce_seg_secp,
                       launch_reduce_level_secp, launch_combine_secp, launch_table_rows_secp, 40};
 
-const Stages& stages_of(int curve) { return curve == 2 ? kBls : curve == 1 ? kBn : kSecp; }
+// Banderwagon: extended Edwards points (36 words); no window tables.
+const Stages kBw = {launch_accumulate_bw, launch_fold_bw,    launch_reduce_seg_bw,
+                    launch_reduce_level_bw, launch_combine_bw, nullptr, 36};
+
+const Stages& stages_of(int curve) {
+    return curve == 2 ? kBls : curve == 1 ? kBn : curve == 3 ? kBw : kSecp;
+}
 
 // Sizes of one pipeline run, single MSM or batch.
 struct Shape {
@@ -651,6 +657,7 @@ void msm_batch_run(const MsmBatchPlan& plan, const uint32_t* d_pts, const uint8_
 void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t* d_inf, size_t n,
                       hipStream_t s) {
     if (n == 0) return;
+    if (curve == 3) throw std::invalid_argument("Banderwagon points are read by the validating reader");
     if (curve == 2) launch_points_from_wire_bls(d_wire, d_pts, d_inf, (uint32_t)n, s);
     else if (curve == 1) launch_points_from_wire_bn(d_wire, d_pts, d_inf, (uint32_t)n, s);
     else launch_points_from_wire_secp(d_wire, d_pts, d_inf, (uint32_t)n, s);

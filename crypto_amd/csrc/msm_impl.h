@@ -25,8 +25,8 @@
 // few point-operation sites with `#pragma unroll 1` loops.
 #pragma once
 
-#include "curve_quad.h"
 #include "msm_internal.h"
+#include "point_ops.h"
 
 namespace kz {
 
@@ -64,10 +64,11 @@ k_points_from_wire(const uint8_t* __restrict__ wire, uint32_t* __restrict__ pts,
 
 // ------------------------------------------------------------ accumulation
 template <class F>
-__device__ __forceinline__ Affine<F> load_signed_point(const uint32_t* __restrict__ pts, uint32_t entry) {
-    Affine<F> q = affine_load<F>(pts, entry & 0x7FFFFFFFu);
-    if (entry >> 31) q.y = fe_neg<F>(q.y);
-    return q;
+__device__ __forceinline__ typename PointOps<F>::Aff load_signed_point(const uint32_t* __restrict__ pts,
+                                                                      uint32_t entry) {
+    using P = PointOps<F>;
+    const typename P::Aff q = P::load_aff(pts, entry & 0x7FFFFFFFu);
+    return (entry >> 31) ? P::neg_aff(q) : q;
 }
 
 // Largest g with offsets[g] <= pos < offsets[g + 1] (pos < offsets[nb]).
@@ -86,6 +87,7 @@ __global__ void __launch_bounds__(64, 3)
 k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets, uint32_t nb,
              uint32_t chunk, const uint32_t* __restrict__ pts, uint32_t* __restrict__ bsum,
              uint32_t* __restrict__ carry) {
+    using P = PointOps<F>;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t total = offsets[nb];
     const uint32_t start = j * chunk;
@@ -94,24 +96,24 @@ k_accumulate(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ 
     uint32_t g = bucket_of(offsets, nb, start);
     uint32_t next = offsets[g + 1];
     bool is_carry = offsets[g] < start;
-    Xyzz<F> acc = xyzz_from_affine<F>(load_signed_point<F>(pts, entries[start]));
+    typename P::Pt acc = P::from_aff(load_signed_point<F>(pts, entries[start]));
     for (uint32_t e = start + 1; e < end; ++e) {
-        const Affine<F> q = load_signed_point<F>(pts, entries[e]);
+        const typename P::Aff q = load_signed_point<F>(pts, entries[e]);
         if (e == next) {
-            if (is_carry) xyzz_store<F>(carry, j, xyzz_canon_acc<F>(acc));
-            else xyzz_store<F>(bsum, g, xyzz_canon_acc<F>(acc));
+            if (is_carry) P::store(carry, j, P::canon(acc));
+            else P::store(bsum, g, P::canon(acc));
             do {
                 ++g;
                 next = offsets[g + 1];
             } while (next == e);
             is_carry = false;
-            acc = xyzz_from_affine<F>(q);
+            acc = P::from_aff(q);
         } else {
-            xyzz_madd_lazy<F>(acc, q);  // coordinates below 8p / 4p / 2p until stored
+            P::madd_acc(acc, q);
         }
     }
-    if (is_carry) xyzz_store<F>(carry, j, xyzz_canon_acc<F>(acc));
-    else xyzz_store<F>(bsum, g, xyzz_canon_acc<F>(acc));
+    if (is_carry) P::store(carry, j, P::canon(acc));
+    else P::store(bsum, g, P::canon(acc));
 }
 
 // ------------------------------------------------------------ carry fold
@@ -147,6 +149,7 @@ template <class F>
 __global__ void __launch_bounds__(256)
 k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, uint2* __restrict__ runs,
              uint32_t* __restrict__ ctl, uint32_t* __restrict__ carry) {
+    using P = PointOps<F>;
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (g >= nb) return;  // whole quads leave together
     const uint32_t lo = offsets[g], hi = offsets[g + 1];
@@ -154,10 +157,10 @@ k_carry_runs(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t chunk, 
     const uint32_t head = lo / chunk + 1, len = (hi - 1) / chunk + 1 - head;
     if (len <= 1) return;  // no carry, or one already in place
     if (len <= kShortRun) {
-        Xyzz<F> acc = xyzz_load<F>(carry, head);
+        typename P::Pt acc = P::load(carry, head);
 #pragma unroll 1
-        for (uint32_t k = 1; k < len; ++k) acc = xyzz_add_quad<F, MulCallLazy>(acc, xyzz_load<F>(carry, head + k));
-        if (quad_lane() == 0) xyzz_store<F>(carry, head, xyzz_canon_acc<F>(acc));
+        for (uint32_t k = 1; k < len; ++k) acc = P::template add_q<MulCallLazy>(acc, P::load(carry, head + k));
+        if (quad_lane() == 0) P::store(carry, head, P::canon(acc));
         return;
     }
     if (quad_lane() == 0) {
@@ -187,7 +190,8 @@ template <class F>
 __global__ void __launch_bounds__(4 * kFoldFan)
 k_fold_groups(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, uint32_t stride,
               uint32_t* __restrict__ carry) {
-    __shared__ Xyzz<F> sh[kFoldFan];
+    using P = PointOps<F>;
+    __shared__ typename P::Pt sh[kFoldFan];
     if (stride >= ctl[1]) return;  // every run is one element at this level
     const uint32_t count = ctl[0], q = threadIdx.x >> 2;
     const uint64_t span = (uint64_t)stride * kFoldFan;
@@ -196,15 +200,15 @@ k_fold_groups(const uint4* __restrict__ list, const uint32_t* __restrict__ ctl, 
         const uint4 en = list[i];                                 // {chunk j, run position r, run length}
         if (en.y % span != 0) continue;                           // not a group leader at this level
         const uint64_t pos = en.y + (uint64_t)q * stride;
-        Xyzz<F> p = pos < en.z ? xyzz_load<F>(carry, en.x + (size_t)q * stride) : xyzz_inf<F>();
+        typename P::Pt p = pos < en.z ? P::load(carry, en.x + (size_t)q * stride) : P::inf();
 #pragma unroll 1
         for (uint32_t st = 1; st < kFoldFan; st <<= 1) {
             __syncthreads();  // the previous step's readers of sh are done
             if ((q & (2 * st - 1)) == st && quad_lane() == 0) sh[q] = p;
             __syncthreads();
-            if ((q & (2 * st - 1)) == 0) p = xyzz_add_quad<F, MulCallLazy>(p, sh[q + st]);
+            if ((q & (2 * st - 1)) == 0) p = P::template add_q<MulCallLazy>(p, sh[q + st]);
         }
-        if (q == 0 && quad_lane() == 0) xyzz_store<F>(carry, en.x, xyzz_canon_acc<F>(p));
+        if (q == 0 && quad_lane() == 0) P::store(carry, en.x, P::canon(p));
     }
 }
 
@@ -243,25 +247,26 @@ __global__ void __launch_bounds__(256, 2)
 k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_t* __restrict__ bsum,
              const uint32_t* __restrict__ carry, uint32_t nseg, uint32_t seg, uint32_t* __restrict__ segA,
              uint32_t* __restrict__ segB) {
+    using P = PointOps<F>;
     const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (t >= nseg) return;  // whole quads leave together
-    Xyzz<F> run = xyzz_inf<F>(), tot = xyzz_inf<F>();
+    typename P::Pt run = P::inf(), tot = P::inf();
 #pragma unroll 1
     for (int l = (int)seg - 1; l >= 0; --l) {
         const uint32_t gb = t * seg + (uint32_t)l;
         const uint32_t off = offsets[gb], end = offsets[gb + 1];
-        Xyzz<F> S = xyzz_inf<F>();
+        typename P::Pt S = P::inf();
         if (end > off) {
-            S = xyzz_load<F>(bsum, gb);
+            S = P::load(bsum, gb);
             const uint32_t head = off / chunk + 1;
-            if ((end - 1) / chunk >= head) S = xyzz_add_quad<F, MulCallLazy>(S, xyzz_load<F>(carry, head));
+            if ((end - 1) / chunk >= head) S = P::template add_q<MulCallLazy>(S, P::load(carry, head));
         }
-        run = xyzz_add_quad<F, MulCallLazy>(run, S);
-        tot = xyzz_add_quad<F, MulCallLazy>(tot, run);
+        run = P::template add_q<MulCallLazy>(run, S);
+        tot = P::template add_q<MulCallLazy>(tot, run);
     }
     if (quad_lane() == 0) {
-        xyzz_store<F>(segA, t, xyzz_canon_acc<F>(run));
-        xyzz_store<F>(segB, t, xyzz_canon_acc<F>(tot));
+        P::store(segA, t, P::canon(run));
+        P::store(segB, t, P::canon(tot));
     }
 }
 
@@ -273,7 +278,8 @@ __global__ void __launch_bounds__(kLevelThreads)
 k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ inB,
                const RedLevel* __restrict__ tab, uint32_t ntab, uint32_t uni_n, uint32_t uni_parts, uint32_t lm,
                uint32_t* __restrict__ outA, uint32_t* __restrict__ outB) {
-    __shared__ Xyzz<F> sh[kLevelQuads];
+    using P = PointOps<F>;
+    __shared__ typename P::Pt sh[kLevelQuads];
     uint32_t in_base, n_in, out_idx, g;
     if (tab) {
         uint32_t lo = 0, hi = ntab;
@@ -301,42 +307,42 @@ k_reduce_level(const uint32_t* __restrict__ inA, const uint32_t* __restrict__ in
     while (span < cnt) span <<= 1;
     const bool lead = quad_lane() == 0;
 
-    Xyzz<F> R = xyzz_inf<F>(), term = xyzz_inf<F>();
+    typename P::Pt R = P::inf(), term = P::inf();
     if (k < cnt) {
-        R = xyzz_load<F>(inA, (size_t)in_base + first + k);
-        term = xyzz_load<F>(inB, (size_t)in_base + first + k);
+        R = P::load(inA, (size_t)in_base + first + k);
+        term = P::load(inB, (size_t)in_base + first + k);
     }
     // suffix scan R_k = sum_{j >= k} A_j over the part
     if (lead) sh[k] = R;
     __syncthreads();
 #pragma unroll 1
     for (uint32_t s = 1; s < span; s <<= 1) {
-        const Xyzz<F> other = (k + s < span) ? sh[k + s] : xyzz_inf<F>();
+        const typename P::Pt other = (k + s < span) ? sh[k + s] : P::inf();
         __syncthreads();
-        if (k < span) R = xyzz_add_quad<F, MulCallLazy>(R, other);
+        if (k < span) R = P::template add_q<MulCallLazy>(R, other);
         if (lead) sh[k] = R;
         __syncthreads();
     }
     // term_k = B_k + m R_k (k > 0)
     if (k > 0 && k < span) {
-        Xyzz<F> mR = R;
+        typename P::Pt mR = R;
 #pragma unroll 1
-        for (uint32_t j = 0; j < lm; ++j) mR = xyzz_dbl_quad<F, MulCallLazy>(mR);
-        term = xyzz_add_quad<F, MulCallLazy>(term, mR);
+        for (uint32_t j = 0; j < lm; ++j) mR = P::template dbl_q<MulCallLazy>(mR);
+        term = P::template add_q<MulCallLazy>(term, mR);
     }
     if (lead) sh[k] = term;
     __syncthreads();
 #pragma unroll 1
     for (uint32_t s = span >> 1; s > 0; s >>= 1) {
         if (k < s) {
-            const Xyzz<F> v = xyzz_add_quad<F, MulCallLazy>(sh[k], sh[k + s]);
+            const typename P::Pt v = P::template add_q<MulCallLazy>(sh[k], sh[k + s]);
             if (lead) sh[k] = v;
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        xyzz_store<F>(outA, out_idx, xyzz_canon_acc<F>(R));
-        xyzz_store<F>(outB, out_idx, xyzz_canon_acc<F>(sh[0]));
+        P::store(outA, out_idx, P::canon(R));
+        P::store(outB, out_idx, P::canon(sh[0]));
     }
 }
 
@@ -377,6 +383,28 @@ __device__ void write_output(const Xyzz<typename C::Fp>& r, uint8_t* __restrict_
     write_compressed<C>(x, y, out + 2 * C::FB);
 }
 
+// Banderwagon record: affine x || y (the identity as (0, 1), as the
+// reference's serialize_uncompressed writes it, element.cpp:286-300), then the
+// 32-byte compressed form of serialize_compressed (element.cpp:266-284): x,
+// negated unless y > (q - 1) / 2.
+template <class C>
+__device__ void write_output(const EdExt<typename C::Fp>& r, uint8_t* __restrict__ out) {
+    using F = typename C::Fp;
+    EdAff<F> a;
+    bool inf;
+    ed_to_affine<F>(r, a, inf);
+    Fe<F> x = fe_zero<F>(), y = fe_zero<F>();
+    y.v[0] = 1;
+    if (!inf) {
+        x = fe_from_mont<F>(a.x);
+        y = fe_from_mont<F>(a.y);
+    }
+    fe_to_be_bytes<F>(x, out, C::FB);
+    fe_to_be_bytes<F>(y, out + C::FB, C::FB);
+    const bool largest = fe_gt<F>(y, fe_neg<F>(y));  // y > q - y, i.e. y > (q - 1) / 2
+    fe_to_be_bytes<F>(largest ? x : fe_neg<F>(x), out + 2 * C::FB, C::FB);
+}
+
 // Horner over the window sums, MSW first: r = 2^c r + S_w.  One quad of lanes
 // per MSM (quad-cooperative point operations: the chain of ~256 doublings is
 // the critical path of the whole MSM).  jobs = null: a single MSM described
@@ -385,21 +413,21 @@ template <class C>
 __global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ winres,
                                                 const MsmJob* __restrict__ jobs, MsmJob one, uint32_t njobs,
                                                 uint8_t* __restrict__ out) {
-    using F = typename C::Fp;
+    using P = PointOps<typename C::Fp>;
     const uint32_t job = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (job >= njobs) return;
     const MsmJob jd = jobs ? jobs[job] : one;
     const int c = (int)jd.c;
-    Xyzz<F> r = xyzz_load<F>(winres, jd.win_base + jd.nwin - 1);
+    typename P::Pt r = P::load(winres, jd.win_base + jd.nwin - 1);
 #pragma unroll 1
     for (int w = (int)jd.nwin - 2; w >= 0; --w) {
-        if (!xyzz_is_inf<F>(r)) {
+        if (!P::is_inf(r)) {
 #pragma unroll 1
-            for (int i = 0; i < c; ++i) r = xyzz_dbl_quad<F, MulLazy>(r);
+            for (int i = 0; i < c; ++i) r = P::template dbl_q<MulLazy>(r);
         }
-        r = xyzz_add_quad<F, MulLazy>(r, xyzz_load<F>(winres, jd.win_base + (uint32_t)w));
+        r = P::template add_q<MulLazy>(r, P::load(winres, jd.win_base + (uint32_t)w));
     }
-    if (quad_lane() == 0) write_output<C>(xyzz_canon_acc<F>(r), out + (size_t)job * 3 * C::FB);
+    if (quad_lane() == 0) write_output<C>(P::canon(r), out + (size_t)job * 3 * C::FB);
 }
 
 // ------------------------------------------------------------ window tables

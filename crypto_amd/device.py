@@ -12,7 +12,7 @@ from .lib import GPUKIT_OK, GPUKIT_STATUS, KinetError, point_bytes, require_devi
 
 def affine_words(curve: int) -> int:
     """Words per prepared (device-form) affine point record."""
-    return 32 if curve == 2 else 20
+    return 32 if curve in (2, 3) else 20
 
 
 def stream_handle(stream=None) -> int:

@@ -22,6 +22,7 @@ _LIB_PATH = os.path.join(_HERE, "lib", "libkinet_crypto_hip.so")
 
 CURVE_BN254_G1 = 1
 CURVE_BLS12_381_G1 = 2
+CURVE_BANDERWAGON = 3
 FIELD_BLS12_381_FR = 0
 FIELD_BN254_FR = 1
 
@@ -316,7 +317,9 @@ def batch_inverse(curve: int, elems: bytes) -> bytes:
 
 def multi_pippenger(curve: int, scalars: bytes, points: bytes) -> bytes:
     """sum s_i P_i through gpukit_multi_pippenger_hip (the reference's wire
-    format: 32-byte LE scalars, BE affine points, all-zero = identity)."""
+    format: 32-byte LE scalars, BE affine points, all-zero = identity; for
+    Banderwagon the result's identity is (0, 1) and bad inputs raise with
+    GPUKIT_ERR_BAD_SIZE)."""
     lib = require_device()
     n = len(scalars) // 32
     if len(scalars) != 32 * n or len(points) != point_bytes(curve) * n:

@@ -28,10 +28,13 @@ int kinet_hip_available(void);
 const char* kinet_hip_last_error(void);
 
 /* Wire points (BE affine, zero = identity) -> device form: Montgomery affine
- * records of 32 (BLS12-381: 28 words of radix-2^29 limbs in a 128-byte slot)
- * or 20 (BN254, secp256k1) 32-bit words per point, 16-byte aligned, and a byte
- * per point flagging the identity.  d_pts must hold kinet_hip_points_bytes(curve,
- * n) bytes and d_inf n bytes. */
+ * records of 32 (BLS12-381: 28 words of radix-2^29 limbs in a 128-byte slot;
+ * Banderwagon: x, y and d x y, 27 words in a 128-byte slot) or 20 (BN254,
+ * secp256k1) 32-bit words per point, 16-byte aligned, and a byte per point
+ * flagging the identity.  d_pts must hold kinet_hip_points_bytes(curve, n)
+ * bytes and d_inf n bytes.  Banderwagon (curve 3) points are validated as
+ * gpukit_multi_pippenger_hip validates them: this call then synchronises the
+ * stream and returns GPUKIT_ERR_BAD_SIZE if any point is rejected. */
 int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void* d_pts, void* d_inf,
                              void* stream);
 
@@ -43,7 +46,10 @@ size_t kinet_hip_points_bytes(uint32_t curve, size_t n);
 /* MSM over prepared points.  d_scalars: n x 32 bytes LE.  window_bits: 0 picks
  * from n, else 5..18.  chunk: sorted bucket entries per accumulation thread,
  * 0 for the default.  Writes the affine wire result (64/96 bytes) followed by
- * its compressed form (32/48 bytes) at d_out. */
+ * its compressed form (32/48 bytes) at d_out.  Curves 0-3; for Banderwagon the
+ * scalars are used as 256-bit integers (the host entry checks that they are
+ * below the group order), the identity is written as (0, 1), and the
+ * compressed form is the reference's serialize_compressed. */
 int kinet_hip_msm_prepared(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                            size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
@@ -230,7 +236,9 @@ int kinet_hip_tune_get_ntt(uint32_t field, int log_n, int* p1, int* p2, int* p3)
  * (32-byte LE, any 256-bit value) on the device, written in the point wire
  * format (BE affine x || y, identity = zero bytes).  The table of multiples of
  * G is built on first use per curve and stays resident.  Synthetic MSM inputs
- * and SRS points [tau^i] G come from here.  gpukit_status codes. */
+ * and SRS points [tau^i] G come from here.  Curves 0-3 (Banderwagon: the
+ * Bandersnatch base point of banderwagon/cpp/element.cpp:48-75).
+ * gpukit_status codes. */
 int kinet_hip_g1_fixed_base_mul(uint32_t curve, const void* d_scalars, size_t n, void* d_out_wire, void* stream);
 
 /* Opt-in stage profiler of the MSM pipeline: while enabled, each MSM records

@@ -17,6 +17,16 @@
  * scalars >= 2^255 this backend keeps the signed-digit carry that the
  * reference drops out of its last window (multi_pippenger.cpp:118-126).
  *
+ * Banderwagon (multi_pippenger.cpp:314-350): points are the 64-byte
+ * uncompressed form (affine x || y on the Bandersnatch curve, big-endian) and
+ * are checked as Element::deserialize_uncompressed does (banderwagon/cpp/
+ * element.cpp:330-355: x, y below q, on the curve, 1 - a x^2 a square);
+ * all-zero bytes are the identity; scalars must be canonical, below the group
+ * order (fr.cpp:305-328).  Any rejected point or scalar fails the whole call
+ * with GPUKIT_ERR_BAD_SIZE, as in the reference.  The result is the exact sum
+ * of the given representatives (two-torsion component included) written as
+ * affine x || y; the identity, and n == 0, give (0, 1).
+ *
  * Return codes are gpukit_status (gpukit.h).
  */
 #pragma once
@@ -33,7 +43,7 @@ typedef enum gpukit_curve {
     GPUKIT_CURVE_SECP256K1 = 0,   /* 64-byte BE x || y points, like BN254 */
     GPUKIT_CURVE_BN254_G1 = 1,
     GPUKIT_CURVE_BLS12_381_G1 = 2,
-    GPUKIT_CURVE_BANDERWAGON = 3, /* not provided by the HIP backend: GPUKIT_ERR_NOTIMPL */
+    GPUKIT_CURVE_BANDERWAGON = 3, /* 64-byte BE x || y points, validated (see below) */
 } gpukit_curve;
 
 /* The HIP backend, a sibling of gpukit_multi_pippenger_{cpu,cuda,metal,wgsl}
@@ -47,9 +57,8 @@ typedef enum gpukit_curve {
  * n == 0 gives the identity.  n >= 2^26 gives GPUKIT_ERR_BAD_SIZE: sorted
  * bucket entries carry 31-bit indices and an MSM has up to 22 windows per
  * point (2^26 BLS12-381 points are 6.4 GB of wire input; the reference's CPU
- * path has no cap).  Banderwagon gives GPUKIT_ERR_NOTIMPL (the dispatcher's
- * CPU fallback then serves it); device failures give GPUKIT_ERR_BACKEND
- * (message via kinet_hip_last_error()). */
+ * path has no cap).  Device failures give GPUKIT_ERR_BACKEND (message via
+ * kinet_hip_last_error()). */
 int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
                                uint8_t* result);
 

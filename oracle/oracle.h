@@ -48,6 +48,7 @@ extern "C" {
 #define ORC_FIELD_BLS_FR 1
 #define ORC_FIELD_BN_FP  2
 #define ORC_FIELD_BN_FR  3
+#define ORC_FIELD_BW_FR  4   /* Banderwagon scalar field (banderwagon/cpp/fr.cpp:11-16) */
 
 /* ---- field probes (canonical little-endian bytes in and out) ---- */
 int  orc_field_nbytes(int field);
@@ -95,7 +96,7 @@ int  orc_kzg_commit(int curve, const uint8_t* srs_wire, const uint8_t* evals, si
 /* closed form: compress([f(tau)] G) with f = iNTT(evals), Horner at tau. */
 int  orc_kzg_commit_closed_form(int curve, const uint8_t* tau_le, const uint8_t* evals, size_t n, uint8_t* out_compressed);
 
-/* ---- O(n) closed forms over Fr (fr: 0 = BLS12-381, 1 = BN254) ---- */
+/* ---- O(n) closed forms over Fr (fr: 0 = BLS12-381, 1 = BN254; 2 = Banderwagon, dot only) ---- */
 /* sum a_i b_i mod r (a_i, b_i any 256-bit LE values). */
 int  orc_fr_dot(int fr, const uint8_t* a, const uint8_t* b, size_t n, uint8_t* out_le);
 /* sum c_i x^i (canonical inputs; -1 otherwise). */
@@ -103,6 +104,34 @@ int  orc_fr_eval(int fr, const uint8_t* coeffs, size_t n, const uint8_t* x_le, u
 /* f(tau) from f(w^j), j < n, by the barycentric formula; -1 if tau is in the
  * domain, n is not a supported power of two or an input is not canonical. */
 int  orc_fr_lagrange(int fr, const uint8_t* evals, size_t n, const uint8_t* tau_le, uint8_t* out_le);
+
+/* ---- Banderwagon (curve 3 of the multi-curve entry) ----
+ * Twisted Edwards a x^2 + y^2 = 1 + d x^2 y^2 over BLS12-381 Fr, a = -5
+ * (banderwagon/cpp/element.hpp:6-22), projective (X : Y : Z) with the
+ * reference's add-2008-bbjlp / dbl-2008-bbjlp (element.cpp:145-209).  Points on
+ * the wire: 64 bytes, affine x || y big-endian (serialize_uncompressed,
+ * element.cpp:286-300); all-zero bytes = identity on input
+ * (multi_pippenger.cpp:324-331); the identity is written as (0, 1).
+ * Compressed: 32-byte BE x, negated when y is not lexicographically largest
+ * (element.cpp:266-284).  Scalars: 32-byte LE, canonical below the scalar field
+ * order (fr.cpp:305-328). */
+#define ORC_CURVE_BANDERWAGON 3
+void orc_bw_generator(uint8_t* out64);
+/* 1 if the 64 wire bytes pass deserialize_uncompressed (element.cpp:330-355). */
+int  orc_bw_check(const uint8_t* p64);
+/* out = k P for any 256-bit LE k (double-and-add; the exact integer multiple). */
+void orc_bw_mul(const uint8_t* p64, const uint8_t* k_le, uint8_t* out64);
+void orc_bw_add(const uint8_t* a64, const uint8_t* b64, uint8_t* out64);
+void orc_bw_dbl(const uint8_t* a64, uint8_t* out64);
+void orc_bw_compress(const uint8_t* p64, uint8_t* out32);
+/* The reference's Banderwagon MSM (multi_pippenger.cpp:314-350 into
+ * multiexp.cpp:36-157): 0, or -2 (GPUKIT_ERR_BAD_SIZE) for a point or scalar
+ * the reference rejects. */
+int  orc_bw_msm(const uint8_t* scalars, const uint8_t* points, size_t n, uint8_t* out64);
+/* Same sum by double-and-add per term (no Pippenger). */
+int  orc_bw_msm_naive(const uint8_t* scalars, const uint8_t* points, size_t n, uint8_t* out64);
+/* points k_i G (k_i any 256-bit LE). */
+void orc_bw_gen_points(const uint8_t* ks, size_t n, uint8_t* out_wire);
 
 /* ---- seeded inputs ---- */
 uint64_t orc_splitmix64(uint64_t* state);

@@ -20,7 +20,9 @@
 #include <string.h>
 #include <unistd.h>
 
-static const orc_field* fr_of(int fr) { return orc_get_field(fr == 0 ? ORC_FIELD_BLS_FR : ORC_FIELD_BN_FR); }
+static const orc_field* fr_of(int fr) {
+    return orc_get_field(fr == 0 ? ORC_FIELD_BLS_FR : fr == 1 ? ORC_FIELD_BN_FR : ORC_FIELD_BW_FR);
+}
 
 /* 256-bit LE value mod r, into Montgomery form */
 static void load_reduced(const orc_field* R, u64* m, const uint8_t* le) {

@@ -13,7 +13,7 @@
 
 #include <string.h>
 
-static orc_field g_fields[4];
+static orc_field g_fields[5];
 static int g_init = 0;
 
 /* ---- plain multi-limb helpers ---- */
@@ -222,12 +222,16 @@ static void init_fields(void) {
                                 0x30644E72E131A029ULL};
     static const u64 bn_r[4] = {0x43E1F593F0000001ULL, 0x2833E84879B97091ULL, 0xB85045B68181585DULL,
                                 0x30644E72E131A029ULL};
+    /* Banderwagon scalar field r (banderwagon/cpp/fr.cpp:11-16). */
+    static const u64 bw_r[4] = {0x74FD06B52876E7E1ULL, 0xFF8F870074190471ULL, 0x0CCE760202687600ULL,
+                                0x1CFB69D4CA675F52ULL};
     memset(g_fields, 0, sizeof g_fields);
     g_fields[ORC_FIELD_BLS_FP].n = 6; memcpy(g_fields[ORC_FIELD_BLS_FP].p, bls_p, sizeof bls_p);
     g_fields[ORC_FIELD_BLS_FR].n = 4; memcpy(g_fields[ORC_FIELD_BLS_FR].p, bls_r, sizeof bls_r);
     g_fields[ORC_FIELD_BN_FP].n = 4;  memcpy(g_fields[ORC_FIELD_BN_FP].p, bn_p, sizeof bn_p);
     g_fields[ORC_FIELD_BN_FR].n = 4;  memcpy(g_fields[ORC_FIELD_BN_FR].p, bn_r, sizeof bn_r);
-    for (int i = 0; i < 4; ++i) derive(&g_fields[i]);
+    g_fields[ORC_FIELD_BW_FR].n = 4;  memcpy(g_fields[ORC_FIELD_BW_FR].p, bw_r, sizeof bw_r);
+    for (int i = 0; i < 5; ++i) derive(&g_fields[i]);
     g_init = 1;
 }
 

@@ -28,7 +28,9 @@ int main(void) {
     uint8_t buf[160];
     memset(buf, 0, sizeof buf);
     CHECK(gpukit_multi_pippenger_hip(GPUKIT_CURVE_BLS12_381_G1, buf, buf, 1, NULL) == GPUKIT_ERR_NULL_ARG);
-    CHECK(gpukit_multi_pippenger_hip(GPUKIT_CURVE_BANDERWAGON, buf, buf, 1, out) == GPUKIT_ERR_NOTIMPL);
+    CHECK(gpukit_multi_pippenger_hip(4, buf, buf, 1, out) == GPUKIT_ERR_BACKEND); /* no such curve */
+    CHECK(gpukit_multi_pippenger_hip(GPUKIT_CURVE_BANDERWAGON, NULL, NULL, 0, out) == GPUKIT_OK
+          && out[63] == 1 && out[0] == 0); /* the identity (0, 1) */
     CHECK(gpukit_multi_pippenger_hip(GPUKIT_CURVE_BN254_G1, NULL, NULL, 0, out) == GPUKIT_OK);
     CHECK(ntt_fr_forward(KINET_NTT_FIELD_BLS12_381_FR, NULL, 4) == -1);
     CHECK(ntt_fr_forward(KINET_NTT_FIELD_BN254_FR, buf, 3) == -1); /* not a power of two */

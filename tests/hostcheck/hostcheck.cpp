@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "curve.h"
+#include "curve_ed.h"
 #include "curve_quad.h"
 #include "glv.h"
 #include "ntt_math.h"
@@ -145,9 +146,9 @@ thread_local int HostQuad::id = 0;
 
 // Runs fn(lane) on four lanes; true if their results (returned as Xyzz)
 // agree, with the result in out.
-template <class F, class Fn>
-bool run_quad(Fn fn, Xyzz<F>& out) {
-    Xyzz<F> res[4];
+template <class F, class Fn, class Pt = Xyzz<F>>
+bool run_quad(Fn fn, Pt& out) {
+    Pt res[4];
     std::thread t[3];
     for (int k = 1; k < 4; ++k)
         t[k - 1] = std::thread([&, k] {
@@ -384,6 +385,74 @@ void reduce_wide_bytes(const uint8_t* in36, uint8_t* out32) {
     fe_to_words<F>(y, wd, 8);
     for (int b = 0; b < 32; ++b) out32[b] = (uint8_t)(wd[b / 4] >> (8 * (b % 4)));
 }
+
+// ---------------------------------------------------------------- Edwards
+// Banderwagon point operations of curve_ed.h.  Points arrive as affine wire
+// (BE x || y) with a projective scale z (LE, non-zero): X = x z, Y = y z,
+// T = x y z, Z = z.  Results leave as affine wire; -1 if the extended
+// coordinates are inconsistent (T Z != X Y), 0 if the four quad lanes
+// disagree, 1 otherwise.
+namespace {
+
+using BwF = BwFp;
+
+Fe<BwF> bw_be(const uint8_t* b) { return fe_to_mont<BwF>(fe_from_be_bytes<BwF>(b, 32)); }
+
+EdExt<BwF> bw_in(const uint8_t* w, const uint8_t* z_le) {
+    const Fe<BwF> x = bw_be(w), y = bw_be(w + 32), z = fe_to_mont<BwF>(from_le<BwF>(z_le, 32));
+    EdExt<BwF> p;
+    p.X = fe_mul<BwF>(x, z);
+    p.Y = fe_mul<BwF>(y, z);
+    p.T = fe_mul<BwF>(fe_mul<BwF>(x, y), z);
+    p.Z = z;
+    return p;
+}
+
+int bw_out(const EdExt<BwF>& p, uint8_t* w) {
+    if (!fe_eq<BwF>(fe_mul<BwF>(p.T, p.Z), fe_mul<BwF>(p.X, p.Y))) return -1;
+    const Fe<BwF> zi = fe_inv<BwF>(p.Z);
+    fe_to_be_bytes<BwF>(fe_from_mont<BwF>(fe_mul<BwF>(p.X, zi)), w, 32);
+    fe_to_be_bytes<BwF>(fe_from_mont<BwF>(fe_mul<BwF>(p.Y, zi)), w + 32, 32);
+    return 1;
+}
+
+// op: 0 add, 1 dbl, 2 mixed add (b affine, record t = d x y), 3 add_quad,
+// 4 dbl_quad, 5 negate a then add b
+int bw_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, uint8_t* out) {
+    const EdExt<BwF> P = bw_in(a, za), Q = bw_in(b, zb);
+    EdExt<BwF> r;
+    bool same = true;
+    switch (op) {
+        case 0: r = ed_add<BwF>(P, Q); break;
+        case 1: r = ed_dbl<BwF>(P); break;
+        case 2: {
+            EdAff<BwF> q;
+            q.x = bw_be(b);
+            q.y = bw_be(b + 32);
+            q.t = fe_mul<BwF>(fe_mul<BwF>(q.x, q.y), fe_const<BwF>(EdConst::D));
+            r = P;
+            ed_madd<BwF>(r, q);
+            break;
+        }
+        case 3: same = run_quad<BwF>([&] { return ed_add_quad<BwF, MulCall, HostQuad>(P, Q); }, r); break;
+        case 4: same = run_quad<BwF>([&] { return ed_dbl_quad<BwF, MulCall, HostQuad>(P); }, r); break;
+        default: r = ed_add<BwF>(ed_neg<BwF>(P), Q); break;
+    }
+    if (!same) return 0;
+    return bw_out(r, out);
+}
+
+// Euler's criterion as the Banderwagon reader applies it.
+int bw_is_square(const uint8_t* a_be) { return fe_is_square_nonzero<BwF>(bw_be(a_be)) ? 1 : 0; }
+
+}  // namespace
+
+extern "C" {
+int hc_bw_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, uint8_t* out) {
+    return bw_op(op, a, b, za, zb, out);
+}
+int hc_bw_is_square(const uint8_t* a_be) { return bw_is_square(a_be); }
+}  // extern "C"
 
 extern "C" {
 

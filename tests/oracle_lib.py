@@ -20,7 +20,10 @@ CURVE_BN254 = 1
 CURVE_BLS12_381 = 2
 FIELD_BLS_FP, FIELD_BLS_FR, FIELD_BN_FP, FIELD_BN_FR = 0, 1, 2, 3
 
-POINT_BYTES = {0: 64, CURVE_BN254: 64, CURVE_BLS12_381: 96}  # 0 = secp256k1
+CURVE_BANDERWAGON = 3
+FIELD_BW_FR = 4  # Banderwagon scalar field
+
+POINT_BYTES = {0: 64, CURVE_BN254: 64, CURVE_BLS12_381: 96, CURVE_BANDERWAGON: 64}  # 0 = secp256k1
 COMP_BYTES = {CURVE_BN254: 32, CURVE_BLS12_381: 48}
 
 _u8p = ctypes.c_char_p
@@ -58,6 +61,12 @@ def orc():
         lib.orc_fr_dot.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
         lib.orc_fr_eval.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
         lib.orc_fr_lagrange.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
+        for name, args in (("orc_bw_generator", [_u8p]), ("orc_bw_check", [_u8p]),
+                           ("orc_bw_mul", [_u8p, _u8p, _u8p]), ("orc_bw_add", [_u8p, _u8p, _u8p]),
+                           ("orc_bw_dbl", [_u8p, _u8p]), ("orc_bw_compress", [_u8p, _u8p]),
+                           ("orc_bw_msm", [_u8p, _u8p, _sz, _u8p]), ("orc_bw_msm_naive", [_u8p, _u8p, _sz, _u8p]),
+                           ("orc_bw_gen_points", [_u8p, _sz, _u8p])):
+            getattr(lib, name).argtypes = args
         _orc = lib
     return _orc
 
@@ -353,3 +362,68 @@ def ref_generator(curve: int) -> bytes:
     out = _buf(POINT_BYTES[curve])
     ref().ref_g1_generator(curve, out)
     return out.raw
+
+
+# ---------------------------------------------------------------- Banderwagon
+def bw_generator() -> bytes:
+    out = _buf(64)
+    orc().orc_bw_generator(out)
+    return out.raw
+
+
+def bw_check(p: bytes) -> bool:
+    return bool(orc().orc_bw_check(p))
+
+
+def bw_mul(p: bytes, k: int | bytes) -> bytes:
+    kb = k if isinstance(k, bytes) else k.to_bytes(32, "little")
+    out = _buf(64)
+    orc().orc_bw_mul(p, kb, out)
+    return out.raw
+
+
+def bw_mul_gen(k: int | bytes) -> bytes:
+    return bw_mul(bw_generator(), k)
+
+
+def bw_add(a: bytes, b: bytes) -> bytes:
+    out = _buf(64)
+    orc().orc_bw_add(a, b, out)
+    return out.raw
+
+
+def bw_dbl(a: bytes) -> bytes:
+    out = _buf(64)
+    orc().orc_bw_dbl(a, out)
+    return out.raw
+
+
+def bw_compress(p: bytes) -> bytes:
+    out = _buf(32)
+    orc().orc_bw_compress(p, out)
+    return out.raw
+
+
+def bw_msm(scalars: bytes, points: bytes) -> tuple[int, bytes]:
+    out = _buf(64)
+    rc = orc().orc_bw_msm(scalars, points, len(scalars) // 32, out)
+    return rc, out.raw
+
+
+def bw_msm_naive(scalars: bytes, points: bytes) -> tuple[int, bytes]:
+    out = _buf(64)
+    rc = orc().orc_bw_msm_naive(scalars, points, len(scalars) // 32, out)
+    return rc, out.raw
+
+
+def bw_gen_points(ks: bytes) -> bytes:
+    out = _buf(max(1, 2 * len(ks)))
+    orc().orc_bw_gen_points(ks, len(ks) // 32, out)
+    return out.raw[: 2 * len(ks)]
+
+
+def ref_bw_msm(scalars: bytes, points: bytes) -> tuple[int, bytes]:
+    """The reference's gpukit_multi_pippenger_cpu on curve 3: (status, 64 bytes)."""
+    out = _buf(64)
+    rc = ref().gpukit_multi_pippenger_cpu(CURVE_BANDERWAGON, scalars, points, len(scalars) // 32, out)
+    return rc, out.raw

@@ -53,9 +53,9 @@ def test_msm_empty_and_backend_env(ca, curve, monkeypatch):
 
 
 def test_msm_unsupported_curves(ca):
-    with pytest.raises(ca.KinetError) as e:  # Banderwagon
-        ca.multi_pippenger(3, bytes(32), bytes(64))
-    assert e.value.code == -5
+    with pytest.raises(ca.KinetError) as e:  # no curve 4 (GPUKIT_ERR_BACKEND, the reference's code)
+        ca.multi_pippenger(4, bytes(32), bytes(64))
+    assert e.value.code == -3
 
 
 def _closed_form(curve, scalars, k0, d):
@@ -307,7 +307,8 @@ def test_points_prepare_fills_exactly_the_queried_size(ca, curve):
     want_rec = {0: 80, 1: 80, 2: 128}[curve]
     nbytes = lib.kinet_hip_points_bytes(curve, n)
     assert nbytes == want_rec * n
-    assert lib.kinet_hip_points_bytes(3, n) == 0
+    assert lib.kinet_hip_points_bytes(3, n) == 128 * n
+    assert lib.kinet_hip_points_bytes(4, n) == 0
     k0, d = 99, 0x10001
     if curve == 0:  # secp256k1: the oracle has no secp256k1; pure-Python points
         G, r = M.CURVES[0]["g"], M.CURVES[0]["r"]
