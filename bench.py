@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G1 MSM Mpoints/s at N=2^20; batched commits/s + p99 vs plain GPU path"
 FULL_RATE_VALU = 256 * 64 * 2.4e9  # MI355X: CUs x lanes x peak engine clock, VALU instructions/s
-CURVE_NAMES = {0: "secp256k1", 1: "bn254", 2: "bls12_381"}
+CURVE_NAMES = {0: "secp256k1", 1: "bn254", 2: "bls12_381", 3: "banderwagon"}
 
 
 def progress(msg: str):
@@ -848,6 +848,12 @@ def main():
                                             g1=False)
         extras["secp256k1_plain_path_mpoints_s"] = round(n / sms / 1e3, 3)
         extras["secp256k1_plain_path_verified"] = sres[:64] == scheck
+        # Banderwagon (curve 3): twisted Edwards, plain path
+        bms, bst, bres, bcheck, _ = timed_msm(lib, torch, D, 3, n, 3333, max(3, args.steps // 2), 2, 0, True,
+                                              g1=False)
+        extras["banderwagon_plain_path_mpoints_s"] = round(n / bms / 1e3, 3)
+        extras["banderwagon_plain_path_verified"] = bres[:64] == bcheck
+        extras["banderwagon_stage_ms"] = {k: round(v, 4) for k, v in bst.items()}
         # fixed point sets (an SRS): window table built once, then one window per MSM
         for cv, seed in ((curve, 20261203), (other, 777)):
             tms, tst, tres, tcheck, text = timed_msm(lib, torch, D, cv, n, seed, max(3, args.steps // 2), 2, 0,

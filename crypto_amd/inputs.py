@@ -13,7 +13,8 @@ import numpy as np
 BLS12_381_R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 BN254_R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 SECP256K1_N = 0xFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFEBAAEDCE6AF48A03BBFD25E8CD0364141
-ORDER = {0: SECP256K1_N, 1: BN254_R, 2: BLS12_381_R}
+BANDERWAGON_R = 0x1CFB69D4CA675F520CCE760202687600FF8F87007419047174FD06B52876E7E1
+ORDER = {0: SECP256K1_N, 1: BN254_R, 2: BLS12_381_R, 3: BANDERWAGON_R}
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _C1 = np.uint64(0xBF58476D1CE4E5B9)
