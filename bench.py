@@ -54,9 +54,9 @@ def parse_args():
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="open-loop serving run length (0 = skip); the run with SRS window tables takes at most 10 s")
     ap.add_argument("--profile-run", action="store_true",
-                    help="for rocprofv3 counter runs: 2 MSMs of the workload, one forward Fr NTT per field "
-                         "at 2^20 and 2^22, and one 1 GiB device copy (the HBM-counter calibration kernel); "
-                         "no JSON line")
+                    help="for rocprofv3 counter runs: 2 MSMs of the workload, 2 Banderwagon MSMs of the same "
+                         "size, one forward Fr NTT per field at 2^20 and 2^22, and one 1 GiB device copy (the "
+                         "HBM-counter calibration kernel); no JSON line")
     return ap.parse_args()
 
 
@@ -734,6 +734,8 @@ def main():
 
     if args.profile_run:
         timed_msm(lib, torch, D, curve, n, 20261203, 2, 0, args.window_bits, profile=False)
+        # two Banderwagon MSMs of the same size (plain path)
+        timed_msm(lib, torch, D, 3, n, 3333, 2, 0, 0, profile=False, g1=False)
         # one forward Fr NTT per field at 2^20 and 2^22 (canonical random inputs)
         gen = torch.Generator(device="cuda")
         gen.manual_seed(11)
@@ -750,7 +752,7 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        print("profile run done: 2 MSMs + 4 NTTs + 1 GiB copy", flush=True)
+        print("profile run done: 2 + 2 MSMs + 4 NTTs + 1 GiB copy", flush=True)
         return
 
     ms, stage, result, check, ext = timed_msm(lib, torch, D, curve, n, 20261203, args.steps, args.warmup,
