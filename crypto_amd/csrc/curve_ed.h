@@ -134,6 +134,40 @@ KZ_DEV void ed_madd(EdExt<F>& p, const EdAff<F>& q) {
     p.Z = M::template mul<F>(Fv, G);
 }
 
+// ed_madd for the accumulation loop, lazily reduced (field.h's lazy section):
+// on entry the accumulator's coordinates are below 2q (canonical ones
+// included), the record canonical; the result keeps that bound and ed_canon
+// makes it canonical.  A and B are full products (below q), so
+//   X1 + Y1 < 4q, x2 + y2 < 2q, E = E' + 2q - A - B < 4q, F = Z1 + 2q - C < 4q,
+//   G = Z1 + C < 4q, H = B + 5A < 6q,
+// every product operand pair is at most 24 q^2 < R q (R / q = 70.7), and a
+// lazy product of such a pair is below q (1 + 24 / 70.7) < 2q.
+template <class F>
+KZ_DEV void ed_madd_lazy(EdExt<F>& p, const EdAff<F>& q) {
+    using L = MulLazy;
+    const Fe<F> A = fe_mul<F>(p.X, q.x);
+    const Fe<F> B = fe_mul<F>(p.Y, q.y);
+    const Fe<F> C = L::template mul<F>(p.T, q.t);
+    const Fe<F> Ep = L::template mul<F>(fe_add_nored<F>(p.X, p.Y), fe_add_nored<F>(q.x, q.y));
+    const Fe<F> E = fe_kp_sub2<F, 2>(Ep, A, B);
+    const Fe<F> Fv = fe_sub_kp<F, 2>(p.Z, C), G = fe_add_nored<F>(p.Z, C);
+    const Fe<F> H = fe_add_nored<F>(B, fe_small_mul<F, 5>(A));
+    p.X = L::template mul<F>(E, Fv);
+    p.Y = L::template mul<F>(G, H);
+    p.T = L::template mul<F>(E, H);
+    p.Z = L::template mul<F>(Fv, G);
+}
+
+// Canonical coordinates of a point whose coordinates are below 2q.
+template <class F>
+KZ_DEV EdExt<F> ed_canon(EdExt<F> p) {
+    fe_reduce_once<F>(p.X);
+    fe_reduce_once<F>(p.Y);
+    fe_reduce_once<F>(p.T);
+    fe_reduce_once<F>(p.Z);
+    return p;
+}
+
 // p + q (add-2008-hwcd): as ed_madd with C = d T1 T2 and D = Z1 Z2 in place of Z1.
 template <class F, class M = MulInline>
 KZ_DEV EdExt<F> ed_add(const EdExt<F>& p, const EdExt<F>& q) {
@@ -207,6 +241,60 @@ __host__ __device__ EdExt<F> ed_dbl_quad(const EdExt<F>& p) {
     const Fe<F> D = fe_neg<F>(fe_mul5<F>(A));
     const Fe<F> G = fe_add<F>(D, B), Fv = fe_sub<F>(G, C), H = fe_sub<F>(D, B);
     m = M::template mul<F>(quad_pick<F>(l, E, G, E, Fv), quad_pick<F>(l, Fv, H, H, G));
+    EdExt<F> r;
+    r.X = quad_read<0, F, Lanes>(m);
+    r.Y = quad_read<1, F, Lanes>(m);
+    r.T = quad_read<2, F, Lanes>(m);
+    r.Z = quad_read<3, F, Lanes>(m);
+    return r;
+}
+
+// Lazily reduced quad forms for chains of point operations (bucket reduction,
+// window combine): inputs and outputs have coordinates below 2q.  MC is a
+// canonical product policy, ML its form without the final subtraction; the
+// first level's products are canonical (below q), the later ones lazy, and
+// differences are one-pass a + k q - b with k q above b's bound:
+//   addition  A, B, T1 T2, D < q; C < 2q; E' = (X1+Y1)(X2+Y2) < 2q
+//             (operands below 4q); E = E' + 2q - A - B < 4q,
+//             F = D + 2q - C < 3q, G = D + C < 3q, H = B + 5A < 6q;
+//             level 3 operand pairs at most 24 q^2
+//   doubling  A = X^2, B = Y^2, Z^2, S^2 < q (S = X + Y < 4q); C = 2 Z^2 < 2q;
+//             D = 5q - 5A in (0, 5q]; E = S^2 + 2q - A - B < 3q; G = D + B < 6q;
+//             F = G + 2q - C < 8q; H = D + q - B < 6q; level 2 pairs at most 48 q^2
+// so every lazy product has operands below R q / 70.7 and stays below 2q.
+template <class F, class MC, class ML, class Lanes = DeviceQuad>
+__host__ __device__ EdExt<F> ed_add_quad_lazy(const EdExt<F>& p, const EdExt<F>& q) {
+    const int l = Lanes::lane();
+    Fe<F> m = MC::template mul<F>(quad_pick<F>(l, p.X, p.Y, p.T, p.Z), quad_pick<F>(l, q.X, q.Y, q.T, q.Z));
+    const Fe<F> A = quad_read<0, F, Lanes>(m), B = quad_read<1, F, Lanes>(m);
+    const Fe<F> TT = quad_read<2, F, Lanes>(m), D = quad_read<3, F, Lanes>(m);
+    const Fe<F> S1 = fe_add_nored<F>(p.X, p.Y), S2 = fe_add_nored<F>(q.X, q.Y);
+    m = ML::template mul<F>(quad_pick<F>(l, TT, S1, TT, S1), quad_pick<F>(l, fe_const<F>(EdConst::D), S2,
+                                                                             fe_const<F>(EdConst::D), S2));
+    const Fe<F> C = quad_read<0, F, Lanes>(m);
+    const Fe<F> E = fe_kp_sub2<F, 2>(quad_read<1, F, Lanes>(m), A, B);
+    const Fe<F> Fv = fe_sub_kp<F, 2>(D, C), G = fe_add_nored<F>(D, C);
+    const Fe<F> H = fe_add_nored<F>(B, fe_small_mul<F, 5>(A));
+    m = ML::template mul<F>(quad_pick<F>(l, E, G, E, Fv), quad_pick<F>(l, Fv, H, H, G));
+    EdExt<F> r;
+    r.X = quad_read<0, F, Lanes>(m);
+    r.Y = quad_read<1, F, Lanes>(m);
+    r.T = quad_read<2, F, Lanes>(m);
+    r.Z = quad_read<3, F, Lanes>(m);
+    return r;
+}
+
+template <class F, class MC, class ML, class Lanes = DeviceQuad>
+__host__ __device__ EdExt<F> ed_dbl_quad_lazy(const EdExt<F>& p) {
+    const int l = Lanes::lane();
+    const Fe<F> S = fe_add_nored<F>(p.X, p.Y);
+    Fe<F> m = MC::template mul<F>(quad_pick<F>(l, p.X, p.Y, p.Z, S), quad_pick<F>(l, p.X, p.Y, p.Z, S));
+    const Fe<F> A = quad_read<0, F, Lanes>(m), B = quad_read<1, F, Lanes>(m);
+    const Fe<F> C = fe_small_mul<F, 2>(quad_read<2, F, Lanes>(m));
+    const Fe<F> E = fe_kp_sub2<F, 2>(quad_read<3, F, Lanes>(m), A, B);
+    const Fe<F> D = fe_sub_kp<F, 5>(fe_zero<F>(), fe_small_mul<F, 5>(A));
+    const Fe<F> G = fe_add_nored<F>(D, B), Fv = fe_sub_kp<F, 2>(G, C), H = fe_sub_kp<F, 1>(D, B);
+    m = ML::template mul<F>(quad_pick<F>(l, E, G, E, Fv), quad_pick<F>(l, Fv, H, H, G));
     EdExt<F> r;
     r.X = quad_read<0, F, Lanes>(m);
     r.Y = quad_read<1, F, Lanes>(m);

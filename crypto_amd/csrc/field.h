@@ -375,6 +375,34 @@ KZ_DEV Fe<F> fe_r6p_s_2q(const Fe<F>& r, const Fe<F>& s, const Fe<F>& q) {
     return d;
 }
 
+// a + b with normalised limbs and no reduction (a + b < 2^(29N)).
+template <class F>
+KZ_DEV Fe<F> fe_add_nored(const Fe<F>& a, const Fe<F>& b) {
+    Fe<F> s;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const uint32_t t = a.v[i] + b.v[i] + carry;
+        carry = t >> kLimbBits;
+        s.v[i] = t & kLimbMask;
+    }
+    return s;
+}
+
+// a + K p - b - c in one pass (caller: the value lies in (0, 2^(29N)))
+template <class F, int K>
+KZ_DEV Fe<F> fe_kp_sub2(const Fe<F>& a, const Fe<F>& b, const Fe<F>& c) {
+    Fe<F> d;
+    int32_t cy = 0;
+#pragma unroll
+    for (int i = 0; i < F::N; ++i) {
+        const int32_t t = (int32_t)(a.v[i] + kp_limb<F, K>(i)) - (int32_t)b.v[i] - (int32_t)c.v[i] + cy;
+        cy = t >> kLimbBits;
+        d.v[i] = (uint32_t)t & kLimbMask;
+    }
+    return d;
+}
+
 // a + K p - 2b in one pass (caller: the value lies in (0, 2^(29N)))
 template <class F, int K>
 KZ_DEV Fe<F> fe_kp_2b(const Fe<F>& a, const Fe<F>& b) {
@@ -389,7 +417,7 @@ KZ_DEV Fe<F> fe_kp_2b(const Fe<F>& a, const Fe<F>& b) {
     return d;
 }
 
-// m a for m <= 3 and m a < 2^(29N), no reduction
+// m a for m <= 7 (limb products stay below 2^32) and m a < 2^(29N), no reduction
 template <class F, int Mul>
 KZ_DEV Fe<F> fe_small_mul(const Fe<F>& a) {
     Fe<F> d;

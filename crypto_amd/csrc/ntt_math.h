@@ -23,20 +23,6 @@
 
 namespace kz {
 
-// a + b with normalised limbs and no reduction (a + b < 2^(29N)).
-template <class F>
-KZ_DEV Fe<F> fe_add_nored(const Fe<F>& a, const Fe<F>& b) {
-    Fe<F> s;
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < F::N; ++i) {
-        const uint32_t t = a.v[i] + b.v[i] + carry;
-        carry = t >> kLimbBits;
-        s.v[i] = t & kLimbMask;
-    }
-    return s;
-}
-
 // x mod p for a normalised x below 64p.  q = floor(x_top / (p_top + 1)) from
 // the top limbs never exceeds floor(x / p) and falls short of it by at most
 // one (p_top >= 2^21 for the 9-limb fields, so (x_top + p_top + 1) /

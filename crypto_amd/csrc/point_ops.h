@@ -92,15 +92,20 @@ struct PointOps<BwFp> {
     KZ_DEVI static void store_aff(uint32_t* __restrict__ b, size_t i, const Aff& a) { ed_aff_store<F>(b, i, a); }
     KZ_DEVI static Pt load(const uint32_t* __restrict__ b, size_t i) { return ed_load<F>(b, i); }
     KZ_DEVI static void store(uint32_t* __restrict__ b, size_t i, const Pt& p) { ed_store<F>(b, i, p); }
-    KZ_DEVI static void madd_acc(Pt& acc, const Aff& q) { ed_madd<F>(acc, q); }
-    KZ_DEVI static Pt canon(const Pt& p) { return p; }
+    // coordinates below 2q until stored
+    KZ_DEVI static void madd_acc(Pt& acc, const Aff& q) { ed_madd_lazy<F>(acc, q); }
+    KZ_DEVI static Pt canon(const Pt& p) { return ed_canon<F>(p); }
+    // lazy policies keep coordinates below 2q (curve_ed.h); canonical ones
+    // keep them canonical
     template <class M>
     KZ_DEVI static Pt add_q(const Pt& a, const Pt& b) {
-        return ed_add_quad<F, typename CanonMul<M>::type>(a, b);
+        if constexpr (M::kLazy) return ed_add_quad_lazy<F, typename CanonMul<M>::type, M>(a, b);
+        else return ed_add_quad<F, M>(a, b);
     }
     template <class M>
     KZ_DEVI static Pt dbl_q(const Pt& a) {
-        return ed_dbl_quad<F, typename CanonMul<M>::type>(a);
+        if constexpr (M::kLazy) return ed_dbl_quad_lazy<F, typename CanonMul<M>::type, M>(a);
+        else return ed_dbl_quad<F, M>(a);
     }
     template <class M>
     KZ_DEVI static Pt add(const Pt& a, const Pt& b) {

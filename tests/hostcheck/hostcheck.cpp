@@ -442,6 +442,67 @@ int bw_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const u
     return bw_out(r, out);
 }
 
+// The accumulation loop's lazily reduced mixed additions: a (scaled by z,
+// every coordinate raised by q, the largest representative the loop accepts)
+// plus the n affine records of pts, with no canonical step in between.
+int bw_madd_chain(const uint8_t* a, const uint8_t* za, const uint8_t* pts, int n, uint8_t* out) {
+    EdExt<BwF> P = bw_in(a, za);
+    P.X = add_kp<BwF>(P.X, 1);
+    P.Y = add_kp<BwF>(P.Y, 1);
+    P.T = add_kp<BwF>(P.T, 1);
+    P.Z = add_kp<BwF>(P.Z, 1);
+    for (int i = 0; i < n; ++i) {
+        EdAff<BwF> q;
+        q.x = bw_be(pts + 64 * i);
+        q.y = bw_be(pts + 64 * i + 32);
+        q.t = fe_mul<BwF>(fe_mul<BwF>(q.x, q.y), fe_const<BwF>(EdConst::D));
+        ed_madd_lazy<BwF>(P, q);
+        const Fe<BwF>* c[4] = {&P.X, &P.Y, &P.T, &P.Z};
+        for (const Fe<BwF>* v : c)  // the bound the loop relies on: below 2q
+            if (!fe_gt<BwF>(add_kp<BwF>(fe_zero<BwF>(), 2), *v)) return -2;
+    }
+    return bw_out(ed_canon<BwF>(P), out);
+}
+
+bool bw_below_2q(const EdExt<BwF>& p) {
+    const Fe<BwF> two_q = add_kp<BwF>(fe_zero<BwF>(), 2);
+    return fe_gt<BwF>(two_q, p.X) && fe_gt<BwF>(two_q, p.Y) && fe_gt<BwF>(two_q, p.T) && fe_gt<BwF>(two_q, p.Z);
+}
+
+EdExt<BwF> bw_raise(EdExt<BwF> p) {  // the largest representative below 2q
+    p.X = add_kp<BwF>(p.X, 1);
+    p.Y = add_kp<BwF>(p.Y, 1);
+    p.T = add_kp<BwF>(p.T, 1);
+    p.Z = add_kp<BwF>(p.Z, 1);
+    return p;
+}
+
+// The lazily reduced quad forms over four host lanes, in a chain with no
+// canonical step: `rounds` times P = 2P; P = P + Q, from the largest
+// representatives of a and b; -2 if a coordinate ever reaches 2q, 0 if the
+// lanes disagree.
+int bw_quad_lazy_chain(const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, int rounds,
+                       int inline_mul, uint8_t* out) {
+    EdExt<BwF> P = bw_raise(bw_in(a, za));
+    const EdExt<BwF> Q = bw_raise(bw_in(b, zb));
+    for (int i = 0; i < rounds; ++i) {
+        EdExt<BwF> r;
+        bool same = inline_mul
+            ? run_quad<BwF>([&] { return ed_dbl_quad_lazy<BwF, MulInline, MulLazy, HostQuad>(P); }, r)
+            : run_quad<BwF>([&] { return ed_dbl_quad_lazy<BwF, MulCall, MulCallLazy, HostQuad>(P); }, r);
+        if (!same) return 0;
+        if (!bw_below_2q(r)) return -2;
+        P = r;
+        same = inline_mul
+            ? run_quad<BwF>([&] { return ed_add_quad_lazy<BwF, MulInline, MulLazy, HostQuad>(P, Q); }, r)
+            : run_quad<BwF>([&] { return ed_add_quad_lazy<BwF, MulCall, MulCallLazy, HostQuad>(P, Q); }, r);
+        if (!same) return 0;
+        if (!bw_below_2q(r)) return -2;
+        P = r;
+    }
+    return bw_out(ed_canon<BwF>(P), out);
+}
+
 // Euler's criterion as the Banderwagon reader applies it.
 int bw_is_square(const uint8_t* a_be) { return fe_is_square_nonzero<BwF>(bw_be(a_be)) ? 1 : 0; }
 
@@ -452,6 +513,13 @@ int hc_bw_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, cons
     return bw_op(op, a, b, za, zb, out);
 }
 int hc_bw_is_square(const uint8_t* a_be) { return bw_is_square(a_be); }
+int hc_bw_quad_lazy_chain(const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, int rounds,
+                          int inline_mul, uint8_t* out) {
+    return bw_quad_lazy_chain(a, b, za, zb, rounds, inline_mul, out);
+}
+int hc_bw_madd_chain(const uint8_t* a, const uint8_t* za, const uint8_t* pts, int n, uint8_t* out) {
+    return bw_madd_chain(a, za, pts, n, out);
+}
 }  // extern "C"
 
 extern "C" {

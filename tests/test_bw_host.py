@@ -100,3 +100,52 @@ def test_euler_criterion(hc):
     assert hc.hc_bw_is_square(bytes(32)) == 0
     assert hc.hc_bw_is_square((1).to_bytes(32, "big")) == 1
     assert hc.hc_bw_is_square((7).to_bytes(32, "big")) == 0  # 7 generates the multiplicative group
+
+
+def test_lazy_mixed_addition_chains(hc):
+    """The accumulation loop's mixed additions (ed_madd_lazy): chains of 1, 8
+    and 40 additions from the largest accepted representative of the start
+    point, no canonical step in between, every coordinate checked below 2q
+    after each addition; the chains include the identity, (0, -1), a point
+    and its negation, the point itself (doubling) and (-x, -y)."""
+    rnd = random.Random(81)
+    start = O.bw_mul_gen(rnd.randrange(1, R_BW))
+    for n in (1, 8, 40):
+        pts = [O.bw_mul_gen(rnd.randrange(1, R_BW)) for _ in range(n)]
+        if n >= 8:
+            pts[1] = IDENTITY
+            pts[2] = TWO_TORSION
+            pts[3] = neg(pts[0])
+            pts[5] = other_rep(pts[4])
+        for s0 in (start, IDENTITY, pts[0]):
+            za = rnd.randrange(1, Q).to_bytes(32, "little")
+            out = ctypes.create_string_buffer(64)
+            rc = hc.hc_bw_madd_chain(s0, za, b"".join(pts), n, out)
+            assert rc == 1, rc
+            want = s0
+            for p in pts:
+                want = O.bw_add(want, p)
+            assert out.raw == want
+
+
+@pytest.mark.parametrize("inline_mul", [0, 1])
+def test_lazy_quad_chains(hc, inline_mul):
+    """The lazily reduced quad doubling and addition (bucket reduction and
+    window combine): chains of 2P then P + Q over four host lanes from the
+    largest accepted representatives, no canonical step, every coordinate
+    checked below 2q; with Q the identity, (0, -1), -P-like and random points."""
+    rnd = random.Random(91 + inline_mul)
+    g = O.bw_generator()
+    cases = [(O.bw_mul_gen(rnd.randrange(1, R_BW)), O.bw_mul_gen(rnd.randrange(1, R_BW)), 24),
+             (g, IDENTITY, 8), (g, TWO_TORSION, 8), (IDENTITY, g, 6), (other_rep(g), neg(g), 5),
+             (TWO_TORSION, TWO_TORSION, 4)]
+    for a, b, rounds in cases:
+        za = rnd.randrange(1, Q).to_bytes(32, "little")
+        zb = rnd.randrange(1, Q).to_bytes(32, "little")
+        out = ctypes.create_string_buffer(64)
+        rc = hc.hc_bw_quad_lazy_chain(a, b, za, zb, rounds, inline_mul, out)
+        assert rc == 1, rc
+        want = a
+        for _ in range(rounds):
+            want = O.bw_add(O.bw_dbl(want), b)
+        assert out.raw == want
