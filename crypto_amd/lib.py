@@ -93,6 +93,10 @@ def library():
         lib.kzg_srs_create.argtypes = [u32, ctypes.c_char_p, sz, pp]
         lib.kzg_open.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
         lib.kzg_open.restype = i32
+        lib.kzg_open_coeffs.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+        lib.kzg_open_coeffs.restype = i32
+        lib.kzg_commit_coeffs.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p]
+        lib.kzg_commit_coeffs.restype = i32
         lib.kzg_srs_generate.argtypes = [u32, ctypes.c_char_p, sz, pp]
         lib.kzg_srs_destroy.argtypes = [vp]
         lib.kzg_srs_destroy.restype = None
@@ -240,6 +244,23 @@ class KzgSrs:
         rc = library().kzg_open(self.handle, evals, len(evals) // 32, z.to_bytes(32, "little"), proof, y)
         if rc != 0:
             raise KinetError("kzg_open", rc, CRYPTO_STATUS)
+        return proof.raw, int.from_bytes(y.raw, "little")
+
+    def commit_coeffs(self, coeffs: bytes) -> bytes:
+        """Commitment from coefficients c_i of f = sum c_i X^i (kzg_commit_coeffs)."""
+        out = ctypes.create_string_buffer(compressed_bytes(self.curve))
+        rc = library().kzg_commit_coeffs(self.handle, coeffs, len(coeffs) // 32, out)
+        if rc != 0:
+            raise KinetError("kzg_commit_coeffs", rc, CRYPTO_STATUS)
+        return out.raw
+
+    def open_coeffs(self, coeffs: bytes, z: int) -> tuple[bytes, int]:
+        """Opening proof at any z from coefficients (kzg_open_coeffs): (compressed proof, y = f(z))."""
+        proof = ctypes.create_string_buffer(compressed_bytes(self.curve))
+        y = ctypes.create_string_buffer(32)
+        rc = library().kzg_open_coeffs(self.handle, coeffs, len(coeffs) // 32, z.to_bytes(32, "little"), proof, y)
+        if rc != 0:
+            raise KinetError("kzg_open_coeffs", rc, CRYPTO_STATUS)
         return proof.raw, int.from_bytes(y.raw, "little")
 
     def msm(self, scalars: bytes) -> bytes:

@@ -75,6 +75,16 @@ int kzg_open(const kzg_srs* srs, const uint8_t* evals, size_t n, const uint8_t z
  * only, the form documented for the reference's planned C API
  * (docs/content/docs/api/kzg.mdx:289-331, `kzg_commit(srs, coeffs, ...)`). */
 int kzg_commit_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, uint8_t* commit_out);
+/* Opening from coefficients, the documented `kzg_open(srs, coeffs, num_coeffs,
+ * z, proof_out, y_out)` (docs/content/docs/api/kzg.mdx:305-312): y = f(z)
+ * (32 bytes LE canonical) and proof = [q(tau)] G, compressed like a
+ * commitment, for the quotient q(X) = (f(X) - y) / (X - z) by synthetic
+ * division (the reference's kzg/gpu/metal/kzg.metal:216-251).  Any
+ * 1 <= n <= SRS size and any canonical z, domain points included; n = 1 gives
+ * the identity as proof.  CRYPTO_ERR_INPUT for bad arguments or non-canonical
+ * coefficients or z. */
+int kzg_open_coeffs(const kzg_srs* srs, const uint8_t* coeffs, size_t n, const uint8_t z[32], uint8_t* proof_out,
+                    uint8_t y_out[32]);
 /* Alias of kzg_srs_destroy under the documented name (kzg.mdx: kzg_srs_free). */
 void kzg_srs_free(kzg_srs* srs);
 

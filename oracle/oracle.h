@@ -104,6 +104,10 @@ int  orc_fr_eval(int fr, const uint8_t* coeffs, size_t n, const uint8_t* x_le, u
 /* f(tau) from f(w^j), j < n, by the barycentric formula; -1 if tau is in the
  * domain, n is not a supported power of two or an input is not canonical. */
 int  orc_fr_lagrange(int fr, const uint8_t* evals, size_t n, const uint8_t* tau_le, uint8_t* out_le);
+/* Synthetic division f = (X - z) q + y (kzg/gpu/metal/kzg.metal:201-251):
+ * q_out n - 1 elements, y_out = f(z); -1 for n = 0 or non-canonical input. */
+int  orc_fr_divide_linear(int fr, const uint8_t* coeffs, size_t n, const uint8_t* z_le, uint8_t* q_out,
+                          uint8_t* y_out);
 
 /* ---- Banderwagon (curve 3 of the multi-curve entry) ----
  * Twisted Edwards a x^2 + y^2 = 1 + d x^2 y^2 over BLS12-381 Fr, a = -5

@@ -10,6 +10,8 @@
  *                         f(tau) = (tau^n - 1) / n * sum_j f(w^j) w^j / (tau - w^j)
  *                      with one field inversion (Montgomery's batch trick);
  *                      it uses neither an NTT nor the coefficients.
+ *   orc_fr_divide_linear  f = (X - z) q + y by synthetic division (the
+ *                      quotient of an opening proof from coefficients)
  * Elements are 32-byte little-endian.  Inputs of dot may be any 256-bit
  * values (reduced mod r first); eval / lagrange inputs must be canonical.
  */
@@ -71,6 +73,25 @@ int orc_fr_eval(int fr, const uint8_t* coeffs, size_t n, const uint8_t* x_le, ui
         f_add(R, acc, acc, c);
     }
     store_plain(R, out, acc);
+    return 0;
+}
+
+/* Synthetic division of f = sum c_i X^i by (X - z), following the reference's
+ * poly_quotient (kzg/gpu/metal/kzg.metal:216-251): q[n-2] = c[n-1],
+ * q[i] = c[i+1] + z q[i+1] from the top, and y = f(z) = c[0] + z q[0] (its
+ * Horner p(z), kzg.metal:201-213).  q_out receives n - 1 elements. */
+int orc_fr_divide_linear(int fr, const uint8_t* coeffs, size_t n, const uint8_t* z_le, uint8_t* q_out,
+                         uint8_t* y_out) {
+    const orc_field* R = fr_of(fr);
+    u64 z[4], acc[4] = {0, 0, 0, 0};
+    if (n == 0 || load_canonical(R, z, z_le)) return -1;
+    for (size_t i = n; i-- > 0;) {
+        u64 c[4];
+        if (load_canonical(R, c, coeffs + 32 * i)) return -1;
+        f_mul(R, acc, acc, z);
+        f_add(R, acc, acc, c);
+        store_plain(R, i ? q_out + 32 * (i - 1) : y_out, acc);
+    }
     return 0;
 }
 

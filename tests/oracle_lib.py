@@ -61,6 +61,7 @@ def orc():
         lib.orc_fr_dot.argtypes = [ctypes.c_int, _u8p, _u8p, _sz, _u8p]
         lib.orc_fr_eval.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
         lib.orc_fr_lagrange.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p]
+        lib.orc_fr_divide_linear.argtypes = [ctypes.c_int, _u8p, _sz, _u8p, _u8p, _u8p]
         for name, args in (("orc_bw_generator", [_u8p]), ("orc_bw_check", [_u8p]),
                            ("orc_bw_mul", [_u8p, _u8p, _u8p]), ("orc_bw_add", [_u8p, _u8p, _u8p]),
                            ("orc_bw_dbl", [_u8p, _u8p]), ("orc_bw_compress", [_u8p, _u8p]),
@@ -137,6 +138,15 @@ def fr_eval(fr: int, coeffs: bytes, x: int) -> int | None:
     if orc().orc_fr_eval(fr, coeffs, len(coeffs) // 32, x.to_bytes(32, "little"), out):
         return None
     return int.from_bytes(out.raw, "little")
+
+
+def fr_divide_linear(fr: int, coeffs: bytes, z: int) -> tuple[bytes, int] | None:
+    """Synthetic division f = (X - z) q + y (C): (q's n - 1 coefficients, y)."""
+    n = len(coeffs) // 32
+    q, y = _buf(32 * max(n - 1, 1)), _buf(32)
+    if orc().orc_fr_divide_linear(fr, coeffs, n, z.to_bytes(32, "little"), q, y):
+        return None
+    return q.raw[:32 * (n - 1)], int.from_bytes(y.raw, "little")
 
 
 def fr_lagrange(fr: int, evals: bytes, tau: int) -> int | None:

@@ -173,3 +173,73 @@ def test_open_rejects_domain_points_and_bad_inputs(ca, curve):
             srs.open(ev, z % (1 << 256))
     with pytest.raises(crypto_amd.KinetError):
         srs.open(ev[:32 * 12], 9)  # not a power of two
+
+
+def _identity_compressed(curve):
+    return O.compress(curve, bytes(96 if curve == 2 else 64))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 1000, 1025, 4096])
+def test_open_coeffs_matches_oracle(ca, curve, n):
+    """kzg_open_coeffs against the oracle's synthetic division and its MSM of
+    the quotient over the same SRS points, for random z, 0, 1, -1 and a
+    domain point; n crosses the 32-element segments of the device scan."""
+    r = M.CURVES[curve]["r"]
+    fr = O.fr_of_curve(curve)
+    tau = int.from_bytes(O.gen_scalars(curve, 31, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, 4096)
+    pts = srs.export(0, max(n - 1, 1))
+    c = O.gen_scalars(curve, 500 + n, n)
+    rnd = random.Random(n)
+    for z in (rnd.randrange(r), 0, 1, r - 1, O.root_of_unity(fr, 64)):
+        print(f"kzg open_coeffs curve={curve} n={n} z={z % 1000}", flush=True)
+        proof, y = srs.open_coeffs(c, z)
+        q, y_want = O.fr_divide_linear(fr, c, z)
+        assert y == y_want
+        want = O.compress(curve, O.msm(curve, q, pts)) if n > 1 else _identity_compressed(curve)
+        assert proof == want
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_open_coeffs_closed_form_large(ca, curve):
+    """At n = 2^20 + 3 coefficients (five scan levels, ragged at each): y = f(z)
+    by Horner and the proof equals [(f(tau) - y) / (tau - z)] G."""
+    r = M.CURVES[curve]["r"]
+    fr = O.fr_of_curve(curve)
+    n = (1 << 20) + 3
+    tau = int.from_bytes(O.gen_scalars(curve, 32, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, n)
+    c = O.gen_scalars(curve, 600, n)
+    z = random.Random(7).randrange(r)
+    print(f"kzg open_coeffs closed form curve={curve} n={n}", flush=True)
+    proof, y = srs.open_coeffs(c, z)
+    assert y == O.fr_eval(fr, c, z)
+    q_tau = (O.fr_eval(fr, c, tau) - y) * pow((tau - z) % r, -1, r) % r
+    assert proof == O.compress(curve, O.g1_mul_gen(curve, q_tau.to_bytes(32, "little")))
+    assert srs.commit_coeffs(c) == O.compress(curve, O.g1_mul_gen(curve, O.fr_eval(fr, c, tau).to_bytes(32, "little")))
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_open_coeffs_matches_evaluation_form(ca, curve):
+    """The two openings agree where both apply: kzg_open on f's evaluations
+    (the forward NTT of its coefficients) and kzg_open_coeffs on the
+    coefficients, z outside the domain."""
+    fr = O.fr_of_curve(curve)
+    n = 256
+    srs = ca.KzgSrs.generate(curve, 4242, n)
+    c = O.gen_scalars(curve, 700, n)
+    ev = O.ntt(fr, c)
+    for z in (2, 123456789):
+        assert srs.open_coeffs(c, z) == srs.open(ev, z)
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_open_coeffs_refusals(ca, curve):
+    r = M.CURVES[curve]["r"]
+    srs = ca.KzgSrs.generate(curve, 77, 16)
+    c = O.gen_scalars(curve, 800, 16)
+    for bad_c, z in ((b"", 3), (c + c[:32], 3), (c, r), (c[:32 * 15] + r.to_bytes(32, "little"), 3)):
+        with pytest.raises(ca.KinetError) as e:
+            srs.open_coeffs(bad_c, z)
+        assert e.value.code == -1

@@ -434,6 +434,37 @@ def test_fr_lagrange_against_interpolation(fr, n):
 
 
 @pytest.mark.parametrize("fr", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 5, 33, 300])
+def test_fr_divide_linear_against_product(fr, n):
+    """orc_fr_divide_linear returns q and y with (X - z) q(X) + y = f(X),
+    checked by multiplying back on Python ints coefficient by coefficient; y
+    is f(z) by the power sum; z = 0 shifts the coefficients down; a multiple
+    (X - z) g(X) leaves y = 0 and q = g; non-canonical input is refused."""
+    r = FRS[fr]
+    rnd = random.Random(96 + n + fr)
+    c = _rand_fr(fr, n, 97 + n + fr)
+    for z in (rnd.randrange(r), 0, 1, r - 1):
+        q_b, y = O.fr_divide_linear(fr, _enc(c), z)
+        q = _dec(q_b)
+        assert len(q) == n - 1
+        back = [(-z * (q[0] if q else 0)) % r] + [((q[i - 1] if i - 1 < len(q) else 0) -
+                                                   z * (q[i] if i < len(q) else 0)) % r for i in range(1, n)]
+        back[0] = (back[0] + y) % r
+        assert back == c
+        assert y == sum(ci * pow(z, i, r) for i, ci in enumerate(c)) % r
+        if z == 0:
+            assert q == c[1:] and y == c[0]
+    g = _rand_fr(fr, n, 98 + n + fr)
+    z = rnd.randrange(r)
+    f = [(-z * g[0]) % r] + [(g[i - 1] - z * (g[i] if i < n else 0)) % r for i in range(1, n + 1)]
+    q_b, y = O.fr_divide_linear(fr, _enc(f), z)
+    assert y == 0 and _dec(q_b) == g
+    assert O.fr_divide_linear(fr, _enc([r]), 1) is None
+    assert O.fr_divide_linear(fr, _enc(c), r) is None
+    assert O.fr_divide_linear(fr, b"", 1) is None
+
+
+@pytest.mark.parametrize("fr", [0, 1])
 def test_fr_lagrange_matches_ntt_route_at_4096(fr):
     """At 4096 points: barycentric f(tau) equals Horner on the oracle's iNTT."""
     r = FRS[fr]
