@@ -26,7 +26,7 @@ typedef struct kzg_server kzg_server;
 typedef struct kzg_server_config {
     uint32_t lanes;           /* concurrent batches (HIP streams); 0 = 3 */
     uint32_t max_batch;       /* requests per batch; 0 = 128 */
-    uint64_t max_batch_elems; /* evaluations per batch; 0 = 2^21 */
+    uint64_t max_batch_elems; /* evaluations per batch; 0 = the largest power of two within an SRS, at least 2^20 */
     uint64_t hbm_budget;      /* device bytes of admitted, unfinished batches; 0 = 8 GiB */
 } kzg_server_config;
 
