@@ -114,11 +114,14 @@ def test_precomputed_srs_commits_unchanged(ca, curve):
     before = {m: srs.commit(e) for m, e in evs.items()}
     sc = O.gen_scalars(curve, 7, 5000)
     msm_before = srs.msm(sc)
+    coeffs = O.gen_scalars(curve, 8, n - 3)
+    open_before = [srs.open_coeffs(coeffs, z) for z in (5, 1)]
     nbytes = srs.precompute()
     assert nbytes > 0
     for m, e in evs.items():
         assert srs.commit(e) == before[m], m
     assert srs.msm(sc) == msm_before
+    assert [srs.open_coeffs(coeffs, z) for z in (5, 1)] == open_before
     assert before[n] == O.kzg_commit_closed_form(curve, tau.to_bytes(32, "little"), evs[n])
     assert srs.precompute(9) > 0  # rebuilt with other window bits
     assert srs.commit(evs[4096]) == before[4096]
