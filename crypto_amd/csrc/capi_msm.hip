@@ -95,7 +95,7 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
         if (curve == GPUKIT_CURVE_BANDERWAGON) result[63] = 1;  // the identity (0, 1)
         return GPUKIT_OK;
     }
-    if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
+    if (n >= (size_t)1 << 32) return GPUKIT_ERR_BAD_SIZE;  // 32-bit point indices
     try {
         Engine& e = engine();
         std::lock_guard<std::mutex> g(e.mu);
@@ -130,6 +130,7 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
                              void* stream) {
     if (!valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (n > 0 && (!d_wire || !d_pts || !d_inf)) return GPUKIT_ERR_NULL_ARG;
+    if (n >= (size_t)1 << 32) return GPUKIT_ERR_BAD_SIZE;
     try {
         Engine& e = engine();
         hipStream_t s = static_cast<hipStream_t>(stream);
@@ -155,7 +156,7 @@ static int msm_prepared_impl(uint32_t curve, const void* d_scalars, const void* 
     if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
     if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
     if (chunk < 0 || chunk > 4096) return GPUKIT_ERR_BAD_SIZE;
-    if (n >= (size_t)1 << 26) return GPUKIT_ERR_BAD_SIZE;
+    if (n >= (size_t)1 << 32) return GPUKIT_ERR_BAD_SIZE;
     try {
         Engine& e = engine();
         std::lock_guard<std::mutex> g(e.mu);

@@ -29,7 +29,8 @@ struct MsmWorkspace {
     DevBuf glv_pts, glv_inf, glv_sc;  // GLV split: 2n points, flags, 16-byte scalar magnitudes
     DevBuf sort_dig;                  // batch LDS sort: 16-bit digit codes
     DevBuf out;
-    static constexpr int kBuffers = 23;
+    DevBuf slices;                    // MSMs beyond one pipeline: slice results and their sum's inputs
+    static constexpr int kBuffers = 24;
     // Every buffer becomes a piece of `a`, carved afresh on its next ensure.
     void bind(DevArena* a) {
         for (DevBuf* b : buffers()) b->bind(a);
@@ -46,7 +47,7 @@ private:
     std::array<DevBuf*, kBuffers> buffers() {
         return {&counts, &offsets, &cursor, &block_sums, &total, &entries, &bsum, &carry, &run_bucket,
                 &segA,   &segB,    &winres, &lvA[0],     &lvA[1], &lvB[0],  &lvB[1], &sort_cnt, &sort_pre,
-                &glv_pts, &glv_inf, &glv_sc, &sort_dig, &out};
+                &glv_pts, &glv_inf, &glv_sc, &sort_dig, &out, &slices};
     }
 };
 
@@ -143,6 +144,9 @@ void points_from_wire(int curve, const uint8_t* d_wire, uint32_t* d_pts, uint8_t
 
 // sum s_i P_i.  d_pts: Montgomery affine points; d_inf: identity flags or null;
 // d_scalars: n x 32-byte LE.  Writes msm_out_bytes(curve) bytes at d_out.
+// Beyond kMsmSlice points (not for window tables) it runs slices of that size
+// and sums their results.
+constexpr size_t kMsmSlice = (size_t)1 << 25;
 void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out);
 

@@ -388,8 +388,10 @@ void run_stages(const Stages& st, const Shape& sh, Sort&& sort, const MsmBatchPl
 
 }  // namespace
 
-void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
-                const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
+// One pipeline over at most kMsmSlice points (point indices and bucket
+// offsets of a pipeline are 32-bit).
+static void msm_device_one(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars,
+                           size_t n, const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
     const Stages& st = stages_of(curve);
     WorkspaceUse use(ws, s);
     // GLV (points in G1): 2n points with scalar magnitudes below 2^127
@@ -463,6 +465,56 @@ void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const ui
         }
     };
     run_stages(st, sh, sort, nullptr, nullptr, one, 1, d_pts, ws, s, d_out);
+}
+
+// Record j of the slice outputs (stride bytes apart) -> wire point j; unit
+// scalars (1, 0, ..., 0) for the sum of the slices.
+__global__ void k_slice_sum_inputs(const uint8_t* __restrict__ outs, uint32_t stride, uint32_t pb, uint32_t nslices,
+                                   uint8_t* __restrict__ wire, uint32_t* __restrict__ unit) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nslices * pb) wire[t] = outs[(size_t)(t / pb) * stride + t % pb];
+    if (t < nslices * 8) unit[t] = (t % 8 == 0) ? 1u : 0u;
+}
+
+// Larger MSMs run as slices of kMsmSlice points through the same workspace,
+// one after another on the stream; the slice results (wire points) are then
+// prepared like any input and summed by one more MSM with unit scalars, so
+// the output is the exact sum, byte for byte what one pipeline would give.
+void msm_device(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
+                const MsmParams& prm, MsmWorkspace& ws, hipStream_t s, uint8_t* d_out) {
+    if (prm.table_stride || n <= kMsmSlice) {
+        msm_device_one(curve, d_pts, d_inf, d_scalars, n, prm, ws, s, d_out);
+        return;
+    }
+    const size_t J = (n + kMsmSlice - 1) / kMsmSlice;
+    if (J > 4096) throw std::invalid_argument("MSM beyond 2^37 points");
+    const size_t ob = msm_out_bytes(curve), pb = point_wire_bytes(curve);
+    const size_t aw = (size_t)affine_words(curve);
+    // slice outputs | wire points | prepared points | flags | unit scalars | error word
+    const size_t o_wire = J * ob, o_pts = (o_wire + J * pb + 255) / 256 * 256, o_inf = o_pts + J * aw * 4;
+    const size_t o_unit = (o_inf + J + 255) / 256 * 256, o_err = o_unit + J * 32;
+    uint8_t* b = static_cast<uint8_t*>(ws.slices.ensure(o_err + 4));
+    for (size_t j = 0; j < J; ++j) {
+        const size_t first = j * kMsmSlice, len = std::min(kMsmSlice, n - first);
+        msm_device_one(curve, d_pts + first * aw, d_inf ? d_inf + first : nullptr, d_scalars + first * 8, len, prm,
+                       ws, s, b + j * ob);
+    }
+    uint8_t* wire = b + o_wire;
+    uint32_t* pts = reinterpret_cast<uint32_t*>(b + o_pts);
+    uint8_t* inf = b + o_inf;
+    uint32_t* unit = reinterpret_cast<uint32_t*>(b + o_unit);
+    const uint32_t threads = (uint32_t)std::max(J * pb, J * 8);
+    hipLaunchKernelGGL(k_slice_sum_inputs, dim3(ceil_div(threads, 256u)), dim3(256), 0, s, b, (uint32_t)ob,
+                       (uint32_t)pb, (uint32_t)J, wire, unit);
+    KZ_HIP(hipGetLastError());
+    if (curve == 3) {  // the validating reader; slice sums of valid points pass it
+        uint32_t* err = reinterpret_cast<uint32_t*>(b + o_err);
+        device_fill_u32(err, 0, 1, s);
+        bw_points_from_wire(wire, pts, inf, (uint32_t)J, err, s);
+    } else {
+        points_from_wire(curve, wire, pts, inf, J, s);
+    }
+    msm_device_one(curve, pts, inf, unit, J, MsmParams{}, ws, s, d_out);
 }
 
 int msm_table_window_bits(size_t n) {

@@ -54,11 +54,12 @@ typedef enum gpukit_curve {
  * GPUKIT_ERR_BACKEND.  This library defines no gpukit_multi_pippenger, so it
  * links beside the reference's gpukit without a duplicate symbol.
  *
- * n == 0 gives the identity.  n >= 2^26 gives GPUKIT_ERR_BAD_SIZE: sorted
- * bucket entries carry 31-bit indices and an MSM has up to 22 windows per
- * point (2^26 BLS12-381 points are 6.4 GB of wire input; the reference's CPU
- * path has no cap).  Device failures give GPUKIT_ERR_BACKEND (message via
- * kinet_hip_last_error()). */
+ * n == 0 gives the identity.  Above 2^25 points the MSM runs as slices of
+ * 2^25 points (sorted bucket entries carry 31-bit indices) whose results are
+ * summed on the device, with the same bytes as one pass; n >= 2^32 gives
+ * GPUKIT_ERR_BAD_SIZE (32-bit point indices; 2^32 BLS12-381 points would be
+ * 412 GB of wire input).  Device failures, out of device memory included,
+ * give GPUKIT_ERR_BACKEND (message via kinet_hip_last_error()). */
 int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uint8_t* points, size_t n,
                                uint8_t* result);
 

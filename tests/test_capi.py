@@ -82,7 +82,7 @@ def test_eip2537_msm_rejects_without_device_work():
 def test_hip_backend_argument_checks_without_device():
     """gpukit_multi_pippenger_hip refuses bad arguments before touching the
     device, with the reference's status codes: NULL result or inputs, an
-    unknown curve and n >= 2^26 (31-bit bucket entries); n = 0 is the identity
+    unknown curve and n >= 2^32 (32-bit point indices); n = 0 is the identity
     (all-zero bytes, and (0, 1) for Banderwagon, as the reference writes it)."""
     import crypto_amd
 
@@ -95,13 +95,13 @@ def test_hip_backend_argument_checks_without_device():
     assert lib.gpukit_multi_pippenger_hip(2, None, one, 1, out) == -1
     assert lib.gpukit_multi_pippenger_hip(4, one, one, 1, out) == -3
     assert lib.gpukit_multi_pippenger_hip(7, one, one, 1, out) == -3
-    assert lib.gpukit_multi_pippenger_hip(2, one, one, 1 << 26, out) == -2
+    assert lib.gpukit_multi_pippenger_hip(2, one, one, 1 << 32, out) == -2
     out = ctypes.create_string_buffer(b"\xee" * 96, 96)
     assert lib.gpukit_multi_pippenger_hip(2, None, None, 0, out) == 0 and out.raw == bytes(96)
     out = ctypes.create_string_buffer(b"\xee" * 64, 64)
     assert lib.gpukit_multi_pippenger_hip(3, None, None, 0, out) == 0
     assert out.raw == bytes(63) + b"\x01"
-    assert lib.gpukit_multi_pippenger_hip(3, one, one, 1 << 26, out) == -2
+    assert lib.gpukit_multi_pippenger_hip(3, one, one, 1 << 32, out) == -2
 
 
 def test_c_caller_compiles_links_and_runs(tmp_path):

@@ -20,6 +20,8 @@ import pymath as M
 
 pytestmark = pytest.mark.gpu
 
+R_BW = 0x1CFB69D4CA675F520CCE760202687600FF8F87007419047174FD06B52876E7E1  # Banderwagon group order
+
 
 @pytest.fixture(scope="module")
 def ca():
@@ -71,6 +73,48 @@ def test_msm_2p20_closed_form(ca, curve, g1):
     out = D.msm_prepared(pp, D.to_device(sc), g1=g1).cpu().numpy().tobytes()
     t = O.fr_dot(O.fr_of_curve(curve), sc, ks)
     _check_point(curve, out, O.g1_mul_gen(curve, t.to_bytes(32, "little")))
+
+
+@pytest.mark.parametrize("curve,g1,host", [(2, True, False), (1, False, True), (3, False, False)])
+def test_msm_beyond_one_pipeline(ca, curve, g1, host):
+    """n = 2^25 + 3 points: two slices (the second of 3 points) summed on the
+    device, through the prepared entries (BLS12-381 on the GLV path,
+    Banderwagon) and the host-buffer entry gpukit_multi_pippenger_hip
+    (BN254); identity points and edge scalars at both ends of the first
+    slice."""
+    from crypto_amd import device as D
+    from crypto_amd import inputs
+
+    n = (1 << 25) + 3
+    cut = 1 << 25
+    r = M.CURVES[curve]["r"] if curve != 3 else R_BW
+    ks = bytearray(inputs.scalars_bytes(curve, 3003 + curve, n))
+    sc = bytearray(inputs.scalars_bytes(curve, 4004 + curve, n))
+    for i in (0, cut - 1, cut):  # identity points (k = 0)
+        ks[32 * i:32 * i + 32] = bytes(32)
+    for i, v in ((1, r - 1), (cut - 2, 1), (cut + 1, 0), (n - 1, r - 1)):
+        sc[32 * i:32 * i + 32] = v.to_bytes(32, "little")
+    ks, sc = bytes(ks), bytes(sc)
+    print(f"sliced MSM curve={curve} n=2^25+3 g1={g1} host={host}", flush=True)
+    wire = _fixed_base_points(ca, curve, ks)
+    if host:
+        out = ca.multi_pippenger(curve, sc, wire.cpu().numpy().tobytes())
+    else:
+        pp = D.PreparedPoints(curve, wire)
+        out = D.msm_prepared(pp, D.to_device(sc), g1=g1).cpu().numpy().tobytes()
+    del wire
+    if curve == 3:
+        want = O.bw_mul_gen(O.fr_dot(2, sc, ks))
+        assert out[:64] == want
+        if not host:
+            assert out[64:96] == O.bw_compress(want)
+        return
+    t = O.fr_dot(O.fr_of_curve(curve), sc, ks)
+    want = O.g1_mul_gen(curve, t.to_bytes(32, "little"))
+    if host:
+        assert out == want
+    else:
+        _check_point(curve, out, want)
 
 
 @pytest.mark.parametrize("curve", [1, 2])
