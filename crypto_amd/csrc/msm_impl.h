@@ -255,13 +255,13 @@ k_reduce_seg(const uint32_t* __restrict__ offsets, uint32_t chunk, const uint32_
     for (int l = (int)seg - 1; l >= 0; --l) {
         const uint32_t gb = t * seg + (uint32_t)l;
         const uint32_t off = offsets[gb], end = offsets[gb + 1];
-        typename P::Pt S = P::inf();
+        // the bucket sum and its head carry go into run one after the other,
+        // so at most three points are live (run, tot and one loaded point)
         if (end > off) {
-            S = P::load(bsum, gb);
+            run = P::template add_q<MulCallLazy>(run, P::load(bsum, gb));
             const uint32_t head = off / chunk + 1;
-            if ((end - 1) / chunk >= head) S = P::template add_q<MulCallLazy>(S, P::load(carry, head));
+            if ((end - 1) / chunk >= head) run = P::template add_q<MulCallLazy>(run, P::load(carry, head));
         }
-        run = P::template add_q<MulCallLazy>(run, S);
         tot = P::template add_q<MulCallLazy>(tot, run);
     }
     if (quad_lane() == 0) {
