@@ -45,8 +45,9 @@ uint32_t kzg_srs_curve(const kzg_srs* srs);
  * Every later MSM against the SRS uses them: single commits (kzg_commit,
  * kzg_commit_coeffs, kzg_msm, kzg_commit_async) take the table whose c is
  * nearest to the window width the library picks for their size; batched and
- * served commits take, per request, a table within 2 bits of the request's
- * batched window width (else ordinary windows).  A table MSM reduces one
+ * served commits take, per request, a table within 2 bits of that same
+ * width, else one within 2 bits of the built-in width floor(log2 n) - 4,
+ * else ordinary windows.  A table MSM reduces one
  * window and combines none.  Costs R * size * 128 (BLS12-381) or 80 (BN254)
  * bytes of device memory per table: 9.1 GB for a 2^22-point BLS12-381 SRS at
  * c = 16 (kzg_srs_table_bytes gives the exact figure afterwards;
