@@ -12,6 +12,7 @@
 #include <string>
 
 #include "engine.h"
+#include "glv.h"
 #include "msm_stages.h"
 #include "scan.h"
 #include "kinet/gpukit/multi_pippenger.h"
@@ -116,6 +117,19 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
             points_from_wire((int)curve, d_wire, d_pts, d_inf, n, s);
         }
         MsmParams prm;
+        // BN254 G1 has cofactor 1: once every point is on the curve, the GLV
+        // split applies (the same bytes, fewer windows).  Off-curve input keeps
+        // the plain path.  BLS12-381 would need the subgroup check, which costs
+        // more than the split saves in one call.
+        if (curve == GPUKIT_CURVE_BN254_G1 && n >= 4096 && msm_glv_enabled(1, n)) {
+            int* d_bad = static_cast<int*>(e.err.ensure(sizeof(int)));
+            device_fill_u32(reinterpret_cast<uint32_t*>(d_bad), 0, 1, s);
+            check_g1(1, d_pts, d_inf, n, d_bad, s);
+            int bad = 1;
+            KZ_HIP(hipMemcpyAsync(&bad, d_bad, sizeof bad, hipMemcpyDeviceToHost, s));
+            KZ_HIP(hipStreamSynchronize(s));
+            prm.g1 = bad == 0;
+        }
         msm_device((int)curve, d_pts, d_inf, reinterpret_cast<const uint32_t*>(d_sc), n, prm, e.ws, s, d_out);
         KZ_HIP(hipMemcpyAsync(result, d_out, pb, hipMemcpyDeviceToHost, s));
         KZ_HIP(hipStreamSynchronize(s));

@@ -180,6 +180,27 @@ def test_fixed_base_generator(ca, curve):
         assert got[pb * i: pb * (i + 1)] == O.g1_mul_gen(curve, k.to_bytes(32, "little")), (i, k)
 
 
+def test_msm_bn254_host_entry_off_curve_point(ca):
+    """gpukit_multi_pippenger_hip runs BN254 inputs whose points all lie on the
+    curve through the GLV split (cofactor 1; test_msm_large_closed_form checks
+    those against the closed form) and any other input through the plain path.
+    With one point off the curve, carrying scalar 0, the plain path's result is
+    the closed form over the other points.  (Off-curve points with nonzero
+    scalars have no defined sum: the reference reads coordinates mod p without
+    checking them, and the addition formulas are not a group law off the
+    curve.)"""
+    n = 5000
+    k0, d = 0x1234567, 0x89ABCDEF
+    points = bytearray(O.gen_points_progression(1, k0, d, n))
+    y = int.from_bytes(points[64 * 17 + 32: 64 * 18], "big")
+    points[64 * 17 + 32: 64 * 18] = ((y + 1) % M.CURVES[1]["p"]).to_bytes(32, "big")
+    assert not O.on_curve(1, bytes(points[64 * 17: 64 * 18]))
+    scalars = bytearray(O.gen_scalars(1, 77, n))
+    scalars[32 * 17: 32 * 18] = bytes(32)
+    scalars = bytes(scalars)
+    assert ca.multi_pippenger(1, scalars, bytes(points)) == _closed_form(1, scalars, k0, d)
+
+
 def _non_g1_point(curve):
     """A point of E(Fp) outside G1 (BLS12-381 has a large cofactor)."""
     p, b = M.CURVES[curve]["p"], M.CURVES[curve]["b"]
