@@ -281,13 +281,16 @@ KZ_DEV void xyzz_madd_lazy(Xyzz<F>& acc, const Affine<F>& q) {
         return;
     }
     const Fe<F> PPP = M::template mul<F>(P, PP);
-    const Fe<F> Q = M::template mul<F>(acc.X, PP);
-    const Fe<F> X3 = fe_r6p_s_2q<F>(M::template sqr<F>(R), PPP, Q);
-    const Fe<F> Y3 = fe_sub_kp<F, 2>(M::template mul<F>(R, fe_sub_kp<F, 8>(Q, X3)), M::template mul<F>(acc.Y, PPP));
-    acc.X = X3;
-    acc.Y = Y3;
-    acc.ZZ = M::template mul<F>(acc.ZZ, PP);
     acc.ZZZ = M::template mul<F>(acc.ZZZ, PPP);
+    const Fe<F> Q = M::template mul<F>(acc.X, PP);
+    acc.ZZ = M::template mul<F>(acc.ZZ, PP);
+    const Fe<F> X3 = fe_r6p_s_2q<F>(M::template sqr<F>(R), PPP, Q);
+    const Fe<F> nPPP = fe_sub_kp<F, 2>(fe_zero<F>(), PPP);
+    // Y3 = R (Q - X3) - Y1 PPP as R (Q + 8p - X3) + Y1 (2p - PPP): one
+    // Montgomery reduction for both products (R < 6p, Q + 8p - X3 < 10p,
+    // Y1 < 4p, 2p - PPP <= 2p: the sum is far below p 2^(29N), so Y3 < 2p)
+    acc.Y = fe_mul2_lazy<F>(R, fe_sub_kp<F, 8>(Q, X3), acc.Y, nPPP);
+    acc.X = X3;
 }
 
 // a + b, general (add-2008-s), with the doubling and inverse special cases.
