@@ -843,7 +843,11 @@ def main():
                                             g1=False)
         extras[f"{CURVE_NAMES[other]}_plain_path_mpoints_s"] = round(n / oms / 1e3, 3)
         extras[f"{CURVE_NAMES[other]}_plain_path_verified"] = ores[:(96 if other == 2 else 64)] == ocheck
-        # secp256k1 (curve 0 of the multi-curve entry): plain path only
+        # secp256k1 (curve 0 of the multi-curve entry): points on the curve
+        # (cofactor 1) take the GLV split with 128-bit halves; plain beside it
+        sms, _, sres, scheck, _ = timed_msm(lib, torch, D, 0, n, 4242, max(3, args.steps // 2), 2, 0, False)
+        extras["secp256k1_mpoints_s"] = round(n / sms / 1e3, 3)
+        extras["secp256k1_verified"] = sres[:64] == scheck
         sms, _, sres, scheck, _ = timed_msm(lib, torch, D, 0, n, 4242, max(3, args.steps // 2), 2, 0, False,
                                             g1=False)
         extras["secp256k1_plain_path_mpoints_s"] = round(n / sms / 1e3, 3)

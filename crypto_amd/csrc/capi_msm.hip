@@ -43,10 +43,11 @@ Engine& engine() {
 }
 
 // Curves of the device entries: BN254 and BLS12-381 G1 everywhere; secp256k1
-// and Banderwagon only through the plain MSM (gpukit_multi_pippenger_hip,
-// points_prepare, msm_prepared), not through window tables or the G1 (GLV)
-// form.
+// through the plain MSM and the GLV form, not through window tables;
+// Banderwagon through the plain MSM only (gpukit_multi_pippenger_hip,
+// points_prepare, msm_prepared).
 static bool valid_curve(uint32_t curve) { return curve == 1 || curve == 2; }
+static bool valid_glv_curve(uint32_t curve) { return curve <= 2; }
 static bool valid_plain_curve(uint32_t curve) { return curve <= 3; }
 
 // Banderwagon's validating reader on stream s; returns the error flags
@@ -117,14 +118,15 @@ int gpukit_multi_pippenger_hip(uint32_t curve, const uint8_t* scalars, const uin
             points_from_wire((int)curve, d_wire, d_pts, d_inf, n, s);
         }
         MsmParams prm;
-        // BN254 G1 has cofactor 1: once every point is on the curve, the GLV
-        // split applies (the same bytes, fewer windows).  Off-curve input keeps
-        // the plain path.  BLS12-381 would need the subgroup check, which costs
-        // more than the split saves in one call.
-        if (curve == GPUKIT_CURVE_BN254_G1 && n >= 4096 && msm_glv_enabled(1, n)) {
+        // BN254 G1 and secp256k1 have cofactor 1: once every point is on the
+        // curve, the GLV split applies (the same bytes, fewer windows).
+        // Off-curve input keeps the plain path.  BLS12-381 would need the
+        // subgroup check, which costs more than the split saves in one call.
+        if ((curve == GPUKIT_CURVE_BN254_G1 || curve == GPUKIT_CURVE_SECP256K1) && n >= 4096 &&
+            msm_glv_enabled((int)curve, n)) {
             int* d_bad = static_cast<int*>(e.err.ensure(sizeof(int)));
             device_fill_u32(reinterpret_cast<uint32_t*>(d_bad), 0, 1, s);
-            check_g1(1, d_pts, d_inf, n, d_bad, s);
+            check_g1((int)curve, d_pts, d_inf, n, d_bad, s);
             int bad = 1;
             KZ_HIP(hipMemcpyAsync(&bad, d_bad, sizeof bad, hipMemcpyDeviceToHost, s));
             KZ_HIP(hipStreamSynchronize(s));
@@ -166,7 +168,7 @@ int kinet_hip_points_prepare(uint32_t curve, const void* d_wire, size_t n, void*
 
 static int msm_prepared_impl(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                              size_t n, int window_bits, int chunk, void* d_out, void* stream, bool g1) {
-    if (g1 ? !valid_curve(curve) : !valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
+    if (g1 ? !valid_glv_curve(curve) : !valid_plain_curve(curve)) return GPUKIT_ERR_BACKEND;
     if (!d_out || (n > 0 && (!d_scalars || !d_pts))) return GPUKIT_ERR_NULL_ARG;
     if (window_bits != 0 && (window_bits < 5 || window_bits > 18)) return GPUKIT_ERR_BAD_SIZE;
     if (chunk < 0 || chunk > 4096) return GPUKIT_ERR_BAD_SIZE;

@@ -129,6 +129,108 @@ __host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32
 }
 }  // namespace glvbn
 
+// secp256k1 GLV split (Gallant-Lambert-Vanstone 2001): s = k1 + lambda k2
+// mod n for lambda = 0x5363ad4cc05c30e0a5261c028812645a122e22ea20816678df02967c1b23bd72,
+// a cube root of unity mod n matched to beta by phi(G) = lambda G.  Basis
+// v1 = (A1, -B1), v2 = (A2, B2) of the lattice {(a, b): a + b lambda = 0 mod n}
+// from the extended Euclidean algorithm on (n, lambda), det n.  Here the
+// basis is longer than BN254's relative to n, so the Babai coefficients are
+// rounded at 2^384 precision, b_i = round(k G_i / 2^384) with
+// G1 = round(2^384 B2 / n), G2 = round(2^384 B1 / n): the rounding errors stay
+// within 1/2 + 2^-128, so |k1| <= (A1 + A2)/2 + 1 < 2^127.35 and
+// |k2| <= (B1 + B2)/2 + 1 < 2^127.12 (also checked against big integers over
+// 3e5 random scalars and edge values).  Below 2^128 but not 2^127: MSMs over
+// this split use 128-bit windows.
+//   k1 = k - b1 A1 - b2 A2,   k2 = b1 B1 - b2 B2   (k = s mod n).
+// Writes |k1|, |k2| (4 LE words each) and their signs.  Host and device.
+namespace glvsecp {
+constexpr uint32_t kN[8] = {0xD0364141u, 0xBFD25E8Cu, 0xAF48A03Bu, 0xBAAEDCE6u,
+                            0xFFFFFFFEu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+constexpr uint32_t kG1[8] = {0x45DBB031u, 0xE893209Au, 0x71E8CA7Fu, 0x3DAA8A14u,
+                             0x9284EB15u, 0xE86C90E4u, 0xA7D46BCDu, 0x3086D221u};
+constexpr uint32_t kG2[8] = {0x8AC47F71u, 0x1571B4AEu, 0x9DF506C6u, 0x221208ACu,
+                             0x0ABFE4C4u, 0x6F547FA9u, 0x010E8828u, 0xE4437ED6u};
+constexpr uint32_t kA1[4] = {0x9284EB15u, 0xE86C90E4u, 0xA7D46BCDu, 0x3086D221u};
+constexpr uint32_t kA2[5] = {0x9D44CFD8u, 0x57C1108Du, 0xA8E2F3F6u, 0x14CA50F7u, 0x00000001u};
+constexpr uint32_t kB1[4] = {0x0ABFE4C3u, 0x6F547FA9u, 0x010E8828u, 0xE4437ED6u};
+constexpr uint32_t kB2[4] = {0x9284EB15u, 0xE86C90E4u, 0xA7D46BCDu, 0x3086D221u};
+
+// round(k g / 2^384) for k and g of 8 words; below 2^128 for the G_i above
+__host__ __device__ inline void mul_round384(const uint32_t* k, const uint32_t* g, uint32_t* out4) {
+    uint32_t prod[16];
+    for (int i = 0; i < 16; ++i) prod[i] = 0;
+    for (int i = 0; i < 8; ++i) {
+        uint64_t carry = 0;
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t t = (uint64_t)k[i] * g[j] + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + 8] = (uint32_t)carry;
+    }
+    uint64_t c = (uint64_t)prod[11] + 0x80000000u;  // + 2^383
+    c >>= 32;
+    for (int i = 12; i < 16; ++i) {
+        c += prod[i];
+        out4[i - 12] = (uint32_t)c;
+        c >>= 32;
+    }
+}
+
+__host__ __device__ inline void split(const uint32_t* s_in, uint32_t* k1, uint32_t* k2, bool& neg1, bool& neg2) {
+    uint32_t k[8];
+    for (int i = 0; i < 8; ++i) k[i] = s_in[i];
+    bool ge = true;  // k mod n: 2^256 < 2 n
+    for (int i = 7; i >= 0; --i)
+        if (k[i] != kN[i]) {
+            ge = k[i] > kN[i];
+            break;
+        }
+    if (ge) {
+        uint64_t borrow = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t t = (uint64_t)k[i] - kN[i] - borrow;
+            k[i] = (uint32_t)t;
+            borrow = (t >> 32) & 1u;
+        }
+    }
+    uint32_t b1[4], b2[4];
+    mul_round384(k, kG1, b1);
+    mul_round384(k, kG2, b2);
+    // k1 = k - (b1 A1 + b2 A2)  (mod 2^256; the value lies in (-2^128, 2^128))
+    uint32_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    glvbn::mac8<4, 4>(t, b1, kA1);
+    glvbn::mac8<4, 5>(t, b2, kA2);
+    glvbn::neg8(t);
+    uint32_t x[8];
+    {
+        uint64_t c = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t v = (uint64_t)k[i] + t[i] + c;
+            x[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+    }
+    neg1 = (x[7] >> 31) != 0;
+    if (neg1) glvbn::neg8(x);
+    for (int i = 0; i < 4; ++i) k1[i] = x[i];
+    // k2 = b1 B1 - b2 B2
+    uint32_t u[8] = {0, 0, 0, 0, 0, 0, 0, 0}, v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    glvbn::mac8<4, 4>(u, b1, kB1);
+    glvbn::mac8<4, 4>(v, b2, kB2);
+    glvbn::neg8(v);
+    uint64_t c = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t w = (uint64_t)u[i] + v[i] + c;
+        x[i] = (uint32_t)w;
+        c = w >> 32;
+    }
+    neg2 = (x[7] >> 31) != 0;
+    if (neg2) glvbn::neg8(x);
+    for (int i = 0; i < 4; ++i) k2[i] = x[i];
+}
+}  // namespace glvsecp
+
 // BLS12-381 GLV split: lambda = 0xac45a4010001a40200000000ffffffff with r =
 // lambda^2 + lambda + 1.  For s mod r = k1 + lambda k2 (k1 = s mod lambda,
 // k2 = s div lambda <= lambda + 1, both near 2^127.4), two exact moves make

@@ -16,7 +16,9 @@
 // passed k_check_g1: on the curve, and phi(P) = lambda P, which for BLS12
 // curves holds exactly for the points of G1 (Scott, "A note on group
 // membership tests for G1, G2 and GT on BLS pairing-friendly curves", 2021).
-// BN254 G1 has cofactor 1: its check is the curve equation alone.
+// BN254 G1 and secp256k1 have cofactor 1: their check is the curve equation
+// alone.  (secp256k1 is curve 0 of the multi-curve entry; its split is
+// glvsecp's, with magnitudes below 2^128.)
 #include <type_traits>
 
 #include "curve.h"
@@ -33,6 +35,10 @@ namespace {
 // beta = 0x30644e72e131a0295e6dd9e7e0acccb0c28f069fbb966e3de4bd44e5607cfd48
 constexpr uint32_t kBetaMontBn[9] = {0x18CCB791u, 0x175B1C3Au, 0x0B83D6E2u, 0x0E8ED071u, 0x1282BEE2u,
                                      0x04220E84u, 0x1FE4017Fu, 0x15084D4Au, 0x00169119u};
+// secp256k1: beta R mod p (radix 2^29, R = 2^290) for
+// beta = 0x7ae96a2b657c07106e64479eac3434e99cf0497512f58995c1396c28719501ee
+constexpr uint32_t kBetaMontSecp[10] = {0x14EABFDAu, 0x197839E6u, 0x04361D08u, 0x1C057AC5u, 0x1858C712u,
+                                        0x120A07FBu, 0x178D02E3u, 0x09FC74C4u, 0x00F2ECF5u, 0x00000000u};
 // BLS12-381: beta R mod p (radix 2^29, R = 2^406)
 constexpr uint32_t kBetaMont[14] = {0x1195DFEBu, 0x1B04E484u, 0x06026044u, 0x086070A2u, 0x1FD68858u,
                                     0x137E9670u, 0x06871E67u, 0x1E736664u, 0x083B24F6u, 0x08A70373u,
@@ -45,7 +51,7 @@ template <int CURVE>
 __global__ void __launch_bounds__(256)
 k_glv_expand(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, const uint32_t* __restrict__ scalars,
              uint32_t n, uint32_t* __restrict__ out_pts, uint8_t* __restrict__ out_inf, uint32_t* __restrict__ out_sc) {
-    using F = std::conditional_t<CURVE == 1, BnFp, BlsFp>;
+    using F = std::conditional_t<CURVE == 0, SecpFp, std::conditional_t<CURVE == 1, BnFp, BlsFp>>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t s[8];
@@ -57,13 +63,15 @@ k_glv_expand(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, 
     }
     uint32_t k1[4], k2[4];
     bool neg1, neg2;
-    if constexpr (CURVE == 1) glvbn::split(s, k1, k2, neg1, neg2);
+    if constexpr (CURVE == 0) glvsecp::split(s, k1, k2, neg1, neg2);
+    else if constexpr (CURVE == 1) glvbn::split(s, k1, k2, neg1, neg2);
     else glvbls::split(s, k1, k2, neg1, neg2);
     Affine<F> a = affine_load<F>(pts, i);
     const Fe<F> y = a.y, ny = fe_neg<F>(a.y);
     a.y = neg1 ? ny : y;
     affine_store<F>(out_pts, i, a);
-    if constexpr (CURVE == 1) a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontBn));
+    if constexpr (CURVE == 0) a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontSecp));
+    else if constexpr (CURVE == 1) a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMontBn));
     else a.x = fe_mul<F>(a.x, fe_const<F>(kBetaMont));
     a.y = neg2 ? ny : y;
     affine_store<F>(out_pts, (size_t)n + i, a);
@@ -108,7 +116,10 @@ k_check_g1(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, ui
 void glv_expand(int curve, const uint32_t* d_pts, const uint8_t* d_inf, const uint32_t* d_scalars, size_t n,
                 uint32_t* d_pts2, uint8_t* d_inf2, uint32_t* d_scalars2, hipStream_t s) {
     const unsigned grid = ceil_div(n, 256);
-    if (curve == 1)
+    if (curve == 0)
+        hipLaunchKernelGGL(k_glv_expand<0>, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
+                           d_inf2, d_scalars2);
+    else if (curve == 1)
         hipLaunchKernelGGL(k_glv_expand<1>, dim3(grid), dim3(256), 0, s, d_pts, d_inf, d_scalars, (uint32_t)n, d_pts2,
                            d_inf2, d_scalars2);
     else if (curve == 2)
@@ -124,8 +135,10 @@ void check_g1(int curve, const uint32_t* d_pts, const uint8_t* d_inf, size_t n, 
     const unsigned grid = ceil_div(n, 64);
     if (curve == 2)
         hipLaunchKernelGGL(k_check_g1<CurveBLS12381>, dim3(grid), dim3(64), 0, s, d_pts, d_inf, (uint32_t)n, d_bad);
-    else
+    else if (curve == 1)
         hipLaunchKernelGGL(k_check_g1<CurveBN254>, dim3(grid), dim3(64), 0, s, d_pts, d_inf, (uint32_t)n, d_bad);
+    else
+        hipLaunchKernelGGL(k_check_g1<CurveSecp256k1>, dim3(grid), dim3(64), 0, s, d_pts, d_inf, (uint32_t)n, d_bad);
     KZ_HIP(hipGetLastError());
 }
 
@@ -137,7 +150,7 @@ extern "C" {
 
 int kinet_hip_points_check_g1(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int* d_bad,
                               void* stream) {
-    if (curve != 1 && curve != 2) return GPUKIT_ERR_BACKEND;
+    if (curve > 2) return GPUKIT_ERR_BACKEND;
     if (!d_bad || (n > 0 && !d_pts)) return GPUKIT_ERR_NULL_ARG;
     try {
         (void)engine();

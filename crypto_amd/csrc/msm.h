@@ -184,9 +184,9 @@ struct MsmBatchPlan {
 enum MsmMode { kMsmPlain = 0, kMsmBatched = 1, kMsmGlv = 2, kMsmModes = 3 };
 
 // (curve, mode) pairs with a tuning table: every mode of BN254 (1) and
-// BLS12-381 (2); secp256k1 (0) runs single plain MSMs only.
+// BLS12-381 (2); secp256k1 (0) runs single MSMs only, plain or GLV.
 inline bool msm_tunable(int curve, int mode) {
-    return mode >= 0 && mode < kMsmModes && (curve == 1 || curve == 2 || (curve == 0 && mode == kMsmPlain));
+    return mode >= 0 && mode < kMsmModes && (curve == 1 || curve == 2 || (curve == 0 && mode != kMsmBatched));
 }
 // A GLV cell holding kMsmGlvOff instead of window bits records that the
 // plain MSM beat the split at that size class: G1 MSMs of that class without

@@ -23,9 +23,10 @@
 
 namespace kz {
 
-// Windows of c-bit signed digits over BITS-bit scalars (256, or 127 for the
-// magnitudes of a signed GLV split), with the extra window for the final
-// carry when c divides BITS (never for 127).
+// Windows of c-bit signed digits over BITS-bit scalars (256, or 127 / 128 for
+// the magnitudes of a signed GLV split: BN254 and BLS12-381 / secp256k1),
+// with the extra window for the final carry when c divides BITS (never for
+// 127).
 template <int CB, int BITS = 256>
 struct Digits {
     static constexpr int W0 = (BITS + CB - 1) / CB;
@@ -260,15 +261,19 @@ static void lds_sort_launch(const uint32_t* d_scalars, const uint8_t* d_inf, uin
 void lds_sort_dispatch(int c, int bits, const uint32_t* d_scalars, const uint8_t* d_inf, uint32_t n,
                        uint32_t table_stride, uint32_t* cnt, uint32_t* pre, uint32_t* tot, uint32_t* offsets,
                        uint32_t* block_sums, uint32_t* total, uint32_t* entries, hipStream_t s) {
-    if (bits != 256 && bits != 127) throw std::invalid_argument("LDS sort: scalar bits must be 256 or 127");
+    if (bits != 256 && bits != 127 && bits != 128)
+        throw std::invalid_argument("LDS sort: scalar bits must be 256, 127 or 128");
     switch (c) {
 #define KZ_C(CB)                                                                                                     \
     case CB:                                                                                                         \
         if (bits == 256)                                                                                             \
             lds_sort_launch<CB, 256>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
                                      entries, s);                                                                    \
-        else                                                                                                         \
+        else if (bits == 127)                                                                                        \
             lds_sort_launch<CB, 127>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
+                                     entries, s);                                                                    \
+        else                                                                                                         \
+            lds_sort_launch<CB, 128>(d_scalars, d_inf, n, table_stride, cnt, pre, tot, offsets, block_sums, total,  \
                                      entries, s);                                                                    \
         break;
         KZ_C(5) KZ_C(6) KZ_C(7) KZ_C(8) KZ_C(9) KZ_C(10) KZ_C(11) KZ_C(12) KZ_C(13) KZ_C(14) KZ_C(15) KZ_C(16)

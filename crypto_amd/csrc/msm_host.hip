@@ -60,7 +60,7 @@ int msm_window_bits(int curve, size_t n, int mode) {
 
 bool msm_glv_enabled(int curve, size_t n) {
     const int lg = floor_log2(n);
-    if (curve < 1 || curve > 2 || lg >= 32) return true;
+    if (curve < 0 || curve > 2 || lg >= 32) return true;
     std::lock_guard<std::mutex> g(g_table_mu);
     return g_table[curve][kMsmGlv][lg].c != kMsmGlvOff;
 }
@@ -395,6 +395,7 @@ static void msm_device_one(int curve, const uint32_t* d_pts, const uint8_t* d_in
     const Stages& st = stages_of(curve);
     WorkspaceUse use(ws, s);
     // GLV (points in G1): 2n points with scalar magnitudes below 2^127
+    // (BN254, BLS12-381) or 2^128 (secp256k1)
     int bits = 256;
     size_t npts = n;
     int c = prm.c;
@@ -408,11 +409,13 @@ static void msm_device_one(int curve, const uint32_t* d_pts, const uint8_t* d_in
     if (prm.g1 && !table && n <= (1u << 29) && (c || msm_glv_enabled(curve, n))) {
         const int cg = c ? c : msm_window_bits(curve, n, kMsmGlv);
         if (lds_sort_supported(cg, 2 * n)) {
-            // |k1|, |k2| < 2^127 (glv.h): 127-bit windows.  The top window then
-            // holds t < c real bits (c never divides the prime 127), so its digit
-            // plus an incoming carry is at most 2^t <= 2^(c-1) and never carries
-            // out: no carry window (8 windows at c = 16 instead of 9).
-            bits = 127;
+            // |k1|, |k2| < 2^127 for BN254 and BLS12-381 (glv.h): 127-bit
+            // windows.  The top window then holds t < c real bits (c never
+            // divides the prime 127), so its digit plus an incoming carry is at
+            // most 2^t <= 2^(c-1) and never carries out: no carry window (8
+            // windows at c = 16 instead of 9).  secp256k1's halves reach
+            // 2^127.35: 128-bit windows, with the carry window when c divides 128.
+            bits = curve == 0 ? 128 : 127;
             npts = 2 * n;
             c = cg;
             const size_t pw = (size_t)affine_words(curve) * 4;

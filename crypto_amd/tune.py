@@ -1,7 +1,7 @@
 """Schedule tables of the HIP library and its autotuner.
 
 Schedules are chosen per size class k = floor(log2 n): for MSMs per curve and
-mode (single, batched job, GLV single over G1 points; secp256k1 single only) the window bits c and
+mode (single, batched job, GLV single over G1 points; secp256k1 single and GLV) the window bits c and
 the first-level bucket-reduction segment (a GLV cell of window bits 1: the
 plain MSM beat the split there and runs instead); for the Fr NTT per field the split
 of log2 n into one to three LDS passes.  A schedule changes speed only: the
@@ -84,6 +84,7 @@ def tune_all(log_min: int = 10, log_max: int = 22, ntt_min: int = 4, ntt_max: in
     for curve in (2, 1):
         jobs.append((f"msm curve={curve} glv", lambda c=curve: tune_msm(c, MODE_GLV, max(11, log_min), log_max)))
     jobs.append(("msm curve=0 single", lambda: tune_msm(0, MODE_SINGLE, log_min, log_max)))  # secp256k1
+    jobs.append(("msm curve=0 glv", lambda: tune_msm(0, MODE_GLV, max(11, log_min), log_max)))
     for field in (0, 1):
         jobs.append((f"ntt field={field}", lambda f=field: tune_ntt(f, ntt_min, ntt_max)))
     for name, fn in jobs:

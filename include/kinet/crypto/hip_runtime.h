@@ -75,9 +75,11 @@ int kinet_hip_msm_prepared_noalloc(uint32_t curve, const void* d_scalars, const 
  * split as s = k1 + lambda k2 mod r with |k1|, |k2| < 2^127 (BLS12-381: s mod
  * r divided by the 128-bit lambda, then moved by r = lambda^2 + lambda + 1
  * and by lambda to signed parts; BN254: a signed split over a short lattice
- * basis), the signs moved onto the points, giving a 2n-point MSM over half
- * the windows.  Results equal kinet_hip_msm_prepared's for points of G1; for
- * other points they are unspecified. */
+ * basis) or < 2^128 (secp256k1, curve 0: the same kind of lattice split; its
+ * group is the whole curve, so every curve point qualifies), the signs moved
+ * onto the points, giving a 2n-point MSM over half the windows.  Results equal
+ * kinet_hip_msm_prepared's for points of G1; for other points they are
+ * unspecified.  Curves 0, 1, 2. */
 int kinet_hip_msm_prepared_g1(uint32_t curve, const void* d_scalars, const void* d_pts, const void* d_inf,
                               size_t n, int window_bits, int chunk, void* d_out, void* stream);
 
@@ -108,7 +110,8 @@ int kinet_hip_msm_table(uint32_t curve, const void* d_scalars, const void* d_tab
 
 /* Sets *d_bad (device int, zeroed by the caller) nonzero if any non-identity
  * prepared point is off the curve or, for BLS12-381, outside G1 (tested as
- * phi(P) = lambda P).  Asynchronous on `stream`. */
+ * phi(P) = lambda P).  Curves 0 (secp256k1), 1, 2.  Asynchronous on
+ * `stream`. */
 int kinet_hip_points_check_g1(uint32_t curve, const void* d_pts, const void* d_inf, size_t n, int* d_bad,
                               void* stream);
 

@@ -54,10 +54,10 @@ typedef enum gpukit_curve {
  * GPUKIT_ERR_BACKEND.  This library defines no gpukit_multi_pippenger, so it
  * links beside the reference's gpukit without a duplicate symbol.
  *
- * BN254 points that all lie on the curve take the GLV split (cofactor 1:
- * same bytes, faster).  As in the reference, points off the curve are not
- * rejected (BN254, BLS12-381), and their sum is undefined: the result for such
- * input may differ between calls.
+ * BN254 and secp256k1 points that all lie on the curve (n >= 4096) take the
+ * GLV split (cofactor 1: same bytes, faster).  As in the reference, points off
+ * the curve are not rejected (BN254, BLS12-381, secp256k1), and their sum is
+ * undefined: the result for such input may differ between calls.
  * n == 0 gives the identity.  Above 2^25 points the MSM runs as slices of
  * 2^25 points (sorted bucket entries carry 31-bit indices) whose results are
  * summed on the device, with the same bytes as one pass; n >= 2^32 gives

@@ -634,6 +634,19 @@ void hc_glv_split_bn(const uint8_t* s32, uint8_t* out) {
     out[33] = n2 ? 1 : 0;
 }
 
+void hc_glv_split_secp(const uint8_t* s32, uint8_t* out) {
+    uint32_t s[8] = {0}, k1[4], k2[4];
+    for (int k = 0; k < 32; ++k) s[k / 4] |= (uint32_t)s32[k] << (8 * (k % 4));
+    bool n1, n2;
+    glvsecp::split(s, k1, k2, n1, n2);
+    for (int k = 0; k < 16; ++k) {
+        out[k] = (uint8_t)(k1[k / 4] >> (8 * (k % 4)));
+        out[16 + k] = (uint8_t)(k2[k / 4] >> (8 * (k % 4)));
+    }
+    out[32] = n1 ? 1 : 0;
+    out[33] = n2 ? 1 : 0;
+}
+
 void hc_glv_split_bls(const uint8_t* s32, uint8_t* out) {
     uint32_t s[8] = {0}, k1[4], k2[4];
     for (int k = 0; k < 32; ++k) s[k / 4] |= (uint32_t)s32[k] << (8 * (k % 4));

@@ -146,6 +146,42 @@ def test_glv_split_bn(hc):
 
 
 
+SECP_LAMBDA = 0x5363AD4CC05C30E0A5261C028812645A122E22EA20816678DF02967C1B23BD72
+SECP_BETA = 0x7AE96A2B657C07106E64479EAC3434E99CF0497512F58995C1396C28719501EE
+
+
+def test_glv_split_secp(hc):
+    """The secp256k1 GLV split (host build): s = k1 + lambda k2 mod n with
+    |k1| <= (A1 + A2)/2 + 1 and |k2| <= (B1 + B2)/2 + 1, both below 2^128, for
+    random and edge scalars (0, n - 1, n, 2^256 - 1, lambda and the basis
+    entries); lambda is a cube root of unity mod n with lambda G = (beta x_G, y_G),
+    and the basis vectors lie in the lattice {(a, b): a + b lambda = 0 mod n}
+    with determinant n."""
+    n = M.CURVES[0]["r"]
+    p = M.CURVES[0]["p"]
+    lam = SECP_LAMBDA
+    assert pow(lam, 3, n) == 1 and lam != 1 and pow(SECP_BETA, 3, p) == 1
+    G = M.CURVES[0]["g"]
+    assert M.mul(0, G, lam) == (SECP_BETA * G[0] % p, G[1])
+    a1, b1 = 0x3086D221A7D46BCDE86C90E49284EB15, 0xE4437ED6010E88286F547FA90ABFE4C3
+    a2, b2 = 0x114CA50F7A8E2F3F657C1108D9D44CFD8, 0x3086D221A7D46BCDE86C90E49284EB15
+    assert (a1 - b1 * lam) % n == 0 and (a2 + b2 * lam) % n == 0 and a1 * b2 + b1 * a2 == n
+    lim1, lim2 = (a1 + a2) // 2 + 1, (b1 + b2) // 2 + 1
+    rnd = random.Random(7)
+    out = ctypes.create_string_buffer(34)
+    edge = [0, 1, 2, n - 1, n, n + 1, (1 << 256) - 1, lam, n - lam, a1, a2, b1, b2, (n - 1) // 2, n // 3]
+    max_bits = 0
+    for s in edge + [rnd.getrandbits(256) for _ in range(4000)] + [rnd.randrange(n) for _ in range(2000)]:
+        hc.hc_glv_split_secp(s.to_bytes(32, "little"), out)
+        raw = out.raw
+        k1 = int.from_bytes(raw[:16], "little") * (-1 if raw[32] else 1)
+        k2 = int.from_bytes(raw[16:32], "little") * (-1 if raw[33] else 1)
+        assert abs(k1) <= lim1 and abs(k2) <= lim2, hex(s)
+        assert (k1 + lam * k2 - s) % n == 0, hex(s)
+        max_bits = max(max_bits, abs(k1).bit_length(), abs(k2).bit_length())
+    assert max_bits <= 128
+
+
 def test_glv_split_bls(hc):
     """The BLS12-381 signed GLV split (host build): s = k1 + lambda k2 mod r with
     |k1|, |k2| < 2^127 (the magnitude bound that keeps 16-bit windows free of

@@ -115,9 +115,11 @@ def test_secp_msm_empty_and_refusals(ca):
     assert ca.multi_pippenger(0, b"", b"") == bytes(64)
     assert ca.multi_pippenger(0, (3).to_bytes(32, "little"), M.to_wire(0, G)) == M.to_wire(0, M.mul(0, G, 3))
     lib = ca.lib.require_device()
-    # window tables and the G1 (GLV) form are BN254 / BLS12-381 only
+    # window tables are BN254 / BLS12-381 only; the G1 (GLV) form takes
+    # secp256k1 (a missing output is then a null argument, not a bad curve)
     assert lib.kinet_hip_table_bytes(0, 16, 8) == 0
-    assert lib.kinet_hip_msm_prepared_g1(0, None, None, None, 0, 0, 0, None, None) == -3
+    assert lib.kinet_hip_msm_prepared_g1(0, None, None, None, 0, 0, 0, None, None) == -1
+    assert lib.kinet_hip_msm_prepared_g1(4, None, None, None, 0, 0, 0, None, None) == -3
     with pytest.raises(ca.KinetError) as e:
         ca.multi_pippenger(4, bytes(32), bytes(64))
     assert e.value.code == -3
@@ -139,3 +141,66 @@ def test_secp_fixed_base(ca):
     for i, k in enumerate(ks):
         want = M.to_wire(0, M.mul(0, G, k % N)) if k % N else bytes(64)
         assert got[64 * i:64 * i + 64] == want, k
+
+
+LAMBDA = 0x5363AD4CC05C30E0A5261C028812645A122E22EA20816678DF02967C1B23BD72
+
+
+def _glv_inputs(n, k0, d, seed):
+    """Progression points with identities every 211th, and scalars 0, 1,
+    n - 1, n, n + 1, 2^256 - 1, lambda, lambda^2, -lambda, 2 lambda,
+    2^256 - n, (n + 1) / 2 and random ones (about a fifth of the random ones have a
+    GLV half of 2^127 or more, so the 128th bit is exercised)."""
+    pl = bytearray(_points(n, k0, d))
+    zero_at = set(range(3, n, 211))
+    for i in zero_at:
+        pl[64 * i:64 * i + 64] = bytes(64)
+    rnd = random.Random(seed)
+    sc = [rnd.getrandbits(256) for _ in range(n)]
+    sc[:12] = [0, 1, N - 1, N, N + 1, (1 << 256) - 1, LAMBDA, LAMBDA * LAMBDA % N, N - LAMBDA, 2 * LAMBDA,
+               (1 << 256) - N, (N + 1) // 2]
+    scalars = b"".join(x.to_bytes(32, "little") for x in sc)
+    t = sum(x * ((k0 + i * d) % N) for i, x in enumerate(sc) if i not in zero_at) % N
+    want = M.to_wire(0, M.mul(0, G, t)) if t else bytes(64)
+    return bytes(pl), scalars, want
+
+
+@pytest.mark.parametrize("n,c", [(2048, 0), (5000, 0), (5000, 8), (5000, 9), (5000, 16), (1 << 16, 0),
+                                 (1 << 16, 13)])
+def test_secp_msm_glv(ca, n, c):
+    """secp256k1 through the signed GLV split (kinet_hip_msm_prepared_g1,
+    phi(x, y) = (beta x, y) = lambda (x, y), halves below 2^128 in 128-bit
+    windows; c = 8 and 16 take the carry window): the plain path's bytes and
+    the closed form sum s_i (k0 + i d) G."""
+    from crypto_amd import device as D
+
+    k0, d = 0x5EC9256B1, 0xF00DFACE12345
+    points, scalars, want = _glv_inputs(n, k0, d, n * 7 + c)
+    pp = D.PreparedPoints(0, points)
+    assert D.check_g1(pp)
+    d_sc = D.to_device(scalars)
+    print(f"secp256k1 glv n={n} c={c}", flush=True)
+    got = D.msm_prepared(pp, d_sc, window_bits=c, g1=True).cpu().numpy().tobytes()
+    plain = D.msm_prepared(pp, d_sc).cpu().numpy().tobytes()
+    assert got[:64] == want
+    assert got == plain
+
+
+def test_secp_host_entry_glv_and_off_curve(ca):
+    """gpukit_multi_pippenger_hip on secp256k1 at n >= 4096 checks the points
+    and runs the GLV split when all lie on the curve (cofactor 1): the closed
+    form.  With one point off the curve (scalar 0) it keeps the plain path and
+    still gives the closed form; check_g1 flags that point."""
+    from crypto_amd import device as D
+
+    n, k0, d = 6000, 0x1234567, 0xABCDEF0123
+    points, scalars, want = _glv_inputs(n, k0, d, 99)
+    print("secp256k1 host entry glv", flush=True)
+    assert ca.multi_pippenger(0, scalars, points) == want
+    bad = bytearray(points)
+    bad[64 * 3 + 63] ^= 1  # point 3 (an identity) becomes (0, 1), off the curve
+    sc = bytearray(scalars)
+    sc[32 * 3:32 * 4] = bytes(32)
+    assert not D.check_g1(D.PreparedPoints(0, bytes(bad)))
+    print("secp256k1 host entry off-curve point", flush=True)
+    assert ca.multi_pippenger(0, bytes(sc), bytes(bad)) == want

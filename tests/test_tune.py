@@ -56,12 +56,22 @@ def test_glv_off_cell(T, tmp_path):
     assert "msm 2 2 22 1 0" in q.read_text().splitlines()
 
 
+def test_secp_glv_cells(T, tmp_path):
+    """secp256k1 (curve 0) has single and GLV cells, no batched ones."""
+    p = tmp_path / "s.tune"
+    p.write_text("msm 0 0 20 15 0\nmsm 0 2 20 14 4\nmsm 0 2 21 1 0\n")
+    T.load(str(p))
+    assert T.msm_cell(0, 2, 20) == (14, 4)
+    assert T.window_bits(0, 1 << 20, T.MODE_GLV) == 14
+    assert T.window_bits(0, 1 << 21, T.MODE_GLV) == T.window_bits(0, 1 << 21)  # the GLV-off cell
+
+
 @pytest.mark.parametrize("line", ["msm 2 0 20 19 8", "msm 2 0 20 4 8", "msm 2 0 20 16 3", "msm 2 0 20 16 32",
                                   "msm 3 0 20 16 8", "msm 2 3 20 16 8", "msm 2 0 32 16 8", "msm 2 0 20 16",
                                   "msm 2 0 20 16 8 1", "ntt 0 20 10 11 0", "ntt 0 20 12 8 0", "ntt 0 20 0 20 0",
                                   "ntt 0 12 4 0 8", "ntt 2 12 6 6 0", "ntt 0 27 9 9 9", "ntx 0 12 6 6 0",
                                   "msm two 0 20 16 8", "msm 2 0 20 1 0", "msm 2 1 20 1 0",
-                                  "msm 0 1 16 10 0", "msm 0 2 16 10 0", "msm -1 0 16 10 0"])
+                                  "msm 0 1 16 10 0", "msm 3 2 16 10 0", "msm -1 0 16 10 0"])
 def test_malformed_lines_install_nothing(T, tmp_path, line):
     import crypto_amd
 
