@@ -1,5 +1,5 @@
-// secp256k1: bucket-reduction levels, window combine and window-table rows
-// (msm_impl.h) for the plain MSM of the multi-curve entry.
+// secp256k1: bucket-reduction levels and window-table rows (msm_impl.h) for
+// the multi-curve entry; the window combine is in msm_combine_row.hip.
 #include "msm_impl.h"
 #include "msm_stages.h"
 
@@ -12,12 +12,6 @@ void launch_reduce_level_secp(unsigned grid, hipStream_t s, const uint32_t* inA,
                               uint32_t* outA, uint32_t* outB) {
     hipLaunchKernelGGL(k_reduce_level<Secp::Fp>, dim3(grid), dim3(kLevelThreads), 0, s, inA, inB, tab, ntab, uni_n,
                        uni_parts, lm, outA, outB);
-}
-
-void launch_combine_secp(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,
-                         uint32_t njobs, uint8_t* out) {
-    hipLaunchKernelGGL(k_combine<Secp>, dim3(ceil_div((size_t)njobs * 4, 64)), dim3(64), 0, s, winres, jobs, one,
-                       njobs, out);
 }
 
 void launch_table_rows_secp(const uint32_t* pts, const uint8_t* inf, uint32_t n, uint32_t c, uint32_t nwin,

@@ -1,14 +1,9 @@
-// BLS12-381 G1: window combine, normalisation and compression kernels (msm_impl.h).
+// BLS12-381 G1: window-table rows and compression kernels (msm_impl.h); the
+// window combine is in msm_combine_row.hip.
 #include "msm_impl.h"
 #include "msm_stages.h"
 
 namespace kz {
-
-void launch_combine_bls(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one,
-                        uint32_t njobs, uint8_t* out) {
-    hipLaunchKernelGGL(k_combine<CurveBLS12381>, dim3(ceil_div((size_t)njobs * 4, 64)), dim3(64), 0, s, winres,
-                       jobs, one, njobs, out);
-}
 
 void launch_table_rows_bls(const uint32_t* pts, const uint8_t* inf, uint32_t n, uint32_t c, uint32_t nwin,
                           uint32_t* table, hipStream_t s) {

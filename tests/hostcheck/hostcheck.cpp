@@ -13,6 +13,7 @@
 #include "curve.h"
 #include "curve_ed.h"
 #include "curve_quad.h"
+#include "field_row.h"
 #include "glv.h"
 #include "ntt_math.h"
 #include "serve_core.h"
@@ -253,6 +254,171 @@ int quad_dbl_chain(const uint8_t* a, const uint8_t* za, const int* ka, int count
     }, r);
     write_wire<C>(xyzz_canon_acc<F>(r), out);
     return same ? 0 : -1;
+}
+
+// ---------------------------------------------------------------- rows
+// A 64-lane wave in host arrays with the lane operations of WaveDev
+// (field_row.h): the row-distributed product and point operations run
+// unchanged.  DPP row semantics as WaveDev documents them.
+struct WaveHost {
+    struct V {
+        uint32_t l[64];
+    };
+    struct V64 {
+        uint64_t l[64];
+    };
+    struct B {
+        bool l[64];
+    };
+    template <class T, class Fn>
+    static T map(Fn fn) {
+        T r;
+        for (int i = 0; i < 64; ++i) r.l[i] = fn(i);
+        return r;
+    }
+    static V lane16() { return map<V>([](int i) { return (uint32_t)(i & 15); }); }
+    static V row() { return map<V>([](int i) { return (uint32_t)((i >> 4) & 3); }); }
+    static V c(uint32_t x) { return map<V>([&](int) { return x; }); }
+    static V add(V a, V b) { return map<V>([&](int i) { return a.l[i] + b.l[i]; }); }
+    static V sub(V a, V b) { return map<V>([&](int i) { return a.l[i] - b.l[i]; }); }
+    static V and_(V a, uint32_t m) { return map<V>([&](int i) { return a.l[i] & m; }); }
+    static V shl(V a, int s) { return map<V>([&](int i) { return a.l[i] << s; }); }
+    static V mulc(V a, uint32_t k) { return map<V>([&](int i) { return a.l[i] * k; }); }
+    static V sra(V a, int s) { return map<V>([&](int i) { return (uint32_t)((int32_t)a.l[i] >> s); }); }
+    static V64 wide(V a) { return map<V64>([&](int i) { return (uint64_t)a.l[i]; }); }
+    static V lo(V64 a) { return map<V>([&](int i) { return (uint32_t)a.l[i]; }); }
+    static V hi(V64 a) { return map<V>([&](int i) { return (uint32_t)(a.l[i] >> 32); }); }
+    static V64 mk64(V l, V h) { return map<V64>([&](int i) { return ((uint64_t)h.l[i] << 32) | l.l[i]; }); }
+    static V64 add64(V64 a, V64 b) { return map<V64>([&](int i) { return a.l[i] + b.l[i]; }); }
+    static V64 shr64(V64 a, int s) { return map<V64>([&](int i) { return a.l[i] >> s; }); }
+    static V64 mad(V a, V b, V64 c) { return map<V64>([&](int i) { return (uint64_t)a.l[i] * b.l[i] + c.l[i]; }); }
+    static V64 madc(V a, uint32_t b, V64 c) { return map<V64>([&](int i) { return (uint64_t)a.l[i] * b + c.l[i]; }); }
+    static B eqc(V a, uint32_t k) { return map<B>([&](int i) { return a.l[i] == k; }); }
+    static B ltc(V a, uint32_t k) { return map<B>([&](int i) { return a.l[i] < k; }); }
+    static B gtc(V a, uint32_t k) { return map<B>([&](int i) { return a.l[i] > k; }); }
+    static B ne(V a, V b) { return map<B>([&](int i) { return a.l[i] != b.l[i]; }); }
+    static B or_(B a, B b) { return map<B>([&](int i) { return a.l[i] || b.l[i]; }); }
+    static V sel(B b, V x, V y) { return map<V>([&](int i) { return b.l[i] ? x.l[i] : y.l[i]; }); }
+    static V64 sel64(B b, V64 x, V64 y) { return map<V64>([&](int i) { return b.l[i] ? x.l[i] : y.l[i]; }); }
+    template <int K>
+    static V bcast(V x) {
+        return map<V>([&](int i) { return x.l[(i & ~15) | K]; });
+    }
+    template <int S>
+    static V ror(V x) {
+        return map<V>([&](int i) { return x.l[(i & ~15) | (((i & 15) - S) & 15)]; });
+    }
+    template <int S>
+    static V shr(V x) {
+        return map<V>([&](int i) { return (i & 15) < S ? 0u : x.l[i - S]; });
+    }
+    template <int K>
+    static V rowread(V x) {
+        return map<V>([&](int i) { return x.l[(K << 4) | (i & 15)]; });
+    }
+    static bool any(B b) {
+        for (int i = 0; i < 64; ++i)
+            if (b.l[i]) return true;
+        return false;
+    }
+    static uint32_t readlane(V x, int l) { return x.l[l]; }
+};
+
+// Every row holds the same limbs, and lanes N..15 hold zero.
+template <class F>
+bool row_consistent(const FeRow<WaveHost, F>& a) {
+    for (int i = 0; i < 64; ++i) {
+        if ((i & 15) >= F::N && a.v.l[i]) return false;
+        if (a.v.l[i] != a.v.l[i & 15]) return false;
+    }
+    return true;
+}
+
+// Four products at once (row q: a[16 q ..] times b[16 q ..], N limbs each),
+// and fe_mul_k<F, 1, false> of the same pairs.  -1 if a row's lanes N..15
+// are not zero.
+template <class F>
+int row_mul_check(const uint32_t* a, const uint32_t* b, uint32_t* out_row, uint32_t* out_ref) {
+    using W = WaveHost;
+    const RowConsts<W, F> K = row_consts<W, F>();
+    FeRow<W, F> x, y;
+    for (int i = 0; i < 64; ++i) {
+        const bool in = (i & 15) < F::N;
+        x.v.l[i] = in ? a[i] : 0;
+        y.v.l[i] = in ? b[i] : 0;
+    }
+    const FeRow<W, F> r = row_mul<W, F>(x, y, K);
+    int bad = 0;
+    for (int i = 0; i < 64; ++i) {
+        out_row[i] = r.v.l[i];
+        if ((i & 15) >= F::N && r.v.l[i]) bad = 1;
+    }
+    for (int q = 0; q < 4; ++q) {
+        Fe<F> fa, fb;
+        for (int j = 0; j < F::N; ++j) {
+            fa.v[j] = a[16 * q + j];
+            fb.v[j] = b[16 * q + j];
+        }
+        const Fe<F> m = fe_mul_k<F, 1, false>(fa, fb);
+        for (int j = 0; j < 16; ++j) out_ref[16 * q + j] = j < F::N ? m.v[j] : 0;
+    }
+    return bad ? -1 : 0;
+}
+
+template <class F>
+void raw_xyzz(const Xyzz<F>& p, uint32_t* out) {
+    for (int j = 0; j < F::N; ++j) {
+        out[j] = p.X.v[j];
+        out[F::N + j] = p.Y.v[j];
+        out[2 * F::N + j] = p.ZZ.v[j];
+        out[3 * F::N + j] = p.ZZZ.v[j];
+    }
+}
+
+// op 0: a + b, 1: 2a, by the row operations and by the lazy quad ones; raw
+// coordinates of both (4 N limbs each) and the canonical wire result of the
+// rows.  -1 if the rows disagree.
+template <class C>
+int row_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, const int* ka,
+           const int* kb, uint8_t* out, uint32_t* raw_row, uint32_t* raw_quad) {
+    using F = typename C::Fp;
+    using W = WaveHost;
+    const Xyzz<F> P = lazy_point<C>(a, za, ka), Q = lazy_point<C>(b, zb, kb);
+    const RowConsts<W, F> K = row_consts<W, F>();
+    const XyzzRow<W, F> rp = row_from_xyzz<W, F>(P), rq = row_from_xyzz<W, F>(Q);
+    const XyzzRow<W, F> r = op == 0 ? row_add<W, F>(rp, rq, K) : row_dbl<W, F>(rp, K);
+    const bool ok = row_consistent<F>(r.X) && row_consistent<F>(r.Y) && row_consistent<F>(r.ZZ) &&
+                    row_consistent<F>(r.ZZZ);
+    const Xyzz<F> rr = row_to_xyzz<W, F>(r);
+    Xyzz<F> qq;
+    run_quad<F>([&] { return op == 0 ? xyzz_add_quad<F, MulCallLazy, HostQuad>(P, Q)
+                                     : xyzz_dbl_quad<F, MulCallLazy, HostQuad>(P); }, qq);
+    raw_xyzz<F>(rr, raw_row);
+    raw_xyzz<F>(qq, raw_quad);
+    write_wire<C>(xyzz_canon_acc<F>(rr), out);
+    return ok ? 0 : -1;
+}
+
+// The window combine by rows: nwin window sums (wire points raised to lazy
+// forms by z and k), Horner from the top with c doublings per window.
+template <class C>
+int row_combine(const uint8_t* wins, const uint8_t* zs, const int* ks, int nwin, int c, uint8_t* out) {
+    using F = typename C::Fp;
+    using W = WaveHost;
+    const RowConsts<W, F> K = row_consts<W, F>();
+    auto win = [&](int w) {
+        return row_from_xyzz<W, F>(lazy_point<C>(wins + 2 * C::FB * w, zs + C::FB * w, ks + 4 * w));
+    };
+    XyzzRow<W, F> r = win(nwin - 1);
+    for (int w = nwin - 2; w >= 0; --w) {
+        if (!row_pt_is_inf<W, F>(r))
+            for (int i = 0; i < c; ++i) r = row_dbl<W, F>(r, K);
+        r = row_add<W, F>(r, win(w), K);
+    }
+    const bool ok = row_consistent<F>(r.X) && row_consistent<F>(r.Y) && row_consistent<F>(r.ZZ) &&
+                    row_consistent<F>(r.ZZZ);
+    write_wire<C>(xyzz_canon_acc<F>(row_to_xyzz<W, F>(r)), out);
+    return ok ? 0 : -1;
 }
 
 // ---------------------------------------------------------------- serving
@@ -587,6 +753,26 @@ int hc_quad_tree(int curve, int lazy, const uint8_t* pts, const uint8_t* zs, int
     if (curve == 2) return quad_tree<CurveBLS12381>(lazy, pts, zs, n, out);
     if (curve == 1) return quad_tree<CurveBN254>(lazy, pts, zs, n, out);
     return quad_tree<CurveSecp256k1>(lazy, pts, zs, n, out);
+}
+
+// curve: 0 secp256k1, 1 BN254, 2 BLS12-381 (their Fp)
+int hc_row_mul(int curve, const uint32_t* a, const uint32_t* b, uint32_t* out_row, uint32_t* out_ref) {
+    if (curve == 2) return row_mul_check<BlsFp>(a, b, out_row, out_ref);
+    if (curve == 1) return row_mul_check<BnFp>(a, b, out_row, out_ref);
+    return row_mul_check<SecpFp>(a, b, out_row, out_ref);
+}
+
+int hc_row_op(int curve, int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb,
+              const int* ka, const int* kb, uint8_t* out, uint32_t* raw_row, uint32_t* raw_quad) {
+    if (curve == 2) return row_op<CurveBLS12381>(op, a, b, za, zb, ka, kb, out, raw_row, raw_quad);
+    if (curve == 1) return row_op<CurveBN254>(op, a, b, za, zb, ka, kb, out, raw_row, raw_quad);
+    return row_op<CurveSecp256k1>(op, a, b, za, zb, ka, kb, out, raw_row, raw_quad);
+}
+
+int hc_row_combine(int curve, const uint8_t* wins, const uint8_t* zs, const int* ks, int nwin, int c, uint8_t* out) {
+    if (curve == 2) return row_combine<CurveBLS12381>(wins, zs, ks, nwin, c, out);
+    if (curve == 1) return row_combine<CurveBN254>(wins, zs, ks, nwin, c, out);
+    return row_combine<CurveSecp256k1>(wins, zs, ks, nwin, c, out);
 }
 
 int hc_quad_dbl_chain(int curve, const uint8_t* a, const uint8_t* za, const int* ka, int count, uint8_t* out) {

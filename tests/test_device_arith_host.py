@@ -323,3 +323,94 @@ def test_quad_doubling_chains(hc, curve):
         w, z, k = _quad_args(curve, P, rnd, 1, extreme=True)
         assert hc.hc_quad_dbl_chain(curve, w, z, k, count, out) == 0
         assert out.raw == M.to_wire(curve, M.mul(curve, P, 1 << count)), (curve, count)
+
+
+# ---------------------------------------------------------------- rows
+ROW_N = {0: 10, 1: 9, 2: 14}
+
+
+def _limbs(v, n):
+    return [(v >> (29 * i)) & ((1 << 29) - 1) for i in range(n)]
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+def test_row_mul_matches_scan(hc, curve):
+    """The row-distributed Montgomery product (field_row.h, four products per
+    wave, limb j in lane j of a row) against the one-lane product scan
+    fe_mul_k<F, 1, false>, limb for limb, and against a b 2^(-29 N) mod p,
+    below 2p: operands 0, 1, p - 1, p, 2p - 1, 8p - 1 (the lazy bound) and
+    random ones below 8p."""
+    p, n = M.CURVES[curve]["p"], ROW_N[curve]
+    rinv = pow(1 << (29 * n), -1, p)
+    rnd = random.Random(4000 + curve)
+    special = [0, 1, p - 1, p, 2 * p - 1, 8 * p - 1, (1 << 29) - 1, p >> 29 << 29]
+    vals = special + [rnd.randrange(8 * p) for _ in range(200)]
+    A, B = (ctypes.c_uint32 * 64)(), (ctypes.c_uint32 * 64)()
+    out, ref = (ctypes.c_uint32 * 64)(), (ctypes.c_uint32 * 64)()
+    for t in range(0, len(vals), 2):
+        pairs = [(vals[(t + q) % len(vals)], vals[(t * 5 + 3 * q + 1) % len(vals)]) for q in range(4)]
+        for q, (a, b) in enumerate(pairs):
+            la, lb = _limbs(a, n), _limbs(b, n)
+            for j in range(16):
+                A[16 * q + j] = la[j] if j < n else 0
+                B[16 * q + j] = lb[j] if j < n else 0
+        assert hc.hc_row_mul(curve, A, B, out, ref) == 0, "lanes N..15 not zero"
+        for q, (a, b) in enumerate(pairs):
+            got = list(out[16 * q:16 * q + n])
+            assert got == list(ref[16 * q:16 * q + n]), (curve, a, b)
+            v = sum(x << (29 * i) for i, x in enumerate(got))
+            assert v % p == a * b * rinv % p and v < 2 * p, (curve, a, b)
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+def test_row_ops_match_quad(hc, curve):
+    """Row-distributed a + b and 2a (field_row.h) against the lazy quad
+    operations, coordinate limb for coordinate limb, and against the textbook
+    result: random points, a = b through the addition, a = -b, identities,
+    coordinates at random and at the extreme lazy offsets."""
+    pb = O.POINT_BYTES[curve]
+    n = ROW_N[curve]
+    rnd = random.Random(5000 + curve)
+    out = ctypes.create_string_buffer(pb)
+    raw_r, raw_q = (ctypes.c_uint32 * (4 * n))(), (ctypes.c_uint32 * (4 * n))()
+    pts = _rand_points(curve, 24, rnd)
+    cases = [(pts[i], pts[i + 1]) for i in range(0, 16, 2)]
+    cases += [(pts[16], pts[16]), (pts[17], M.neg(curve, pts[17])), (None, pts[18]), (pts[19], None), (None, None)]
+    cases += [(pts[20], M.add(curve, pts[20], pts[20]))]
+    for i, (A, B) in enumerate(cases):
+        for extreme in (False, True):
+            wa, za, ka = _quad_args(curve, A, rnd, 1, extreme)
+            wb, zb, kb = _quad_args(curve, B, rnd, 1, extreme)
+            for op, want in ((0, M.add(curve, A, B)), (1, M.add(curve, A, A))):
+                assert hc.hc_row_op(curve, op, wa, wb, za, zb, ka, kb, out, raw_r, raw_q) == 0, "rows disagree"
+                assert list(raw_r) == list(raw_q), (curve, i, extreme, op)
+                assert out.raw == (M.to_wire(curve, want) if want is not None else bytes(pb)), (curve, i, op)
+
+
+@pytest.mark.parametrize("curve", [0, 1, 2])
+@pytest.mark.parametrize("nwin,c", [(8, 16), (3, 5), (15, 9)])
+def test_row_combine(hc, curve, nwin, c):
+    """The window combine by rows, sum_w 2^(c w) S_w by Horner from the top,
+    over window sums at the extreme lazy offsets; cases with identity
+    windows (the top one included), a window equal to the running sum (the
+    addition's doubling case) and one cancelling it (the identity mid-chain)."""
+    G, r, p, fb = M.CURVES[curve]["g"], M.CURVES[curve]["r"], M.CURVES[curve]["p"], M.CURVES[curve]["fb"]
+    pb = O.POINT_BYTES[curve]
+    rnd = random.Random(6000 + 100 * curve + nwin)
+    out = ctypes.create_string_buffer(pb)
+    for trial in range(4):
+        ks = [rnd.randrange(r) for _ in range(nwin)]
+        if trial == 1:
+            ks[nwin - 1] = 0
+            ks[nwin // 2] = 0
+        if trial >= 2:
+            # running value before window w: sum_{v > w} 2^(c (v - w)) k_v
+            w = nwin // 2
+            run = sum(ks[v] << (c * (v - w)) for v in range(w + 1, nwin)) % r
+            ks[w] = run if trial == 2 else (r - run) % r
+        want = sum(k << (c * w) for w, k in enumerate(ks)) % r
+        wins = b"".join(M.to_wire(curve, M.mul(curve, G, k)) if k else bytes(2 * fb) for k in ks)
+        zs = b"".join(rnd.randrange(1, p).to_bytes(fb, "little") for _ in range(nwin))
+        kk = (ctypes.c_int * (4 * nwin))(*(LAZY_MAX * nwin))
+        assert hc.hc_row_combine(curve, wins, zs, kk, nwin, c, out) == 0, "rows disagree"
+        assert out.raw == (M.to_wire(curve, M.mul(curve, G, want)) if want else bytes(pb)), (curve, trial)
