@@ -53,9 +53,10 @@ __host__ __device__ __forceinline__ void static_for(Fn&& fn) {
 }
 
 // One wave of 64 lanes in a workgroup of 64 threads.  DPP row controls:
-// 0x110 + s row_shr:s (lane j reads lane j - s of its row, 0 below the row),
-// 0x120 + s row_ror:s (lane j reads lane (j - s) mod 16), 0x150 + k
-// row_newbcast:k (every lane reads lane k of its row).
+// 0x110 + s row_shr:s (lane j reads lane j - s of its row; bound_ctrl makes
+// it 0 below the row), 0x120 + s row_ror:s (lane j reads lane (j - s) mod
+// 16), 0x150 + k row_newbcast:k (every lane reads lane k of its row).  Every
+// lane gets a source or the bound_ctrl zero, so the moves need no old value.
 struct WaveDev {
     using V = uint32_t;
     using V64 = uint64_t;
@@ -86,15 +87,15 @@ struct WaveDev {
     __device__ static V64 sel64(B b, V64 x, V64 y) { return b ? x : y; }
     template <int K>
     __device__ static V bcast(V x) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x150 + K, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150 + K, 0xF, 0xF, true);
     }
     template <int S>
     __device__ static V ror(V x) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x120 + S, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x120 + S, 0xF, 0xF, true);
     }
     template <int S>
     __device__ static V shr(V x) {  // zero fill
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + S, 0xF, 0xF, true);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x110 + S, 0xF, 0xF, true);
     }
     // the same lane of row K
     template <int K>
@@ -150,17 +151,19 @@ __host__ __device__ __forceinline__ RowConsts<W, F> row_consts() {
     return K;
 }
 
-// Limbs held as 64-bit column sums (lane j: weight 2^(29 j)) of a value below
-// 2^(29 N) -> normalised limbs.
+// Limbs held as 64-bit column sums (lane j: weight 2^(29 j), each below 2^63)
+// of a value below 2^(29 N) -> normalised limbs.  After one round a limb is
+// below 2^29 + 2^34 and after two below 2^29 + 2^6, so from the third round on
+// the carries are single bits; those rounds repeat until none is left.
 template <class W>
 __host__ __device__ __forceinline__ typename W::V row_carry64(typename W::V64 r) {
-    for (;;) {
-        const typename W::V64 c = W::shr64(r, kLimbBits);
-        r = W::add64(W::wide(W::and_(W::lo(r), kLimbMask)),
-                     W::mk64(W::template shr<1>(W::lo(c)), W::template shr<1>(W::hi(c))));
-        if (!W::any(W::or_(W::ne(W::hi(r), W::c(0)), W::gtc(W::lo(r), kLimbMask)))) break;
-    }
-    return W::lo(r);
+    using V = typename W::V;
+    const typename W::V64 c = W::shr64(r, kLimbBits);
+    r = W::add64(W::wide(W::and_(W::lo(r), kLimbMask)),
+                 W::mk64(W::template shr<1>(W::lo(c)), W::template shr<1>(W::hi(c))));
+    V t = W::add(W::and_(W::lo(r), kLimbMask), W::template shr<1>(W::lo(W::shr64(r, kLimbBits))));
+    while (W::any(W::gtc(t, kLimbMask))) t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::sra(t, kLimbBits)));
+    return t;
 }
 
 // Signed 32-bit limbs (|t| < 2^31) of a value in [0, 2^(29 N)) -> normalised
@@ -169,11 +172,9 @@ __host__ __device__ __forceinline__ typename W::V row_carry64(typename W::V64 r)
 // a value mod 2^(29 16), and the value itself is below 2^(29 N)).
 template <class W>
 __host__ __device__ __forceinline__ typename W::V row_carry32s(typename W::V t) {
-    for (;;) {
-        const typename W::V c = W::sra(t, kLimbBits);
-        t = W::add(W::and_(t, kLimbMask), W::template shr<1>(c));
-        if (!W::any(W::gtc(t, kLimbMask))) break;  // negative limbs read as large
-    }
+    t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::sra(t, kLimbBits)));
+    while (W::any(W::gtc(t, kLimbMask)))  // negative limbs read as large
+        t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::sra(t, kLimbBits)));
     return t;
 }
 
@@ -194,7 +195,8 @@ __host__ __device__ __forceinline__ FeRow<W, F> row_mul(const FeRow<W, F>& a, co
         if constexpr (i > 0) rb = W::template ror<1>(rb);  // lane j: b_((j - i) mod 16)
         const V x = W::template bcast<i>(a.v);
         lo = W::mad(x, W::sel(W::ltc(j, i), W::c(0), rb), lo);
-        if constexpr (i > 0) hi = W::mad(x, W::sel(W::ltc(j, i), rb, W::c(0)), hi);
+        // lanes j < i hold b_(j - i + 16), zero unless j - i + 16 < N
+        if constexpr (i > 16 - N) hi = W::mad(x, W::sel(W::ltc(j, i), rb, W::c(0)), hi);
     });
     V64 carry = W::wide(W::c(0));
     static_for<0, N>([&](auto I) {
@@ -203,7 +205,7 @@ __host__ __device__ __forceinline__ FeRow<W, F> row_mul(const FeRow<W, F>& a, co
         const V m = W::and_(W::mulc(W::lo(col), F::PINV), kLimbMask);
         carry = W::shr64(W::madc(m, F::P[0], col), kLimbBits);
         lo = W::mad(m, K.plo[k], lo);
-        hi = W::mad(m, K.phi[k], hi);
+        if constexpr (k > 16 - N) hi = W::mad(m, K.phi[k], hi);  // else phi[k] is 0 in every lane
     });
     // column c < 16 sits in lane c (lo), column c >= 16 in lane c - 16 (hi);
     // lane i takes column N + i (a rotation by 16 - N), lane 0 the last carry
