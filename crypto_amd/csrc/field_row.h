@@ -41,6 +41,7 @@
 #include <type_traits>
 
 #include "curve.h"
+#include "curve_ed.h"
 
 namespace kz {
 
@@ -70,6 +71,7 @@ struct WaveDev {
     __device__ static V shl(V a, int s) { return a << s; }
     __device__ static V mulc(V a, uint32_t k) { return a * k; }
     __device__ static V sra(V a, int s) { return (uint32_t)((int32_t)a >> s); }
+    __device__ static V shr32(V a, int s) { return a >> s; }
     __device__ static V64 wide(V a) { return a; }
     __device__ static V lo(V64 a) { return (uint32_t)a; }
     __device__ static V hi(V64 a) { return (uint32_t)(a >> 32); }
@@ -83,6 +85,8 @@ struct WaveDev {
     __device__ static B gtc(V a, uint32_t k) { return a > k; }
     __device__ static B ne(V a, V b) { return a != b; }
     __device__ static B or_(B a, B b) { return a || b; }
+    __device__ static B ge(V a, V b) { return a >= b; }
+    __device__ static B gt(V a, V b) { return a > b; }
     __device__ static V sel(B b, V x, V y) { return b ? x : y; }
     __device__ static V64 sel64(B b, V64 x, V64 y) { return b ? x : y; }
     template <int K>
@@ -103,6 +107,9 @@ struct WaveDev {
         return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((K << 4) | (threadIdx.x & 15u)) << 2), (int)x);
     }
     __device__ static bool any(B b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
+    __device__ static uint64_t ballot(B b) { return __builtin_amdgcn_ballot_w64(b); }
+    // the 16 bits of a wave mask that belong to this lane's row
+    __device__ static V rowbits(uint64_t m) { return (uint32_t)(m >> (threadIdx.x & 48u)) & 0xFFFFu; }
     __device__ static uint32_t readlane(V x, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)x, l); }
 };
 
@@ -120,7 +127,7 @@ struct XyzzRow {
 // of p per step k (lane j: p_(j-k) for j > k, p_(j-k+16) for j < k, else 0).
 template <class W, class F>
 struct RowConsts {
-    typename W::V p, kp2, kp4, kp6, kp8;
+    typename W::V p, kp2, kp4, kp5, kp6, kp8;
     typename W::V plo[F::N], phi[F::N];
 };
 
@@ -139,6 +146,7 @@ __host__ __device__ __forceinline__ RowConsts<W, F> row_consts() {
     K.p = row_limbs_kp<W, F, 1>(j);
     K.kp2 = row_limbs_kp<W, F, 2>(j);
     K.kp4 = row_limbs_kp<W, F, 4>(j);
+    K.kp5 = row_limbs_kp<W, F, 5>(j);
     K.kp6 = row_limbs_kp<W, F, 6>(j);
     K.kp8 = row_limbs_kp<W, F, 8>(j);
     V rp = K.p;
@@ -175,6 +183,14 @@ __host__ __device__ __forceinline__ typename W::V row_carry32s(typename W::V t) 
     t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::sra(t, kLimbBits)));
     while (W::any(W::gtc(t, kLimbMask)))  // negative limbs read as large
         t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::sra(t, kLimbBits)));
+    return t;
+}
+
+// Non-negative 32-bit limbs (below 2^32) -> normalised limbs.
+template <class W>
+__host__ __device__ __forceinline__ typename W::V row_carry32u(typename W::V t) {
+    t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::shr32(t, kLimbBits)));
+    while (W::any(W::gtc(t, kLimbMask))) t = W::add(W::and_(t, kLimbMask), W::template shr<1>(W::shr32(t, kLimbBits)));
     return t;
 }
 
@@ -325,6 +341,91 @@ __host__ __device__ XyzzRow<W, F> row_add(const XyzzRow<W, F>& a, const XyzzRow<
     return r;
 }
 
+// a mod p for a normalised a < 2p: a - p when a >= p.  Per row, a >= p when
+// the highest limb where a and p differ is larger in a (or none differs):
+// with G, L the row's lanes where a's limb is above / below p's, a >= p
+// exactly when G >= L as 16-bit integers (the sets are disjoint).
+template <class W, class F>
+__host__ __device__ __forceinline__ FeRow<W, F> row_reduce_once(const FeRow<W, F>& a, const RowConsts<W, F>& K) {
+    const typename W::V G = W::rowbits(W::ballot(W::gt(a.v, K.p)));
+    const typename W::V L = W::rowbits(W::ballot(W::gt(K.p, a.v)));
+    return FeRow<W, F>{row_carry32s<W>(W::sel(W::ge(G, L), W::sub(a.v, K.p), a.v))};
+}
+
+// a b 2^(-29 N) mod p, canonical: fe_mul_k<F, K, true>'s value and limbs.
+template <class W, class F>
+__host__ __device__ __forceinline__ FeRow<W, F> row_mul_canon(const FeRow<W, F>& a, const FeRow<W, F>& b,
+                                                              const RowConsts<W, F>& K) {
+    return row_reduce_once<W, F>(row_mul<W, F>(a, b, K), K);
+}
+
+// a + b, no reduction (fe_add_nored)
+template <class W, class F>
+__host__ __device__ __forceinline__ FeRow<W, F> row_add_nored(const FeRow<W, F>& a, const FeRow<W, F>& b) {
+    return FeRow<W, F>{row_carry32u<W>(W::add(a.v, b.v))};
+}
+// a + k p - b - c (fe_kp_sub2)
+template <class W, class F>
+__host__ __device__ __forceinline__ FeRow<W, F> row_kp_sub2(const FeRow<W, F>& a, typename W::V kp,
+                                                            const FeRow<W, F>& b, const FeRow<W, F>& c) {
+    return FeRow<W, F>{row_carry32s<W>(W::sub(W::sub(W::add(a.v, kp), b.v), c.v))};
+}
+template <class W, class F>
+__host__ __device__ __forceinline__ FeRow<W, F> row_small_mul5(const FeRow<W, F>& a) {
+    return FeRow<W, F>{row_carry32u<W>(W::add(W::shl(a.v, 2), a.v))};
+}
+
+// Extended twisted Edwards points (Banderwagon), by rows.
+template <class W, class F>
+struct EdRow {
+    FeRow<W, F> X, Y, T, Z;
+};
+
+template <class W, class F>
+__host__ __device__ __forceinline__ bool row_ed_is_identity(const EdRow<W, F>& p) {
+    return row_is_zero<W, F>(p.X) && !W::any(W::ne(p.Y.v, p.Z.v));
+}
+
+// p + q (ed_add_quad_lazy with a canonical first level): three product levels.
+template <class W, class F>
+__host__ __device__ EdRow<W, F> row_ed_add(const EdRow<W, F>& p, const EdRow<W, F>& q, const FeRow<W, F>& d,
+                                           const RowConsts<W, F>& K) {
+    using E = FeRow<W, F>;
+    const typename W::V l = W::row();
+    E m = row_mul_canon<W, F>(row_pick<W, F>(l, p.X, p.Y, p.T, p.Z), row_pick<W, F>(l, q.X, q.Y, q.T, q.Z), K);
+    const E A = row_read<0, W, F>(m), B = row_read<1, W, F>(m), TT = row_read<2, W, F>(m), D = row_read<3, W, F>(m);
+    const E S1 = row_add_nored<W, F>(p.X, p.Y), S2 = row_add_nored<W, F>(q.X, q.Y);
+    m = row_mul<W, F>(row_pick<W, F>(l, TT, S1, TT, S1), row_pick<W, F>(l, d, S2, d, S2), K);
+    const E C = row_read<0, W, F>(m);
+    const E Ev = row_kp_sub2<W, F>(row_read<1, W, F>(m), K.kp2, A, B);
+    const E Fv = row_sub_kp<W, F>(D, K.kp2, C), G = row_add_nored<W, F>(D, C);
+    const E H = row_add_nored<W, F>(B, row_small_mul5<W, F>(A));
+    m = row_mul<W, F>(row_pick<W, F>(l, Ev, G, Ev, Fv), row_pick<W, F>(l, Fv, H, H, G), K);
+    return EdRow<W, F>{row_read<0, W, F>(m), row_read<1, W, F>(m), row_read<2, W, F>(m), row_read<3, W, F>(m)};
+}
+
+// 2p (ed_dbl_quad_lazy with a canonical first level): two product levels.
+template <class W, class F>
+__host__ __device__ EdRow<W, F> row_ed_dbl(const EdRow<W, F>& p, const RowConsts<W, F>& K) {
+    using E = FeRow<W, F>;
+    const typename W::V l = W::row();
+    const E S = row_add_nored<W, F>(p.X, p.Y);
+    E m = row_mul_canon<W, F>(row_pick<W, F>(l, p.X, p.Y, p.Z, S), row_pick<W, F>(l, p.X, p.Y, p.Z, S), K);
+    const E A = row_read<0, W, F>(m), B = row_read<1, W, F>(m);
+    const E C = row_small_mul2<W, F>(row_read<2, W, F>(m));
+    const E Ev = row_kp_sub2<W, F>(row_read<3, W, F>(m), K.kp2, A, B);
+    const E D = row_sub_kp<W, F>(E{W::c(0)}, K.kp5, row_small_mul5<W, F>(A));
+    const E G = row_add_nored<W, F>(D, B), Fv = row_sub_kp<W, F>(G, K.kp2, C), H = row_sub_kp<W, F>(D, K.p, B);
+    m = row_mul<W, F>(row_pick<W, F>(l, Ev, G, Ev, Fv), row_pick<W, F>(l, Fv, H, H, G), K);
+    return EdRow<W, F>{row_read<0, W, F>(m), row_read<1, W, F>(m), row_read<2, W, F>(m), row_read<3, W, F>(m)};
+}
+
+template <class W, class F>
+__host__ __device__ __forceinline__ EdRow<W, F> row_from_ed(const EdExt<F>& p) {
+    return EdRow<W, F>{row_from_fe<W, F>(p.X), row_from_fe<W, F>(p.Y), row_from_fe<W, F>(p.T),
+                       row_from_fe<W, F>(p.Z)};
+}
+
 // Standard (one-lane) form <-> rows.
 template <class W, class F>
 __host__ __device__ __forceinline__ FeRow<W, F> row_from_fe(const Fe<F>& a) {
@@ -351,6 +452,15 @@ __host__ __device__ __forceinline__ Xyzz<F> row_to_xyzz(const XyzzRow<W, F>& p) 
     r.Y = row_to_fe<W, F>(p.Y);
     r.ZZ = row_to_fe<W, F>(p.ZZ);
     r.ZZZ = row_to_fe<W, F>(p.ZZZ);
+    return r;
+}
+template <class W, class F>
+__host__ __device__ __forceinline__ EdExt<F> row_to_ed(const EdRow<W, F>& p) {
+    EdExt<F> r;
+    r.X = row_to_fe<W, F>(p.X);
+    r.Y = row_to_fe<W, F>(p.Y);
+    r.T = row_to_fe<W, F>(p.T);
+    r.Z = row_to_fe<W, F>(p.Z);
     return r;
 }
 

@@ -137,10 +137,4 @@ void launch_reduce_level_bw(unsigned grid, hipStream_t s, const uint32_t* inA, c
                        uni_parts, lm, outA, outB);
 }
 
-void launch_combine_bw(hipStream_t s, const uint32_t* winres, const MsmJob* jobs, const MsmJob& one, uint32_t njobs,
-                       uint8_t* out) {
-    hipLaunchKernelGGL(k_combine<Bw>, dim3(ceil_div((size_t)njobs * 4, 64)), dim3(64), 0, s, winres, jobs, one,
-                       njobs, out);
-}
-
 }  // namespace kz

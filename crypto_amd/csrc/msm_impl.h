@@ -16,11 +16,12 @@
 //                      bucket sums, then (sum, weighted sum) by running sums
 //   k_reduce_level     log-depth levels (LDS suffix scan + tree) down to one
 //                      sum per window
-//   k_combine          Horner over windows, affine normalisation, compression
+//   (window combine: k_combine_row, msm_combine_row.hip)
 //
 // The accumulation inlines its field products (throughput-bound, one compact
-// loop).  The latency-bound reduction and combine kernels use quad-cooperative
-// point operations (curve_quad.h); the reduction calls one out-of-line field
+// loop).  The reduction kernels use quad-cooperative point operations
+// (curve_quad.h), the combine row-distributed ones (field_row.h); the
+// reduction calls one out-of-line field
 // multiply (MulCall) so its code stays inside the instruction cache, and keeps
 // few point-operation sites with `#pragma unroll 1` loops.
 #pragma once
@@ -403,31 +404,6 @@ __device__ void write_output(const EdExt<typename C::Fp>& r, uint8_t* __restrict
     fe_to_be_bytes<F>(y, out + C::FB, C::FB);
     const bool largest = fe_gt<F>(y, fe_neg<F>(y));  // y > q - y, i.e. y > (q - 1) / 2
     fe_to_be_bytes<F>(largest ? x : fe_neg<F>(x), out + 2 * C::FB, C::FB);
-}
-
-// Horner over the window sums, MSW first: r = 2^c r + S_w.  One quad of lanes
-// per MSM (quad-cooperative point operations: the chain of ~256 doublings is
-// the critical path of the whole MSM).  jobs = null: a single MSM described
-// by `one`; else njobs MSMs, job j's record at out + j * 3 * FB.
-template <class C>
-__global__ void __launch_bounds__(64) k_combine(const uint32_t* __restrict__ winres,
-                                                const MsmJob* __restrict__ jobs, MsmJob one, uint32_t njobs,
-                                                uint8_t* __restrict__ out) {
-    using P = PointOps<typename C::Fp>;
-    const uint32_t job = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
-    if (job >= njobs) return;
-    const MsmJob jd = jobs ? jobs[job] : one;
-    const int c = (int)jd.c;
-    typename P::Pt r = P::load(winres, jd.win_base + jd.nwin - 1);
-#pragma unroll 1
-    for (int w = (int)jd.nwin - 2; w >= 0; --w) {
-        if (!P::is_inf(r)) {
-#pragma unroll 1
-            for (int i = 0; i < c; ++i) r = P::template dbl_q<MulLazy>(r);
-        }
-        r = P::template add_q<MulLazy>(r, P::load(winres, jd.win_base + (uint32_t)w));
-    }
-    if (quad_lane() == 0) write_output<C>(P::canon(r), out + (size_t)job * 3 * C::FB);
 }
 
 // ------------------------------------------------------------ window tables

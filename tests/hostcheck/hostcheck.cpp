@@ -298,6 +298,15 @@ struct WaveHost {
     static B gtc(V a, uint32_t k) { return map<B>([&](int i) { return a.l[i] > k; }); }
     static B ne(V a, V b) { return map<B>([&](int i) { return a.l[i] != b.l[i]; }); }
     static B or_(B a, B b) { return map<B>([&](int i) { return a.l[i] || b.l[i]; }); }
+    static B ge(V a, V b) { return map<B>([&](int i) { return a.l[i] >= b.l[i]; }); }
+    static B gt(V a, V b) { return map<B>([&](int i) { return a.l[i] > b.l[i]; }); }
+    static V shr32(V a, int s) { return map<V>([&](int i) { return a.l[i] >> s; }); }
+    static uint64_t ballot(B b) {
+        uint64_t m = 0;
+        for (int i = 0; i < 64; ++i) m |= (uint64_t)b.l[i] << i;
+        return m;
+    }
+    static V rowbits(uint64_t m) { return map<V>([&](int i) { return (uint32_t)(m >> (i & 48)) & 0xFFFFu; }); }
     static V sel(B b, V x, V y) { return map<V>([&](int i) { return b.l[i] ? x.l[i] : y.l[i]; }); }
     static V64 sel64(B b, V64 x, V64 y) { return map<V64>([&](int i) { return b.l[i] ? x.l[i] : y.l[i]; }); }
     template <int K>
@@ -674,9 +683,44 @@ int bw_is_square(const uint8_t* a_be) { return fe_is_square_nonzero<BwF>(bw_be(a
 
 }  // namespace
 
+// The row-distributed Edwards operations (field_row.h) in the chain of
+// bw_quad_lazy_chain: every step's coordinates against the lazy quad forms,
+// limb for limb; 0 when they all agree, then the canonical result in out.
+int bw_row_chain(const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, int rounds,
+                 uint8_t* out) {
+    using W = WaveHost;
+    EdExt<BwF> P = bw_raise(bw_in(a, za));
+    const EdExt<BwF> Q = bw_raise(bw_in(b, zb));
+    const RowConsts<W, BwF> K = row_consts<W, BwF>();
+    const FeRow<W, BwF> d = row_from_fe<W, BwF>(fe_const<BwF>(EdConst::D));
+    EdRow<W, BwF> R = row_from_ed<W, BwF>(P);
+    const EdRow<W, BwF> RQ = row_from_ed<W, BwF>(Q);
+    auto same = [](const EdExt<BwF>& x, const EdRow<W, BwF>& y) {
+        const EdExt<BwF> z = row_to_ed<W, BwF>(y);
+        return std::memcmp(&x, &z, sizeof x) == 0 && row_consistent<BwF>(y.X) && row_consistent<BwF>(y.Y) &&
+               row_consistent<BwF>(y.T) && row_consistent<BwF>(y.Z);
+    };
+    for (int i = 0; i < rounds; ++i) {
+        EdExt<BwF> r;
+        run_quad<BwF>([&] { return ed_dbl_quad_lazy<BwF, MulCall, MulCallLazy, HostQuad>(P); }, r);
+        P = r;
+        R = row_ed_dbl<W, BwF>(R, K);
+        if (!same(P, R)) return -1;
+        run_quad<BwF>([&] { return ed_add_quad_lazy<BwF, MulCall, MulCallLazy, HostQuad>(P, Q); }, r);
+        P = r;
+        R = row_ed_add<W, BwF>(R, RQ, d, K);
+        if (!same(P, R)) return -2;
+    }
+    return bw_out(ed_canon<BwF>(row_to_ed<W, BwF>(R)), out);
+}
+
 extern "C" {
 int hc_bw_op(int op, const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, uint8_t* out) {
     return bw_op(op, a, b, za, zb, out);
+}
+int hc_bw_row_chain(const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, int rounds,
+                    uint8_t* out) {
+    return bw_row_chain(a, b, za, zb, rounds, out);
 }
 int hc_bw_is_square(const uint8_t* a_be) { return bw_is_square(a_be); }
 int hc_bw_quad_lazy_chain(const uint8_t* a, const uint8_t* b, const uint8_t* za, const uint8_t* zb, int rounds,

@@ -149,3 +149,24 @@ def test_lazy_quad_chains(hc, inline_mul):
         for _ in range(rounds):
             want = O.bw_add(O.bw_dbl(want), b)
         assert out.raw == want
+
+
+def test_row_chains_match_quad(hc):
+    """The row-distributed Edwards doubling and addition (field_row.h, the
+    window combine's form) in the chains of test_lazy_quad_chains: after every
+    step the coordinates equal the lazy quad forms' limb for limb, in every
+    row; the end point against the oracle."""
+    rnd = random.Random(93)
+    g = O.bw_generator()
+    cases = [(O.bw_mul_gen(rnd.randrange(1, R_BW)), O.bw_mul_gen(rnd.randrange(1, R_BW)), 24),
+             (g, IDENTITY, 8), (g, TWO_TORSION, 8), (IDENTITY, g, 6), (other_rep(g), neg(g), 5),
+             (TWO_TORSION, TWO_TORSION, 4)]
+    for a, b, rounds in cases:
+        za = rnd.randrange(1, Q).to_bytes(32, "little")
+        zb = rnd.randrange(1, Q).to_bytes(32, "little")
+        out = ctypes.create_string_buffer(64)
+        assert hc.hc_bw_row_chain(a, b, za, zb, rounds, out) == 1
+        want = a
+        for _ in range(rounds):
+            want = O.bw_add(O.bw_dbl(want), b)
+        assert out.raw == want
