@@ -63,11 +63,12 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
     const int field = curve == 2 ? 0 : 1;
     for (size_t i = 0; i < count; ++i)
         if (i == 0 || n[i] != n[i - 1]) ntt_prepare(field, log2_exact(n[i]), true, s);
-    // A lone job over table-free SRS points runs the single-MSM pipeline with
-    // the GLV split (SRS points lie in G1): half the windows of the batched
-    // plain form, the same commitment.  Not captured: that pipeline may still
-    // grow its workspace on the first run of a size.
-    if (count == 1 && srs->table_c.empty() && n[0] >= 4096) {
+    // A lone job over table-free SRS points known to lie in G1 (generated, or
+    // checked at creation) runs the single-MSM pipeline with the GLV split:
+    // half the windows of the batched plain form, the same commitment.  Not
+    // captured: that pipeline may still grow its workspace on the first run
+    // of a size.
+    if (count == 1 && srs->table_c.empty() && srs->g1 && n[0] >= 4096) {
         const uint32_t* pts = srs->pts.as<uint32_t>();
         const uint8_t* inf = srs->inf.as<uint8_t>();
         device_fill_u32(reinterpret_cast<uint32_t*>(err), 0, 1, s);

@@ -75,6 +75,49 @@ def test_batch_and_msm(ca, curve):
     assert a.msm(sc) == O.msm(curve, sc, a.export(0, 300))
 
 
+def _non_g1_point(curve):
+    """A point of E(Fp) outside G1 (BLS12-381 has a large cofactor)."""
+    p, b = M.CURVES[curve]["p"], M.CURVES[curve]["b"]
+    for x in range(1, 1000):
+        y = M.sqrt(curve, (x * x * x + b) % p)
+        if y is not None and M.mul(curve, (x, y), M.CURVES[curve]["r"]) is not None:
+            return (x, y)
+    raise AssertionError("no point found")
+
+
+@pytest.mark.parametrize("curve", [1, 2])
+def test_batch_lone_job(ca, curve):
+    """A batch commit of a single polynomial of at least 4096 evaluations over
+    a table-free SRS in G1 runs the single-MSM pipeline with the GLV split:
+    the same commitment as the single commit and the closed form, at 4096 and
+    2^13 evaluations; a lone smaller job keeps the batched form."""
+    tau = int.from_bytes(O.gen_scalars(curve, 16, 1), "little")
+    srs = ca.KzgSrs.generate(curve, tau, 1 << 13)
+    for n in (1024, 4096, 1 << 13):
+        ev = _evals(curve, n, 300 + n)
+        print(f"kzg lone batch job curve={curve} n={n}", flush=True)
+        want = O.kzg_commit_closed_form(curve, tau.to_bytes(32, "little"), ev)
+        assert ca.kzg_commit_batch([srs], [ev]) == [want]
+        assert srs.commit(ev) == want
+
+
+def test_batch_lone_job_srs_outside_g1(ca):
+    """kzg_srs_create accepts on-curve points outside G1 and then keeps MSMs
+    against that SRS off the GLV split.  A lone batch job of 4096 evaluations
+    (over a G1 SRS the GLV pipeline) equals the single commit and the oracle's
+    iNTT + MSM over the same points, the non-G1 point among them."""
+    curve, n = 2, 4096
+    pb = O.POINT_BYTES[curve]
+    pts = bytearray(O.gen_points_progression(curve, 0x5151, 0x77, n))
+    pts[7 * pb:8 * pb] = M.to_wire(curve, _non_g1_point(curve))
+    srs = ca.KzgSrs.from_points(curve, bytes(pts))
+    ev = _evals(curve, n, 4242)
+    want = O.kzg_commit(curve, bytes(pts), ev)
+    print("kzg lone batch job over a non-G1 SRS", flush=True)
+    assert srs.commit(ev) == want
+    assert ca.kzg_commit_batch([srs], [ev]) == [want]
+
+
 @pytest.mark.parametrize("curve", [1, 2])
 def test_commit_rejects_bad_input(ca, curve):
     srs = ca.KzgSrs.generate(curve, 3, 64)
