@@ -63,20 +63,16 @@ void commit_batch_launch(CommitLane& lane, const kzg_srs* srs, const std::vector
     const int field = curve == 2 ? 0 : 1;
     for (size_t i = 0; i < count; ++i)
         if (i == 0 || n[i] != n[i - 1]) ntt_prepare(field, log2_exact(n[i]), true, s);
-    // A lone job over table-free SRS points known to lie in G1 (generated, or
-    // checked at creation) runs the single-MSM pipeline with the GLV split:
-    // half the windows of the batched plain form, the same commitment.  Not
-    // captured: that pipeline may still grow its workspace on the first run
-    // of a size.
-    if (count == 1 && srs->table_c.empty() && srs->g1 && n[0] >= 4096) {
-        const uint32_t* pts = srs->pts.as<uint32_t>();
-        const uint8_t* inf = srs->inf.as<uint8_t>();
+    // A lone job of at least 4096 evaluations runs the single commit's MSM
+    // (srs_msm: a window table of the SRS when it has one, else the GLV split
+    // when its points are known to lie in G1, else plain windows), which beat
+    // the batched form for one job: the same commitment.  Not captured: that
+    // pipeline may still grow its workspace on the first run of a size.
+    if (count == 1 && n[0] >= 4096) {
         device_fill_u32(reinterpret_cast<uint32_t*>(err), 0, 1, s);
         KZ_HIP(hipMemcpyAsync(ev, lane.pin_in.ptr, 32 * total, hipMemcpyHostToDevice, s));
         ntt_device_batch(field, log2_exact(n[0]), true, ev, co, tmp, 1, err, s);
-        MsmParams prm;
-        prm.g1 = true;
-        msm_device(curve, pts, inf, co, n[0], prm, lane.ws, s, out);
+        srs_msm(srs, co, n[0], 0, lane.ws, s, out);
         KZ_HIP(hipMemcpyAsync(pin_out, out, count * rec + sizeof(int), hipMemcpyDeviceToHost, s));
         ++lane.direct_launches;
         KZ_HIP(hipEventRecord(lane.done, s));
