@@ -106,14 +106,19 @@ k_check_g1(const uint32_t* __restrict__ pts, const uint8_t* __restrict__ inf, ui
         }
         // lambda P by signed-digit double-and-add over lambda's NAF (top digit
         // +1 at bit 128): 18 additions instead of lambda's 48 set bits
-        Affine<F> na = a;
-        na.y = fe_neg<F>(a.y);
+        // (-P is formed at each negative digit: kept live, its 2N registers
+        // cost the kernel a wave per SIMD)
         Xyzz<F> acc = xyzz_from_affine<F>(a);
 #pragma unroll 1
         for (int bit = 127; bit >= 0; --bit) {
             acc = xyzz_dbl<F, MulCall>(acc);
-            if ((kLamNafPos[bit >> 5] >> (bit & 31)) & 1u) xyzz_madd<F, MulCall>(acc, a);
-            else if ((kLamNafNeg[bit >> 5] >> (bit & 31)) & 1u) xyzz_madd<F, MulCall>(acc, na);
+            if ((kLamNafPos[bit >> 5] >> (bit & 31)) & 1u) {
+                xyzz_madd<F, MulCall>(acc, a);
+            } else if ((kLamNafNeg[bit >> 5] >> (bit & 31)) & 1u) {
+                Affine<F> na = a;
+                na.y = fe_neg<F>(a.y);
+                xyzz_madd<F, MulCall>(acc, na);
+            }
         }
         const Fe<F> bx = fe_mul<F>(a.x, fe_const<F>(kBetaMont));
         ok = !xyzz_is_inf<F>(acc) && fe_eq<F>(acc.X, fe_mul<F>(bx, acc.ZZ)) && fe_eq<F>(acc.Y, fe_mul<F>(a.y, acc.ZZZ));
